@@ -1,0 +1,197 @@
+/*
+ * pml.h -- C ABI of the MI355X (gfx950) finite-difference + Parareal
+ * library (libpml.so).
+ *
+ * Every entry point returns a status code; on failure a thread-local
+ * message is available from pml_last_error().  All device buffers are owned
+ * by the caller (plain device pointers, e.g. from hipMalloc or a torch
+ * tensor's data_ptr()); the library keeps no pointer to them after a call
+ * returns.  Calls are ordered on the stream passed in (NULL = the null
+ * stream); entry points that produce host-visible results synchronise that
+ * stream before returning, the others do not.
+ *
+ * Reference interfaces replaced (paths relative to the PararealML source
+ * tree):
+ *   pml_fdm_solve          <- FDMOperator.solve time loop
+ *                             (pararealml/operators/fdm/fdm_operator.py:48-165)
+ *                             with ForwardEulerMethod / ExplicitMidpointMethod
+ *                             / RK4.integral
+ *                             (operators/fdm/numerical_integrator.py:42-132)
+ *   pml_stencil_first      <- ThreePointCentralDifferenceMethod._derivative
+ *                             (operators/fdm/numerical_differentiator.py:1012-1057)
+ *   pml_stencil_second     <- ThreePointCentralDifferenceMethod._second_derivative
+ *                             (same axis) + _add_halos_along_axis
+ *                             (numerical_differentiator.py:1059-1095, 1188-1242)
+ *   pml_jacobi_sweep       <- ThreePointCentralDifferenceMethod.
+ *                             _next_anti_laplacian_estimate (Cartesian) and the
+ *                             constraint application of anti_laplacian
+ *                             (numerical_differentiator.py:872-927, 1097-1186)
+ *   pml_parareal_update    <- the per-slice update U[j+1] = G[j] + corr[j] and
+ *                             the RMS update norms of PararealOperator
+ *                             (operators/parareal/parareal_operator.py:53-100,
+ *                             164-188)
+ *   pml_parareal_correction<- correction = F_end - G_end
+ *                             (parareal_operator.py:164)
+ */
+#ifndef PML_H
+#define PML_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define PML_OK 0
+#define PML_E_INVALID 1     /* bad argument   -> Python ValueError        */
+#define PML_E_UNSUPPORTED 2 /* not on device  -> Python NotImplementedError */
+#define PML_E_HIP 3         /* runtime error  -> Python RuntimeError       */
+
+/* ---- state precision ---------------------------------------------------- */
+#define PML_F32 0  /* FP32 state, FP32 arithmetic                           */
+#define PML_F64 1  /* FP64 state, FP64 arithmetic (parity with the oracle)  */
+#define PML_F32C 2 /* compensated FP32 state: hi and lo FP32 planes; stencil
+                      arithmetic in FP32, state update by error-free sums   */
+
+/* ---- equations (fused right-hand sides) -------------------------------- */
+#define PML_EQ_DIFFUSION 0            /* params: d                        */
+#define PML_EQ_CAHN_HILLIARD 1        /* params: d, gamma                 */
+#define PML_EQ_SHALLOW_WATER 2        /* params: h, b, v, f, g            */
+#define PML_EQ_CONVECTION_DIFFUSION 3 /* params: d, vel0, vel1            */
+#define PML_EQ_WAVE 4                 /* params: c^2                      */
+#define PML_EQ_BURGERS 5              /* params: 1/re                     */
+
+/* ---- integrators ------------------------------------------------------- */
+#define PML_INT_FE 0
+#define PML_INT_MIDPOINT 1
+#define PML_INT_RK4 2
+
+/*
+ * One grid of a (possibly ragged) batch.  State is stored field-major:
+ * field q of the grid occupies elements
+ *   [state_offset + q*n0*n1, state_offset + (q+1)*n0*n1)
+ * with axis 1 contiguous.  A 1-D problem is passed as n0 = 1 with the line
+ * along axis 1 and ndim = 1 in pml_fdm_solve.
+ *
+ * Boundary values (always FP64) for the grid start at bc_offset and hold,
+ * per field q, 8 consecutive edge arrays:
+ *   Dirichlet: axis0-lower[n1], axis0-upper[n1], axis1-lower[n0], axis1-upper[n0]
+ *   Neumann:   axis0-lower[n1], axis0-upper[n1], axis1-lower[n0], axis1-upper[n0]
+ * i.e. 4*(n0+n1) values per field.  NaN marks an unconstrained point.  The
+ * Dirichlet edges are the perimeter of the problem's full-grid y
+ * constraints, so corner precedence is resolved before the call.
+ */
+typedef struct {
+  int32_t n0;
+  int32_t n1;
+  double dx0;
+  double dx1;
+  int64_t state_offset;
+  int64_t bc_offset;
+} pml_grid;
+
+/* ---- library ----------------------------------------------------------- */
+int pml_init(int device);
+const char *pml_last_error(void);
+int pml_version(void);
+int pml_device_sync(void *stream);
+
+/*
+ * Advances every grid of the batch by n_steps steps of size dt.
+ *   state_hi/state_lo : in/out state (state_lo only for PML_F32C, else NULL)
+ *   work_hi/work_lo   : ping-pong buffers of the same size
+ *   snapshots_hi/lo   : optional (NULL = none) output of the whole batch state
+ *                       after every snapshot_every-th step; snapshot k (k>=0)
+ *                       is written at element offset k*total_elems
+ *   total_elems       : elements of one state plane (all grids, all fields)
+ *   tile_config       : kernel variant (0 = default; see pml_fdm_configs)
+ *   use_graph         : capture the step loop in hipGraphs when non-zero
+ * After the call the current state is in state_hi/lo.
+ */
+int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
+                  int integrator_id, int precision, int ndim,
+                  const pml_grid *grids, int n_grids, const double *bc_values,
+                  void *state_hi, void *state_lo, void *work_hi, void *work_lo,
+                  int64_t total_elems, double dt, int64_t n_steps,
+                  void *snapshots_hi, void *snapshots_lo,
+                  int64_t snapshot_every, int tile_config, int use_graph,
+                  void *stream);
+
+/* Number of kernel variants for (eq, integrator, precision) and a short
+ * description of variant i (for the autotuner). */
+int pml_fdm_configs(int eq_id, int integrator_id, int precision);
+const char *pml_fdm_config_name(int eq_id, int integrator_id, int precision,
+                                int config);
+
+/*
+ * Standalone stencils on a (outer, n, inner) view of a contiguous array,
+ * differentiating along the middle axis with spacing dx.  lo/hi are FP64
+ * (outer, inner) arrays of boundary derivative values (NaN = none) or NULL.
+ *   pml_stencil_first : D = (y[i+1]-y[i-1])/(2dx), zero halos, D[0]/D[n-1]
+ *                       replaced by lo/hi where given.
+ *   pml_stencil_second: S = ((y[i+1]-2y[i])+y[i-1])/(dx*dx2), ghosts
+ *                       y[1]-2dx*lo / y[n-2]+2dx*hi where given, else 0.
+ * mode: 0 -> out = r, 1 -> out += r, -1 -> out -= r.
+ */
+int pml_stencil_first(int precision, const void *y, void *out, int64_t outer,
+                      int64_t n, int64_t inner, double dx, const double *lo,
+                      const double *hi, int mode, void *stream);
+int pml_stencil_second(int precision, const void *y, void *out, int64_t outer,
+                       int64_t n, int64_t inner, double dx, double dx2,
+                       const double *lo, const double *hi, int mode,
+                       void *stream);
+
+/*
+ * One Jacobi sweep of the Cartesian anti-Laplacian on up to 3 spatial axes
+ * (shape[k] == 1 marks an absent axis; the last dimension holds n_fields).
+ * ghosts: per axis a, FP64 lower/upper slab arrays of the derivative
+ * constraints (NULL = none), laid out as the array with axis a removed.
+ * ycons: full-grid FP64 y constraints (NaN = free) or NULL.
+ * sumsq_out: device FP64 scalar receiving sum((y_new-y_old)^2).
+ */
+int pml_jacobi_sweep(int precision, const void *y_old, void *y_new,
+                     const void *lap, const int64_t *shape, int n_fields,
+                     const double *dx, const double *const *ghost_lo,
+                     const double *const *ghost_hi, const double *ycons,
+                     double *sumsq_out, void *stream);
+
+/*
+ * Parareal, all FP64.
+ *   pml_parareal_correction: corr[s] = f_end[s] - g_end[s] over n elements
+ *     per slice, for slices [s0, s1).  f_end may be FP32 or compensated FP32
+ *     (f_precision), converted exactly to FP64 before the subtraction.
+ *   pml_parareal_update: u_next = g_new + corr (one slice, n elements) and
+ *     per field (n_fields planes of n/n_fields) the sum of
+ *     (u_next - u_old)^2 into sumsq[n_fields] (accumulated: the caller
+ *     zeroes it).
+ */
+int pml_parareal_correction(int f_precision, const void *f_end_hi,
+                            const void *f_end_lo, int64_t f_stride,
+                            const double *g_end, double *corr, int64_t n,
+                            int64_t n_slices, void *stream);
+int pml_parareal_update(const double *g_new, const double *corr,
+                        double *u_next, int64_t n, int n_fields,
+                        double *sumsq, void *stream);
+
+/* Conversions between FP64 and the state precisions (device to device). */
+int pml_from_f64(int precision, const double *src, void *dst_hi, void *dst_lo,
+                 int64_t n, void *stream);
+int pml_to_f64(int precision, const void *src_hi, const void *src_lo,
+               double *dst, int64_t n, void *stream);
+
+/*
+ * 2x grid transfer between a fine vertex grid (2m-1 points per axis) and a
+ * coarse one (m points), batched over `batch` planes, FP64 or FP32 by
+ * precision (PML_F32C not accepted).  No counterpart in the reference.
+ */
+int pml_restrict2x(int precision, const void *fine, void *coarse,
+                   int64_t batch, int64_t m0, int64_t m1, void *stream);
+int pml_prolong2x(int precision, const void *coarse, void *fine,
+                  int64_t batch, int64_t m0, int64_t m1, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PML_H */

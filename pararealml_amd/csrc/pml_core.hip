@@ -1,0 +1,280 @@
+// Library state, precision conversions, Parareal FP64 updates and 2x grid
+// transfers for gfx950.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "pml_common.h"
+
+namespace pml {
+
+static thread_local std::string t_error;
+
+void set_error(const std::string &msg) { t_error = msg; }
+
+int fail(int code, const std::string &msg) {
+  set_error(msg);
+  return code;
+}
+
+static int blocks_for(int64_t n, int nt) {
+  int64_t b = (n + nt - 1) / nt;
+  if (b > (1 << 20)) b = 1 << 20;  // grid-stride loops cover the rest
+  return static_cast<int>(b < 1 ? 1 : b);
+}
+
+// ---- conversions ----------------------------------------------------------
+__global__ void k_from_f64_f32(const double *s, float *d, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = static_cast<float>(s[i]);
+}
+__global__ void k_from_f64_f32c(const double *s, float *hi, float *lo, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float h, l;
+    split_f64(s[i], h, l);
+    hi[i] = h;
+    lo[i] = l;
+  }
+}
+__global__ void k_copy_f64(const double *s, double *d, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = s[i];
+}
+__global__ void k_to_f64_f32(const float *s, double *d, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = static_cast<double>(s[i]);
+}
+__global__ void k_to_f64_f32c(const float *hi, const float *lo, double *d, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] = static_cast<double>(hi[i]) + static_cast<double>(lo[i]);
+}
+
+// ---- Parareal ---------------------------------------------------------------
+// corr[s][i] = f_end(s, i) - g_end[s][i]   (parareal_operator.py:164)
+template <int PREC>
+__global__ void k_correction(const void *f_hi, const void *f_lo, int64_t f_stride,
+                             const double *g, double *corr, int64_t n, int64_t n_slices) {
+  const int64_t total = n * n_slices;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t s = t / n, i = t - s * n;
+    const int64_t fi = s * f_stride + i;
+    double f;
+    if (PREC == PML_F64) f = static_cast<const double *>(f_hi)[fi];
+    else if (PREC == PML_F32) f = static_cast<const float *>(f_hi)[fi];
+    else
+      f = static_cast<double>(static_cast<const float *>(f_hi)[fi]) +
+          static_cast<double>(static_cast<const float *>(f_lo)[fi]);
+    corr[t] = f - g[t];
+  }
+}
+
+// u_next = g_new + corr; per field sum of squared updates against the
+// previous u_next (parareal_operator.py:53-100, 183-185).  One block-level
+// tree reduction per field, then one atomic per block.
+__global__ void k_update(const double *g, const double *corr, double *u, int64_t n,
+                         int n_fields, double *sumsq) {
+  __shared__ double red[256];
+  const int64_t per_field = n / n_fields;
+  for (int f = 0; f < n_fields; ++f) {
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < per_field;
+         i += (int64_t)gridDim.x * blockDim.x) {
+      const int64_t e = f * per_field + i;
+      const double nv = g[e] + corr[e];
+      const double d = nv - u[e];
+      acc += d * d;
+      u[e] = nv;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(&sumsq[f], red[0]);
+    __syncthreads();
+  }
+}
+
+// ---- 2x grid transfer ---------------------------------------------------------
+template <typename T>
+__global__ void k_restrict(const T *fine, T *coarse, int64_t batch, int64_t m0, int64_t m1) {
+  const int64_t f0 = 2 * m0 - 1, f1 = 2 * m1 - 1;
+  const int64_t total = batch * m0 * m1;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / (m0 * m1), r = t - b * m0 * m1;
+    const int64_t i = r / m1, j = r - i * m1;
+    coarse[t] = fine[b * f0 * f1 + (2 * i) * f1 + 2 * j];
+  }
+}
+
+template <typename T>
+__global__ void k_prolong(const T *coarse, T *fine, int64_t batch, int64_t m0, int64_t m1) {
+  const int64_t f0 = 2 * m0 - 1, f1 = 2 * m1 - 1;
+  const int64_t total = batch * f0 * f1;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / (f0 * f1), r = t - b * f0 * f1;
+    const int64_t i = r / f1, j = r - i * f1;
+    const T *c = coarse + b * m0 * m1;
+    const int64_t ci = i >> 1, cj = j >> 1;
+    const bool oi = i & 1, oj = j & 1;
+    T v;
+    if (!oi && !oj) v = c[ci * m1 + cj];
+    else if (oi && !oj) v = T(0.5) * (c[ci * m1 + cj] + c[(ci + 1) * m1 + cj]);
+    else if (!oi && oj) v = T(0.5) * (c[ci * m1 + cj] + c[ci * m1 + cj + 1]);
+    else
+      v = T(0.25) * (c[ci * m1 + cj] + c[(ci + 1) * m1 + cj] + c[ci * m1 + cj + 1] +
+                     c[(ci + 1) * m1 + cj + 1]);
+    fine[t] = v;
+  }
+}
+
+}  // namespace pml
+
+using namespace pml;
+
+extern "C" const char *pml_last_error(void) { return t_error.c_str(); }
+
+extern "C" int pml_version(void) { return 100; }
+
+extern "C" int pml_init(int device) {
+  int count = 0;
+  PML_HIP_CHECK(hipGetDeviceCount(&count));
+  if (device < 0 || device >= count) return fail(PML_E_INVALID, "device index out of range");
+  PML_HIP_CHECK(hipSetDevice(device));
+  hipDeviceProp_t prop;
+  PML_HIP_CHECK(hipGetDeviceProperties(&prop, device));
+  if (std::string(prop.gcnArchName).rfind("gfx950", 0) != 0)
+    return fail(PML_E_UNSUPPORTED, std::string("built for gfx950, found ") + prop.gcnArchName);
+  return PML_OK;
+}
+
+extern "C" int pml_device_sync(void *stream) {
+  PML_HIP_CHECK(hipStreamSynchronize(as_stream(stream)));
+  return PML_OK;
+}
+
+extern "C" int pml_from_f64(int precision, const double *src, void *dst_hi, void *dst_lo,
+                            int64_t n, void *stream) {
+  if (n < 0 || (n && (!src || !dst_hi))) return fail(PML_E_INVALID, "bad conversion buffers");
+  if (n == 0) return PML_OK;
+  hipStream_t st = as_stream(stream);
+  const int nb = blocks_for(n, 256);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_copy_f64, dim3(nb), dim3(256), 0, st, src, (double *)dst_hi, n);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_from_f64_f32, dim3(nb), dim3(256), 0, st, src, (float *)dst_hi, n);
+  else if (precision == PML_F32C) {
+    if (!dst_lo) return fail(PML_E_INVALID, "compensated conversion needs a lo plane");
+    hipLaunchKernelGGL(k_from_f64_f32c, dim3(nb), dim3(256), 0, st, src, (float *)dst_hi,
+                       (float *)dst_lo, n);
+  } else
+    return fail(PML_E_INVALID, "unknown precision");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_to_f64(int precision, const void *src_hi, const void *src_lo, double *dst,
+                          int64_t n, void *stream) {
+  if (n < 0 || (n && (!src_hi || !dst))) return fail(PML_E_INVALID, "bad conversion buffers");
+  if (n == 0) return PML_OK;
+  hipStream_t st = as_stream(stream);
+  const int nb = blocks_for(n, 256);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_copy_f64, dim3(nb), dim3(256), 0, st, (const double *)src_hi, dst, n);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_to_f64_f32, dim3(nb), dim3(256), 0, st, (const float *)src_hi, dst, n);
+  else if (precision == PML_F32C) {
+    if (!src_lo) return fail(PML_E_INVALID, "compensated conversion needs a lo plane");
+    hipLaunchKernelGGL(k_to_f64_f32c, dim3(nb), dim3(256), 0, st, (const float *)src_hi,
+                       (const float *)src_lo, dst, n);
+  } else
+    return fail(PML_E_INVALID, "unknown precision");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_parareal_correction(int f_precision, const void *f_end_hi,
+                                       const void *f_end_lo, int64_t f_stride,
+                                       const double *g_end, double *corr, int64_t n,
+                                       int64_t n_slices, void *stream) {
+  if (n <= 0 || n_slices <= 0 || f_stride < n || !f_end_hi || !g_end || !corr)
+    return fail(PML_E_INVALID, "bad correction arguments");
+  if (f_precision == PML_F32C && !f_end_lo) return fail(PML_E_INVALID, "missing lo plane");
+  hipStream_t st = as_stream(stream);
+  const int nb = blocks_for(n * n_slices, 256);
+  switch (f_precision) {
+    case PML_F64:
+      hipLaunchKernelGGL(k_correction<PML_F64>, dim3(nb), dim3(256), 0, st, f_end_hi, f_end_lo,
+                         f_stride, g_end, corr, n, n_slices);
+      break;
+    case PML_F32:
+      hipLaunchKernelGGL(k_correction<PML_F32>, dim3(nb), dim3(256), 0, st, f_end_hi, f_end_lo,
+                         f_stride, g_end, corr, n, n_slices);
+      break;
+    case PML_F32C:
+      hipLaunchKernelGGL(k_correction<PML_F32C>, dim3(nb), dim3(256), 0, st, f_end_hi, f_end_lo,
+                         f_stride, g_end, corr, n, n_slices);
+      break;
+    default: return fail(PML_E_INVALID, "unknown precision");
+  }
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_parareal_update(const double *g_new, const double *corr, double *u_next,
+                                   int64_t n, int n_fields, double *sumsq, void *stream) {
+  if (n <= 0 || n_fields <= 0 || n % n_fields != 0 || !g_new || !corr || !u_next || !sumsq)
+    return fail(PML_E_INVALID, "bad update arguments");
+  hipStream_t st = as_stream(stream);
+  int nb = blocks_for(n / n_fields, 256);
+  if (nb > 1024) nb = 1024;
+  hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, g_new, corr, u_next, n, n_fields,
+                     sumsq);
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_restrict2x(int precision, const void *fine, void *coarse, int64_t batch,
+                              int64_t m0, int64_t m1, void *stream) {
+  if (batch <= 0 || m0 < 2 || m1 < 2 || !fine || !coarse)
+    return fail(PML_E_INVALID, "bad restriction arguments");
+  hipStream_t st = as_stream(stream);
+  const int nb = blocks_for(batch * m0 * m1, 256);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_restrict<double>, dim3(nb), dim3(256), 0, st, (const double *)fine,
+                       (double *)coarse, batch, m0, m1);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_restrict<float>, dim3(nb), dim3(256), 0, st, (const float *)fine,
+                       (float *)coarse, batch, m0, m1);
+  else
+    return fail(PML_E_INVALID, "restriction takes FP32 or FP64 planes");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_prolong2x(int precision, const void *coarse, void *fine, int64_t batch,
+                             int64_t m0, int64_t m1, void *stream) {
+  if (batch <= 0 || m0 < 2 || m1 < 2 || !fine || !coarse)
+    return fail(PML_E_INVALID, "bad prolongation arguments");
+  hipStream_t st = as_stream(stream);
+  const int nb = blocks_for(batch * (2 * m0 - 1) * (2 * m1 - 1), 256);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_prolong<double>, dim3(nb), dim3(256), 0, st, (const double *)coarse,
+                       (double *)fine, batch, m0, m1);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_prolong<float>, dim3(nb), dim3(256), 0, st, (const float *)coarse,
+                       (float *)fine, batch, m0, m1);
+  else
+    return fail(PML_E_INVALID, "prolongation takes FP32 or FP64 planes");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}

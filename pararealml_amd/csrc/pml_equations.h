@@ -1,0 +1,143 @@
+// Fused right-hand sides of the equations that have device kernels.
+//
+// Each equation describes, per field, which stencil quantities it needs
+// (kLap: Laplacian, kD: first derivatives, as bit masks over fields), which
+// fields are algebraic (LHS.Y: recomputed from the pre-step state after the
+// integrator step, pararealml/operators/fdm/fdm_operator.py:127-139) and
+// whether its time-derivative right-hand sides are independent of the
+// time-derivative fields themselves (kInvariant).  For an invariant system
+// every Runge-Kutta stage sees the same right-hand side -- the stage inputs
+// differ only in fields the right-hand side does not read -- so the stages
+// are evaluated once and combined with the integrator's own weights.
+//
+// The expressions restate pararealml/differential_equation.py:608-819.
+#pragma once
+
+namespace pml {
+
+template <typename C>
+struct Quant {
+  C v;    // value
+  C lap;  // Laplacian (sum over active axes, axis order)
+  C d0;   // derivative along physical axis 0
+  C d1;   // derivative along physical axis 1 (0 for 1-D problems)
+};
+
+// u_t = d lap(u)    (differential_equation.py:608-630)
+struct EqDiffusion {
+  static constexpr int F = 1;
+  static constexpr unsigned kLap = 0x1u, kD = 0x0u, kAlg = 0x0u;
+  static constexpr bool kInvariant = false;
+  template <typename C>
+  __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
+    r[0] = p[0] * q[0].lap;
+  }
+  template <typename C>
+  __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
+};
+
+// u_t = d lap(u) - (v0 u_x0 + v1 u_x1)    (differential_equation.py:633-670)
+struct EqConvectionDiffusion {
+  static constexpr int F = 1;
+  static constexpr unsigned kLap = 0x1u, kD = 0x1u, kAlg = 0x0u;
+  static constexpr bool kInvariant = false;
+  template <typename C>
+  __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
+    r[0] = p[0] * q[0].lap - p[1] * q[0].d0 - p[2] * q[0].d1;
+  }
+  template <typename C>
+  __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
+};
+
+// (u, w)_t = (w, c^2 lap(u))    (differential_equation.py:673-697)
+struct EqWave {
+  static constexpr int F = 2;
+  static constexpr unsigned kLap = 0x1u, kD = 0x0u, kAlg = 0x0u;
+  static constexpr bool kInvariant = false;
+  template <typename C>
+  __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
+    r[0] = q[1].v;
+    r[1] = p[0] * q[0].lap;
+  }
+  template <typename C>
+  __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
+};
+
+// c_t = d lap(mu);  mu = c^3 - c - gamma lap(c)  (LHS.Y)
+// (differential_equation.py:700-731).  c_t reads only mu, which is
+// algebraic, so the system is stage invariant.
+struct EqCahnHilliard {
+  static constexpr int F = 2;
+  static constexpr unsigned kLap = 0x3u, kD = 0x0u, kAlg = 0x2u;
+  static constexpr bool kInvariant = true;
+  template <typename C>
+  __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
+    r[0] = p[0] * q[1].lap;
+  }
+  template <typename C>
+  __device__ __forceinline__ static void alg(const C *p, const Quant<C> *q, C *r) {
+    const C c = q[0].v;
+    r[1] = (c * c * c - c) - p[1] * q[0].lap;
+  }
+};
+
+// 1-D Burgers: u_t = lap(u)/re - u u_x    (differential_equation.py:734-765)
+struct EqBurgers1 {
+  static constexpr int F = 1;
+  static constexpr unsigned kLap = 0x1u, kD = 0x1u, kAlg = 0x0u;
+  static constexpr bool kInvariant = false;
+  template <typename C>
+  __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
+    r[0] = p[0] * q[0].lap - q[0].v * q[0].d0;
+  }
+  template <typename C>
+  __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
+};
+
+// 2-D Burgers: u_i,t = lap(u_i)/re - (u u_i,x0 + v u_i,x1)
+struct EqBurgers2 {
+  static constexpr int F = 2;
+  static constexpr unsigned kLap = 0x3u, kD = 0x3u, kAlg = 0x0u;
+  static constexpr bool kInvariant = false;
+  template <typename C>
+  __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
+    r[0] = p[0] * q[0].lap - (q[0].v * q[0].d0 + q[1].v * q[0].d1);
+    r[1] = p[0] * q[1].lap - (q[0].v * q[1].d0 + q[1].v * q[1].d1);
+  }
+  template <typename C>
+  __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
+};
+
+// Shallow water in (eta, u, v), parameters (h, b, nu, f, g)
+// (differential_equation.py:768-819).
+struct EqShallowWater {
+  static constexpr int F = 3;
+  static constexpr unsigned kLap = 0x6u, kD = 0x7u, kAlg = 0x0u;
+  static constexpr bool kInvariant = false;
+  template <typename C>
+  __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
+    const C h = p[0], b = p[1], nu = p[2], f = p[3], g = p[4];
+    const C eta = q[0].v, u = q[1].v, v = q[2].v;
+    r[0] = -h * (q[1].d0 + q[2].d1) - eta * q[1].d0 - u * q[0].d0 -
+           eta * q[2].d1 - v * q[0].d1;
+    r[1] = nu * q[1].lap - u * q[1].d0 - v * q[1].d1 - g * q[0].d0 - b * u +
+           f * v;
+    r[2] = nu * q[2].lap - u * q[2].d0 - v * q[2].d1 - g * q[0].d1 - b * v -
+           f * u;
+  }
+  template <typename C>
+  __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
+};
+
+// Integrators (pararealml/operators/fdm/numerical_integrator.py:42-132).
+struct IntFE {
+  static constexpr int kStages = 1;
+};
+struct IntMidpoint {
+  static constexpr int kStages = 2;
+};
+struct IntRK4 {
+  static constexpr int kStages = 4;
+};
+
+}  // namespace pml

@@ -1,0 +1,240 @@
+// Standalone three-point stencils and the Jacobi anti-Laplacian sweep for
+// gfx950, backing NumericalDifferentiator's public methods on any number of
+// dimensions.  Semantics restate
+// pararealml/operators/fdm/numerical_differentiator.py:1012-1242.
+#include <hip/hip_runtime.h>
+
+#include "pml_common.h"
+
+namespace pml {
+
+static int grid_for(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  if (b > (1 << 20)) b = 1 << 20;
+  return static_cast<int>(b < 1 ? 1 : b);
+}
+
+template <typename T>
+__device__ __forceinline__ void emit(T *out, int64_t t, T r, int mode) {
+  if (mode == 0) out[t] = r;
+  else if (mode > 0) out[t] = out[t] + r;
+  else out[t] = out[t] - r;
+}
+
+// (y[i+1] - y[i-1]) / (2 dx) with zero halos, boundary override by lo/hi
+// (numerical_differentiator.py:1012-1057).
+template <typename T>
+__global__ void k_first(const T *y, T *out, int64_t outer, int64_t n, int64_t inner,
+                        double two_dx, const double *lo, const double *hi, int mode) {
+  const int64_t total = outer * n * inner;
+  const T den = static_cast<T>(two_dx);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = t / (n * inner), rem = t - o * n * inner;
+    const int64_t i = rem / inner, q = rem - i * inner;
+    const T yp = i > 0 ? y[t - inner] : T(0);
+    const T yn = i < n - 1 ? y[t + inner] : T(0);
+    T d = (yn - yp) / den;
+    if (i == 0 && lo) {
+      const double g = lo[o * inner + q];
+      if (g == g) d = static_cast<T>(g);
+    }
+    if (i == n - 1 && hi) {
+      const double g = hi[o * inner + q];
+      if (g == g) d = static_cast<T>(g);
+    }
+    emit(out, t, d, mode);
+  }
+}
+
+// ((y[i+1] - 2 y[i]) + y[i-1]) / (dx dx2) with constraint ghosts
+// (numerical_differentiator.py:1083-1095, 1188-1242).
+template <typename T>
+__global__ void k_second(const T *y, T *out, int64_t outer, int64_t n, int64_t inner,
+                         double dx, double den_d, const double *lo, const double *hi,
+                         int mode) {
+  const int64_t total = outer * n * inner;
+  const T den = static_cast<T>(den_d);
+  const T m2dx = static_cast<T>(-2.0 * dx), p2dx = static_cast<T>(2.0 * dx);
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t o = t / (n * inner), rem = t - o * n * inner;
+    const int64_t i = rem / inner, q = rem - i * inner;
+    T yp, yn;
+    if (i > 0) {
+      yp = y[t - inner];
+    } else {
+      const double g = lo ? lo[o * inner + q] : NAN;
+      yp = (g == g) ? y[t + inner] + m2dx * static_cast<T>(g) : T(0);
+    }
+    if (i < n - 1) {
+      yn = y[t + inner];
+    } else {
+      const double g = hi ? hi[o * inner + q] : NAN;
+      yn = (g == g) ? y[t - inner] + p2dx * static_cast<T>(g) : T(0);
+    }
+    const T r = ((yn - T(2) * y[t]) + yp) / den;
+    emit(out, t, r, mode);
+  }
+}
+
+struct JacobiGeom {
+  int64_t n[3];
+  int64_t stride[3];  // element strides of the three spatial axes
+  double dx[3];
+  double dxsq[3];
+  const double *glo[3];
+  const double *ghi[3];
+  int active[3];
+  double coef;  // sum over active axes of 2 / dx^2
+};
+
+// One Jacobi update of the Cartesian anti-Laplacian
+// (numerical_differentiator.py:1097-1186) followed by the y constraints
+// and a sum of squared updates (872-927).
+template <typename T>
+__global__ void k_jacobi(const T *y, T *yn, const T *lap, int64_t total, int nf,
+                         JacobiGeom g, const double *ycons, double *sumsq) {
+  __shared__ double red[256];
+  double acc2 = 0.0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t f = t % nf;
+    const int64_t cell = t / nf;
+    int64_t idx[3];
+    idx[2] = cell % g.n[2];
+    idx[1] = (cell / g.n[2]) % g.n[1];
+    idx[0] = cell / (g.n[2] * g.n[1]);
+    T acc = 0;
+    for (int a = 0; a < 3; ++a) {
+      if (!g.active[a]) continue;
+      const int64_t s = g.stride[a];
+      // Slab index: the element with axis a removed.
+      int64_t slab = 0;
+      for (int b = 0; b < 3; ++b) {
+        if (b == a) continue;
+        slab = slab * g.n[b] + idx[b];
+      }
+      slab = slab * nf + f;
+      T prev, next;
+      if (idx[a] > 0) {
+        prev = y[t - s];
+      } else {
+        const double gv = g.glo[a] ? g.glo[a][slab] : NAN;
+        prev = (gv == gv) ? y[t + s] + static_cast<T>(-2.0 * g.dx[a]) * static_cast<T>(gv) : T(0);
+      }
+      if (idx[a] < g.n[a] - 1) {
+        next = y[t + s];
+      } else {
+        const double gv = g.ghi[a] ? g.ghi[a][slab] : NAN;
+        next = (gv == gv) ? y[t - s] + static_cast<T>(2.0 * g.dx[a]) * static_cast<T>(gv) : T(0);
+      }
+      acc = acc + (prev + next) / static_cast<T>(g.dxsq[a]);
+    }
+    acc = acc - lap[t];
+    T v = acc / static_cast<T>(g.coef);
+    if (ycons) {
+      const double c = ycons[t];
+      if (c == c) v = static_cast<T>(c);
+    }
+    const double d = static_cast<double>(v) - static_cast<double>(y[t]);
+    acc2 += d * d;
+    yn[t] = v;
+  }
+  red[threadIdx.x] = acc2;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(sumsq, red[0]);
+}
+
+}  // namespace pml
+
+using namespace pml;
+
+extern "C" int pml_stencil_first(int precision, const void *y, void *out, int64_t outer,
+                                 int64_t n, int64_t inner, double dx, const double *lo,
+                                 const double *hi, int mode, void *stream) {
+  if (n <= 2) return fail(PML_E_INVALID, "y must contain at least 3 points along the axis");
+  if (outer <= 0 || inner <= 0 || !y || !out || !(dx > 0))
+    return fail(PML_E_INVALID, "bad stencil arguments");
+  hipStream_t st = as_stream(stream);
+  const int nb = grid_for(outer * n * inner);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_first<double>, dim3(nb), dim3(256), 0, st, (const double *)y,
+                       (double *)out, outer, n, inner, 2.0 * dx, lo, hi, mode);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_first<float>, dim3(nb), dim3(256), 0, st, (const float *)y,
+                       (float *)out, outer, n, inner, 2.0 * dx, lo, hi, mode);
+  else
+    return fail(PML_E_INVALID, "stencils take FP32 or FP64 arrays");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_stencil_second(int precision, const void *y, void *out, int64_t outer,
+                                  int64_t n, int64_t inner, double dx, double dx2,
+                                  const double *lo, const double *hi, int mode, void *stream) {
+  if (n <= 2) return fail(PML_E_INVALID, "y must contain at least 3 points along the axis");
+  if (outer <= 0 || inner <= 0 || !y || !out || !(dx > 0) || !(dx2 > 0))
+    return fail(PML_E_INVALID, "bad stencil arguments");
+  hipStream_t st = as_stream(stream);
+  const int nb = grid_for(outer * n * inner);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_second<double>, dim3(nb), dim3(256), 0, st, (const double *)y,
+                       (double *)out, outer, n, inner, dx, dx * dx2, lo, hi, mode);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_second<float>, dim3(nb), dim3(256), 0, st, (const float *)y,
+                       (float *)out, outer, n, inner, dx, dx * dx2, lo, hi, mode);
+  else
+    return fail(PML_E_INVALID, "stencils take FP32 or FP64 arrays");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_jacobi_sweep(int precision, const void *y_old, void *y_new, const void *lap,
+                                const int64_t *shape, int n_fields, const double *dx,
+                                const double *const *ghost_lo, const double *const *ghost_hi,
+                                const double *ycons, double *sumsq_out, void *stream) {
+  if (!y_old || !y_new || !lap || !shape || !dx || !sumsq_out || n_fields <= 0)
+    return fail(PML_E_INVALID, "bad Jacobi arguments");
+  JacobiGeom g{};
+  int64_t total = n_fields;
+  g.coef = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    g.n[a] = shape[a];
+    if (shape[a] <= 0) return fail(PML_E_INVALID, "bad Jacobi shape");
+    g.active[a] = shape[a] > 1;
+    if (g.active[a] && shape[a] <= 2)
+      return fail(PML_E_INVALID, "y must contain at least 3 points along all x axes");
+    g.dx[a] = dx[a];
+    g.dxsq[a] = dx[a] * dx[a];
+    g.glo[a] = ghost_lo ? ghost_lo[a] : nullptr;
+    g.ghi[a] = ghost_hi ? ghost_hi[a] : nullptr;
+    if (g.active[a]) {
+      if (!(dx[a] > 0)) return fail(PML_E_INVALID, "spacing must be positive");
+      g.coef += 2.0 / g.dxsq[a];
+    }
+    total *= shape[a];
+  }
+  g.stride[2] = n_fields;
+  g.stride[1] = shape[2] * n_fields;
+  g.stride[0] = shape[1] * shape[2] * n_fields;
+  hipStream_t st = as_stream(stream);
+  int nb = grid_for(total);
+  if (nb > 2048) nb = 2048;
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_jacobi<double>, dim3(nb), dim3(256), 0, st, (const double *)y_old,
+                       (double *)y_new, (const double *)lap, total, n_fields, g, ycons,
+                       sumsq_out);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_jacobi<float>, dim3(nb), dim3(256), 0, st, (const float *)y_old,
+                       (float *)y_new, (const float *)lap, total, n_fields, g, ycons,
+                       sumsq_out);
+  else
+    return fail(PML_E_INVALID, "Jacobi sweep takes FP32 or FP64 arrays");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}

@@ -1,0 +1,411 @@
+"""Three-point finite-difference operators evaluated by gfx950 kernels.
+
+The public methods and their validation follow
+pararealml/operators/fdm/numerical_differentiator.py:14-996 (Cartesian
+branches) and the stencil semantics follow
+``ThreePointCentralDifferenceMethod`` (numerical_differentiator.py:
+999-1242):
+
+* first derivatives: zero halos, boundary values replaced by the derivative
+  constraints (``pml_stencil_first``);
+* same-axis second derivatives: ghosts ``y[1] - 2dx g`` / ``y[n-2] + 2dx g``
+  where constrained, 0 elsewhere (``pml_stencil_second``);
+* mixed second derivatives: a constrained first derivative followed by an
+  unconstrained one;
+* the anti-Laplacian: Jacobi sweeps (``pml_jacobi_sweep``) with the y
+  constraints re-imposed after each sweep until the 2-norm of the update is
+  at most ``tol``.
+
+Inputs may be NumPy arrays (copied to HBM, results copied back) or torch
+tensors already on the GPU (results stay there).  Arrays are in the
+reference layout, channel-last ``(*vertices_shape, y_dim)``.  Curvilinear
+coordinate systems are not covered by the device kernels and raise
+``NotImplementedError``.
+"""
+
+from abc import ABC
+from typing import Optional, Sequence, Union
+
+import ctypes
+import numpy as np
+
+from pararealml_amd import _native as nat
+from pararealml_amd.constraint import Constraint
+from pararealml_amd.mesh import CoordinateSystem, Mesh
+
+
+# --------------------------------------------------------------------------
+# array plumbing
+
+
+class _Arr:
+    """A contiguous device copy of an input plus how to hand results back."""
+
+    def __init__(self, y):
+        import torch
+
+        self.is_numpy = isinstance(y, np.ndarray)
+        if self.is_numpy:
+            arr = np.ascontiguousarray(y)
+            if arr.dtype not in (np.float32, np.float64):
+                arr = arr.astype(np.float64)
+            self.t = torch.from_numpy(arr).to("cuda")
+        else:
+            t = y if y.is_cuda else y.to("cuda")
+            if t.dtype not in (torch.float32, torch.float64):
+                t = t.to(torch.float64)
+            self.t = t.contiguous()
+        self.prec = nat.PML_F64 if self.t.dtype == torch.float64 else nat.PML_F32
+
+    def new(self, shape):
+        import torch
+
+        return torch.empty(shape, dtype=self.t.dtype, device=self.t.device)
+
+    def give(self, t):
+        return t.cpu().numpy() if self.is_numpy else t
+
+
+def _stream():
+    import torch
+
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _slab_array(pairs, side: int, shape, axis: int):
+    """FP64 NaN-encoded (outer, inner) boundary values of the derivative
+    constraints on one side of ``axis`` for an array of ``shape``, or None
+    when nothing is constrained."""
+    if pairs is None:
+        return None
+    slab_shape = list(shape)
+    slab_shape[axis] = 1
+    slab = np.full(slab_shape, np.nan)
+    any_set = False
+    for f, pair in enumerate(pairs):
+        if pair is None or pair[side] is None:
+            continue
+        c = pair[side]
+        slab[..., f : f + 1][..., c.mask] = c.values
+        any_set = True
+    if not any_set:
+        return None
+    import torch
+
+    return torch.from_numpy(np.ascontiguousarray(slab).ravel()).to("cuda")
+
+
+def _view(shape, axis):
+    outer = int(np.prod(shape[:axis], dtype=np.int64))
+    inner = int(np.prod(shape[axis + 1 :], dtype=np.int64))
+    return outer, int(shape[axis]), inner
+
+
+def _first(src: "_Arr", t, out, axis: int, d_x: float, pairs, mode: int):
+    lib = nat.device_lib()
+    lo = _slab_array(pairs, 0, t.shape, axis)
+    hi = _slab_array(pairs, 1, t.shape, axis)
+    outer, n, inner = _view(t.shape, axis)
+    nat.check(
+        lib.pml_stencil_first(
+            src.prec, t.data_ptr(), out.data_ptr(), outer, n, inner, float(d_x),
+            None if lo is None else lo.data_ptr(),
+            None if hi is None else hi.data_ptr(),
+            mode, _stream(),
+        )
+    )
+
+
+def _second(src: "_Arr", t, out, axis: int, d_x: float, d_x2: float, pairs, mode: int):
+    lib = nat.device_lib()
+    lo = _slab_array(pairs, 0, t.shape, axis)
+    hi = _slab_array(pairs, 1, t.shape, axis)
+    outer, n, inner = _view(t.shape, axis)
+    nat.check(
+        lib.pml_stencil_second(
+            src.prec, t.data_ptr(), out.data_ptr(), outer, n, inner, float(d_x),
+            float(d_x2),
+            None if lo is None else lo.data_ptr(),
+            None if hi is None else hi.data_ptr(),
+            mode, _stream(),
+        )
+    )
+
+
+# --------------------------------------------------------------------------
+
+
+class NumericalDifferentiator(ABC):
+    """Validation and public interface (numerical_differentiator.py:14-996)."""
+
+    def __init__(self, tol: float = 1e-3):
+        if tol < 0.0:
+            raise ValueError("tolerance must be non-negative")
+        self._tol = tol
+
+    # ---- validation helpers (numerical_differentiator.py:929-996) ---------
+    @staticmethod
+    def _verify_input_shape_matches_mesh(arr, mesh: Mesh, name: str = "y"):
+        if tuple(arr.shape[:-1]) != tuple(mesh.vertices_shape):
+            raise ValueError(
+                f"{name} shape up to second to last axis {tuple(arr.shape[:-1])} "
+                f"must match mesh vertices shape {mesh.vertices_shape}"
+            )
+
+    @staticmethod
+    def _verify_input_is_a_vector_field(arr, mesh: Mesh):
+        NumericalDifferentiator._verify_input_shape_matches_mesh(arr, mesh)
+        if arr.shape[-1] != mesh.dimensions:
+            raise ValueError(
+                f"y value vector length ({arr.shape[-1]}) must match number of "
+                f"x dimensions ({mesh.dimensions})"
+            )
+
+    @staticmethod
+    def _verify_and_get_derivative_boundary_constraints(dbc, x_axes, y_elements):
+        if dbc is None:
+            return np.empty((x_axes, y_elements), dtype=object)
+        if dbc.shape != (x_axes, y_elements):
+            raise ValueError(
+                "expected derivative boundary constraints shape to be "
+                f"{(x_axes, y_elements)} but got {dbc.shape}"
+            )
+        return dbc
+
+    @staticmethod
+    def _require_cartesian(mesh: Mesh):
+        if mesh.coordinate_system_type != CoordinateSystem.CARTESIAN:
+            raise NotImplementedError(
+                "the device stencils cover Cartesian meshes; "
+                f"got {mesh.coordinate_system_type.name}"
+            )
+
+    @staticmethod
+    def _require_points(shape, axis):
+        if shape[axis] <= 2:
+            raise ValueError(f"y must contain at least 3 points along x-axis ({axis})")
+
+
+class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
+    """Second-order central differences on gfx950
+    (numerical_differentiator.py:999-1242)."""
+
+    def __init__(self, tol: float = 1e-3):
+        super().__init__(tol)
+
+    # ---------------------------------------------------------------- public
+    def gradient(self, y, mesh: Mesh, x_axis: int, derivative_boundary_constraints=None):
+        """numerical_differentiator.py:114-173."""
+        self._verify_input_shape_matches_mesh(y, mesh)
+        if not (0 <= x_axis < mesh.dimensions):
+            raise ValueError(
+                f"x-axis ({x_axis}) must be non-negative and less than number "
+                f"of x dimensions ({mesh.dimensions})"
+            )
+        dbc = self._verify_and_get_derivative_boundary_constraints(
+            derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
+        )
+        self._require_cartesian(mesh)
+        self._require_points(y.shape, x_axis)
+        a = _Arr(y)
+        out = a.new(a.t.shape)
+        _first(a, a.t, out, x_axis, mesh.d_x[x_axis], dbc[x_axis], 0)
+        return a.give(out)
+
+    def hessian(
+        self, y, mesh: Mesh, x_axis1: int, x_axis2: int, derivative_boundary_constraints=None
+    ):
+        """numerical_differentiator.py:175-308."""
+        self._verify_input_shape_matches_mesh(y, mesh)
+        if not (0 <= x_axis1 < mesh.dimensions) or not (0 <= x_axis2 < mesh.dimensions):
+            raise ValueError(
+                f"both first x-axis ({x_axis1}) and second x-axis ({x_axis2}) "
+                "must be non-negative and less than number of x dimensions "
+                f"({mesh.dimensions})"
+            )
+        dbc = self._verify_and_get_derivative_boundary_constraints(
+            derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
+        )
+        self._require_cartesian(mesh)
+        a = _Arr(y)
+        out = a.new(a.t.shape)
+        if x_axis1 == x_axis2:
+            self._require_points(y.shape, x_axis1)
+            _second(a, a.t, out, x_axis1, mesh.d_x[x_axis1], mesh.d_x[x_axis2],
+                    dbc[x_axis1], 0)
+        else:
+            self._require_points(y.shape, x_axis1)
+            self._require_points(y.shape, x_axis2)
+            tmp = a.new(a.t.shape)
+            _first(a, a.t, tmp, x_axis1, mesh.d_x[x_axis1], dbc[x_axis1], 0)
+            _first(a, tmp, out, x_axis2, mesh.d_x[x_axis2], [None] * y.shape[-1], 0)
+        return a.give(out)
+
+    def divergence(self, y, mesh: Mesh, derivative_boundary_constraints=None):
+        """numerical_differentiator.py:310-400 (Cartesian: sum of the
+        component derivatives accumulated in component order)."""
+        self._verify_input_is_a_vector_field(y, mesh)
+        dbc = self._verify_and_get_derivative_boundary_constraints(
+            derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
+        )
+        self._require_cartesian(mesh)
+        a = _Arr(y)
+        out = a.new(tuple(a.t.shape[:-1]) + (1,))
+        out.zero_()
+        for i in range(y.shape[-1]):
+            self._require_points(y.shape, i)
+            comp = a.t[..., i : i + 1].contiguous()
+            _first(a, comp, out, i, mesh.d_x[i], dbc[i, i : i + 1], 1)
+        return a.give(out)
+
+    def curl(self, y, mesh: Mesh, curl_ind: int = 0, derivative_boundary_constraints=None):
+        """numerical_differentiator.py:402-590 (Cartesian)."""
+        self._verify_input_is_a_vector_field(y, mesh)
+        if not (2 <= mesh.dimensions <= 3):
+            raise ValueError(f"number of x dimensions ({mesh.dimensions}) must be 2 or 3")
+        if mesh.dimensions == 2 and curl_ind != 0:
+            raise ValueError(f"curl index ({curl_ind}) must be 0 for 2D curl")
+        if not (0 <= curl_ind < mesh.dimensions):
+            raise ValueError(
+                f"curl index ({curl_ind}) must be non-negative and less than "
+                f"number of x dimensions ({mesh.dimensions})"
+            )
+        dbc = self._verify_and_get_derivative_boundary_constraints(
+            derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
+        )
+        self._require_cartesian(mesh)
+        if mesh.dimensions == 2 or curl_ind == 2:
+            (pc, pa), (mc, ma) = (1, 0), (0, 1)
+        elif curl_ind == 0:
+            (pc, pa), (mc, ma) = (2, 1), (1, 2)
+        else:
+            (pc, pa), (mc, ma) = (0, 2), (2, 0)
+        self._require_points(y.shape, pa)
+        self._require_points(y.shape, ma)
+        a = _Arr(y)
+        out = a.new(tuple(a.t.shape[:-1]) + (1,))
+        plus = a.t[..., pc : pc + 1].contiguous()
+        minus = a.t[..., mc : mc + 1].contiguous()
+        _first(a, plus, out, pa, mesh.d_x[pa], dbc[pa, pc : pc + 1], 0)
+        _first(a, minus, out, ma, mesh.d_x[ma], dbc[ma, mc : mc + 1], -1)
+        return a.give(out)
+
+    def laplacian(self, y, mesh: Mesh, derivative_boundary_constraints=None):
+        """Element-wise scalar Laplacian, accumulated over axes in order
+        (numerical_differentiator.py:592-634)."""
+        self._verify_input_shape_matches_mesh(y, mesh)
+        dbc = self._verify_and_get_derivative_boundary_constraints(
+            derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
+        )
+        self._require_cartesian(mesh)
+        a = _Arr(y)
+        return a.give(self._laplacian_dev(a, a.t, mesh, dbc))
+
+    def _laplacian_dev(self, a, t, mesh, dbc):
+        out = a.new(t.shape)
+        for axis in range(t.dim() - 1):
+            self._require_points(t.shape, axis)
+            _second(a, t, out, axis, mesh.d_x[axis], mesh.d_x[axis], dbc[axis, :],
+                    0 if axis == 0 else 1)
+        return out
+
+    def vector_laplacian(
+        self, y, mesh: Mesh, vector_laplacian_ind: int, derivative_boundary_constraints=None
+    ):
+        """numerical_differentiator.py:727-870 (Cartesian: the scalar
+        Laplacian of the selected component)."""
+        self._verify_input_is_a_vector_field(y, mesh)
+        if not (0 <= vector_laplacian_ind < mesh.dimensions):
+            raise ValueError(
+                f"vector Laplacian index ({vector_laplacian_ind}) must be "
+                "non-negative and less than number of x dimensions "
+                f"({mesh.dimensions})"
+            )
+        dbc = self._verify_and_get_derivative_boundary_constraints(
+            derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
+        )
+        self._require_cartesian(mesh)
+        k = vector_laplacian_ind
+        a = _Arr(y)
+        comp = a.t[..., k : k + 1].contiguous()
+        return a.give(self._laplacian_dev(a, comp, mesh, dbc[:, k : k + 1]))
+
+    def anti_laplacian(
+        self,
+        laplacian,
+        mesh: Mesh,
+        y_constraints: Union[Sequence[Optional[Constraint]], np.ndarray],
+        derivative_boundary_constraints=None,
+        y_init=None,
+    ):
+        """Jacobi solve of the Poisson problem with the given Laplacian
+        (numerical_differentiator.py:872-927, 1097-1186)."""
+        import torch
+
+        self._verify_input_shape_matches_mesh(laplacian, mesh, "Laplacian")
+        dbc = self._verify_and_get_derivative_boundary_constraints(
+            derivative_boundary_constraints, mesh.dimensions, laplacian.shape[-1]
+        )
+        self._require_cartesian(mesh)
+        if mesh.dimensions > 3:
+            raise NotImplementedError("the device Jacobi sweep covers up to 3 axes")
+        if not np.all(np.array(laplacian.shape[:-1]) > 2):
+            raise ValueError("y must contain at least 3 points along all x axes")
+        if y_init is None:
+            y_host = np.random.random(laplacian.shape)
+        else:
+            if y_init.shape != laplacian.shape:
+                raise ValueError
+            y_host = y_init
+        lap = _Arr(laplacian)
+        shape = tuple(laplacian.shape)
+        n_fields = shape[-1]
+        spatial = list(shape[:-1]) + [1] * (3 - (len(shape) - 1))
+        dxs = list(mesh.d_x) + [1.0] * (3 - mesh.dimensions)
+
+        # full-grid y constraints, NaN where free
+        ycons = None
+        if y_constraints is not None:
+            full = np.full(shape, np.nan)
+            for f, c in enumerate(y_constraints):
+                if c is not None:
+                    full[..., f : f + 1][..., c.mask] = c.values
+            ycons = torch.from_numpy(full.ravel()).to("cuda")
+        # ghost slabs per axis in "axis removed" layout
+        glo, ghi, keep = [], [], []
+        for axis in range(3):
+            if axis < mesh.dimensions:
+                for side, dst in ((0, glo), (1, ghi)):
+                    s = _slab_array(dbc[axis], side, shape, axis)
+                    keep.append(s)
+                    dst.append(None if s is None else s.data_ptr())
+            else:
+                glo.append(None)
+                ghi.append(None)
+
+        y = _Arr(np.asarray(y_host, dtype=np.float64) if lap.is_numpy else y_host)
+        cur = y.t.to(lap.t.dtype).clone()
+        if ycons is not None:
+            mask = ~torch.isnan(ycons)
+            cur.view(-1)[mask] = ycons[mask].to(cur.dtype)
+        nxt = torch.empty_like(cur)
+        sumsq = torch.zeros(1, dtype=torch.float64, device=cur.device)
+        lib = nat.device_lib()
+        c_shape = (ctypes.c_int64 * 3)(*spatial)
+        c_dx = (ctypes.c_double * 3)(*dxs)
+        c_lo = (ctypes.c_void_p * 3)(*glo)
+        c_hi = (ctypes.c_void_p * 3)(*ghi)
+        diff = np.inf
+        while diff > self._tol:
+            sumsq.zero_()
+            nat.check(
+                lib.pml_jacobi_sweep(
+                    lap.prec, cur.data_ptr(), nxt.data_ptr(), lap.t.data_ptr(), c_shape,
+                    n_fields, c_dx, c_lo, c_hi,
+                    None if ycons is None else ycons.data_ptr(),
+                    sumsq.data_ptr(), _stream(),
+                )
+            )
+            cur, nxt = nxt, cur
+            diff = float(np.sqrt(sumsq.item()))
+        return lap.give(cur)

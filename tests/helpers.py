@@ -1,0 +1,85 @@
+"""Shared builders for the tests: problems of the benchmark configurations
+at reduced sizes, and KAT fixture decoding."""
+
+import numpy as np
+
+import pararealml_amd as pml
+from pararealml_amd.constraint import Constraint
+
+
+def dbc_from_spec(spec, x_dim, y_dim):
+    """Derivative boundary constraints from the JSON fixture encoding."""
+    if spec is None:
+        return None
+    out = np.empty((x_dim, y_dim), dtype=object)
+    for key, pair in spec.items():
+        a, f = (int(v) for v in key.split(","))
+        cons = []
+        for side in pair:
+            if side is None:
+                cons.append(None)
+            else:
+                cons.append(
+                    Constraint(np.array(side["values"], dtype=float), np.array(side["mask"], dtype=bool))
+                )
+        out[a, f] = tuple(cons)
+    return out
+
+
+def zero_bc(n_fields, kind="dirichlet"):
+    if kind == "dirichlet":
+        bc = pml.DirichletBoundaryCondition(
+            lambda x, t: np.zeros((len(x), n_fields)), is_static=True
+        )
+    else:
+        bc = pml.NeumannBoundaryCondition(
+            lambda x, t: np.zeros((len(x), n_fields)), is_static=True
+        )
+    return bc
+
+
+def diffusion_c1(n=101, length=10.0, d=1.0):
+    """Diffusion (d) on [0, L]^2 with zero Dirichlet edges and a Gaussian
+    initial condition of mean (L/2, L/2), covariance L/4 * I."""
+    step = length / (n - 1)
+    eq = pml.DiffusionEquation(2, d)
+    mesh = pml.Mesh([(0.0, length), (0.0, length)], [step, step])
+    bc = zero_bc(1)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    ic = pml.GaussianInitialCondition(
+        cp, [(np.array([length / 2, length / 2]), np.eye(2) * (length / 4))]
+    )
+    return cp, ic
+
+
+def cahn_hilliard(n, seed=0, d=0.1, gamma=0.01, dx=0.1):
+    """Cahn-Hilliard on [0, dx (n-1)]^2, zero-Neumann edges, uniform noise
+    in [-0.05, 0.05] for c and mu_0 = c^3 - c - gamma lap(c) (from the
+    oracle's Laplacian)."""
+    from oracle import fdm_numpy as ora
+
+    eq = pml.CahnHilliardEquation(2, d, gamma)
+    mesh = pml.Mesh([(0.0, dx * (n - 1))] * 2, [dx, dx])
+    bc = zero_bc(2, "neumann")
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    rng = np.random.RandomState(seed)
+    c = rng.uniform(-0.05, 0.05, (n, n, 1))
+    lap = ora.laplacian(c, mesh.d_x, cp.static_boundary_vertex_constraints[1][:, :1])
+    mu = c**3 - c - gamma * lap
+    y0 = np.concatenate([c, mu], axis=-1)
+    ic = pml.DiscreteInitialCondition(cp, y0, True)
+    return cp, ic
+
+
+def shallow_water(n, seed=0, h=1.0, v=0.01, g=9.81, length=10.0):
+    eq = pml.ShallowWaterEquation(h=h, v=v, g=g)
+    step = length / (n - 1)
+    mesh = pml.Mesh([(-length / 2, length / 2)] * 2, [step, step])
+    bc = zero_bc(3, "neumann")
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    rng = np.random.RandomState(seed)
+    mean = rng.uniform(-length / 4, length / 4, 2)
+    ic = pml.GaussianInitialCondition(
+        cp, [(mean, 0.25 * np.eye(2))] * 3, [0.1, 0.0, 0.0]
+    )
+    return cp, ic

@@ -1,0 +1,291 @@
+"""Parity of the fused gfx950 FDM kernels (through FDMOperator and the C ABI)
+with the CPU oracle.
+
+Tolerances: FP64 device results must match the FP64 oracle to a relative
+max-norm difference of 1e-12 (different but equally rounded op order and
+reciprocal multiplications); FP32 results are compared with the FP64 oracle
+at a tolerance derived from the step count (stated per test)."""
+
+import numpy as np
+import pytest
+
+import pararealml_amd as pml
+from oracle import fdm_numpy as ora
+from pararealml_amd.operators.fdm import (
+    RK4,
+    ExplicitMidpointMethod,
+    FDMOperator,
+    ForwardEulerMethod,
+    ThreePointCentralDifferenceMethod,
+)
+from tests.helpers import cahn_hilliard, diffusion_c1, shallow_water, zero_bc
+
+pytestmark = pytest.mark.gpu
+
+INTEGRATORS = {"fe": ForwardEulerMethod, "midpoint": ExplicitMidpointMethod, "rk4": RK4}
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def run_both(cp, ic, t_interval, d_t, integ="fe", precision="float64", **kw):
+    ivp = pml.InitialValueProblem(cp, t_interval, ic)
+    op = FDMOperator(INTEGRATORS[integ](), ThreePointCentralDifferenceMethod(), d_t,
+                     precision=precision, **kw)
+    sol = op.solve(ivp)
+    t_ref, y_ref = ora.fdm_solve(cp, ic.discrete_y_0(True), t_interval, d_t, integ)
+    np.testing.assert_allclose(sol.t_coordinates, t_ref)
+    return sol, y_ref
+
+
+def test_c1_diffusion_fe_fp64_parity():
+    """The reference-runnable configuration: 101x101, FE, d_t=0.002, 1000
+    steps, zero Dirichlet, Gaussian IC; full trajectory."""
+    print("case: c1 diffusion fe fp64 101x101x1000", flush=True)
+    cp, ic = diffusion_c1()
+    sol, y_ref = run_both(cp, ic, (0.0, 2.0), 0.002, "fe")
+    y = sol.discrete_y()
+    assert y.shape == (1000, 101, 101, 1)
+    assert sol.d_t == 0.002
+    assert rel_err(y, y_ref) <= 1e-12
+    # Dirichlet edges are exactly zero at every step
+    assert np.all(y[:, 0] == 0) and np.all(y[:, :, -1] == 0)
+
+
+def test_c1_diffusion_fe_fp32_tolerance():
+    """FP32 state vs the FP64 oracle after 1000 steps.  The oracle's own
+    float32 mode drifts by 1.0e-7 (relative max norm) on this case; the
+    stated tolerance is 1e-6."""
+    print("case: c1 diffusion fe fp32", flush=True)
+    cp, ic = diffusion_c1()
+    sol, y_ref = run_both(cp, ic, (0.0, 2.0), 0.002, "fe", precision="float32")
+    err = rel_err(sol.discrete_y(), y_ref)
+    assert err <= 1e-6, err
+
+
+def test_compensated_fp32_beats_plain_fp32():
+    print("case: c1 diffusion fe fp32c", flush=True)
+    cp, ic = diffusion_c1(n=41, length=4.0)
+    t = (0.0, 1.0)
+    ivp = pml.InitialValueProblem(cp, t, ic)
+    _, y_ref = ora.fdm_solve(cp, ic.discrete_y_0(True), t, 0.001, "rk4", keep="last")
+    errs = {}
+    for prec in ("float32", "float32c"):
+        op = FDMOperator(RK4(), ThreePointCentralDifferenceMethod(), 0.001, precision=prec)
+        errs[prec] = rel_err(op.solve(ivp).discrete_y()[-1], y_ref[0])
+    assert errs["float32c"] <= 1e-6, errs
+    assert errs["float32c"] < errs["float32"], errs
+
+
+@pytest.mark.parametrize("integ", ["fe", "midpoint", "rk4"])
+def test_diffusion_mixed_bcs_fp64(integ):
+    """Dirichlet on axis 0 (different constants, so the axis-1 Neumann edge
+    corners show the later-axis rule), non-zero Neumann on axis 1."""
+    print(f"case: diffusion mixed bcs {integ}", flush=True)
+    eq = pml.DiffusionEquation(2, 0.7)
+    mesh = pml.Mesh([(0.0, 2.0), (0.0, 3.0)], [0.1, 0.125])
+    bcs = [
+        (
+            pml.DirichletBoundaryCondition(lambda x, t: np.full((len(x), 1), 0.5), is_static=True),
+            pml.DirichletBoundaryCondition(lambda x, t: x[:, 1:2] * 0.1, is_static=True),
+        ),
+        (
+            pml.NeumannBoundaryCondition(lambda x, t: np.full((len(x), 1), -0.3), is_static=True),
+            pml.CauchyBoundaryCondition(
+                lambda x, t: np.full((len(x), 1), 1.0),
+                lambda x, t: x[:, :1] * 0.2,
+                is_static=True,
+            ),
+        ),
+    ]
+    cp = pml.ConstrainedProblem(eq, mesh, bcs)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([1.0, 1.5]), 0.2 * np.eye(2))], [2.0])
+    sol, y_ref = run_both(cp, ic, (0.0, 0.2), 0.002, integ)
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+def test_partial_nan_boundary_conditions_fp64():
+    """Boundary functions returning None (NaN) for some points and fields."""
+    print("case: partial nan bcs", flush=True)
+    eq = pml.WaveEquation(2, 0.8)
+
+    def lower(x, t):
+        return (0.0 if x[1] < 1.0 else None, None)
+
+    bcs = [
+        (
+            pml.DirichletBoundaryCondition(pml.vectorize_bc_function(lower), is_static=True),
+            pml.NeumannBoundaryCondition(
+                pml.vectorize_bc_function(lambda x, t: (None, 0.5)), is_static=True
+            ),
+        ),
+        (
+            pml.CauchyBoundaryCondition(
+                pml.vectorize_bc_function(lambda x, t: (None, 0.1)),
+                pml.vectorize_bc_function(lambda x, t: (0.2, None)),
+                is_static=True,
+            ),
+            zero_bc(2, "neumann"),
+        ),
+    ]
+    mesh = pml.Mesh([(0.0, 2.0), (0.0, 2.0)], [0.1, 0.1])
+    cp = pml.ConstrainedProblem(eq, mesh, bcs)
+    ic = pml.GaussianInitialCondition(
+        cp, [(np.array([1.0, 1.0]), 0.1 * np.eye(2))] * 2, [1.0, 0.0]
+    )
+    sol, y_ref = run_both(cp, ic, (0.0, 0.1), 0.005, "rk4")
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+@pytest.mark.parametrize("integ", ["fe", "rk4"])
+def test_cahn_hilliard_fp64(integ):
+    print(f"case: cahn-hilliard {integ}", flush=True)
+    cp, ic = cahn_hilliard(24, seed=3)
+    sol, y_ref = run_both(cp, ic, (0.0, 0.005), 1e-4, integ)
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+def test_cahn_hilliard_rk4_fp32_tolerance():
+    """FP32 CH (the batched benchmark equation), 200 steps; stated tolerance
+    1e-5 relative."""
+    print("case: cahn-hilliard fp32", flush=True)
+    cp, ic = cahn_hilliard(40, seed=5)
+    sol, y_ref = run_both(cp, ic, (0.0, 0.02), 1e-4, "rk4", precision="float32")
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-5
+
+
+@pytest.mark.parametrize("integ", ["fe", "midpoint", "rk4"])
+def test_shallow_water_fp64(integ):
+    print(f"case: shallow water {integ}", flush=True)
+    cp, ic = shallow_water(21, seed=1)
+    sol, y_ref = run_both(cp, ic, (0.0, 0.03), 0.001, integ)
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+def test_shallow_water_reference_example_bcs_fp64():
+    """The reference example's boundary set: Neumann on eta only, u and v
+    free (pararealml examples/shallow_water_fdm.py)."""
+    print("case: shallow water example bcs", flush=True)
+    eq = pml.ShallowWaterEquation(0.5)
+    mesh = pml.Mesh([(-5.0, 5.0), (0.0, 5.0)], [0.5, 0.25])
+    bc = pml.NeumannBoundaryCondition(
+        pml.vectorize_bc_function(lambda x, t: (0.0, None, None)), is_static=True
+    )
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc)] * 2)
+    ic = pml.GaussianInitialCondition(
+        cp, [(np.array([2.5, 1.25]), 0.25 * np.eye(2))] * 3, [1.0, 0.0, 0.0]
+    )
+    sol, y_ref = run_both(cp, ic, (0.0, 0.05), 0.0025, "rk4")
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+def test_convection_diffusion_and_burgers_fp64():
+    print("case: convection-diffusion / burgers 2d", flush=True)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [0.05, 0.05])
+    bc1 = zero_bc(1)
+    cp = pml.ConstrainedProblem(
+        pml.ConvectionDiffusionEquation(2, [0.3, -0.2], 0.05), mesh, [(bc1, bc1)] * 2
+    )
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.5, 0.5]), 0.02 * np.eye(2))])
+    sol, y_ref = run_both(cp, ic, (0.0, 0.02), 0.001, "rk4")
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+    bc2 = zero_bc(2)
+    cp = pml.ConstrainedProblem(pml.BurgersEquation(2, 100.0), mesh, [(bc2, bc2)] * 2)
+    ic = pml.GaussianInitialCondition(
+        cp, [(np.array([0.5, 0.5]), 0.02 * np.eye(2))] * 2, [0.5, 0.3]
+    )
+    sol, y_ref = run_both(cp, ic, (0.0, 0.02), 0.001, "midpoint")
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+@pytest.mark.parametrize("eq_name", ["diffusion", "burgers", "wave"])
+def test_one_dimensional_fp64(eq_name):
+    """1-D problems run on the line-along-axis-1 layout; Cauchy lower end,
+    Neumann upper end."""
+    print(f"case: 1-d {eq_name}", flush=True)
+    eq = {
+        "diffusion": pml.DiffusionEquation(1, 0.5),
+        "burgers": pml.BurgersEquation(1, 50.0),
+        "wave": pml.WaveEquation(1, 1.2),
+    }[eq_name]
+    n_f = eq.y_dimension
+    mesh = pml.Mesh([(0.0, 4.0)], [0.1])
+    bcs = [
+        (
+            pml.CauchyBoundaryCondition(
+                lambda x, t: np.full((len(x), n_f), 0.25),
+                lambda x, t: np.full((len(x), n_f), 0.1),
+                is_static=True,
+            ),
+            pml.NeumannBoundaryCondition(
+                lambda x, t: np.full((len(x), n_f), -0.2), is_static=True
+            ),
+        )
+    ]
+    cp = pml.ConstrainedProblem(eq, mesh, bcs)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([2.0]), np.array([[0.3]]))] * n_f)
+    sol, y_ref = run_both(cp, ic, (0.0, 0.1), 0.002, "rk4")
+    assert sol.discrete_y().shape == (50, 41, n_f)
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+def test_all_tile_configs_bitwise_identical():
+    """Every kernel variant (tile height, fused steps per launch) and the
+    graph / eager launch paths compute the same operations per vertex."""
+    print("case: tile configs", flush=True)
+    from pararealml_amd import _device as dev
+    from pararealml_amd import _native as nat
+
+    cp, ic = cahn_hilliard(70, seed=2)
+    problem = dev.device_problem(cp)
+    lib = nat.device_lib()
+    results = []
+    for prec in ("float32", "float32c"):
+        n_cfg = lib.pml_fdm_configs(nat.PML_EQ["cahn_hilliard"], nat.PML_INT_RK4, dev.precision_id(prec))
+        assert n_cfg >= 1
+        outs = []
+        for cfg in range(n_cfg):
+            for graph in (False, True):
+                b = dev.FDMBatch([problem], [0], prec, nat.PML_INT_RK4)
+                b.load_host([ic.discrete_y_0(True)])
+                b.advance(1e-4, 130, tile_config=cfg, use_graph=graph)
+                outs.append(b.state_f64().cpu().numpy())
+        for o in outs[1:]:
+            np.testing.assert_array_equal(o, outs[0])
+        results.append(outs[0])
+
+
+def test_ragged_batch_matches_per_grid():
+    """One launch over grids of different sizes equals per-grid solves."""
+    print("case: ragged batch", flush=True)
+    from pararealml_amd import _device as dev
+    from pararealml_amd import _native as nat
+
+    sizes = [16, 24, 40, 33]
+    problems, y0s = [], []
+    for i, n in enumerate(sizes):
+        cp, ic = cahn_hilliard(n, seed=10 + i)
+        problems.append(dev.device_problem(cp))
+        y0s.append(ic.discrete_y_0(True))
+    b = dev.FDMBatch(problems, list(range(len(sizes))), "float64", nat.PML_INT_RK4)
+    b.load_host(y0s)
+    b.advance(1e-4, 25)
+    flat = b.state_f64().cpu().numpy()
+    for k, n in enumerate(sizes):
+        cp, ic = cahn_hilliard(n, seed=10 + k)
+        _, y_ref = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, 25e-4), 1e-4, "rk4", keep="last")
+        got = problems[k].from_device_layout(b.grid_view(flat, k), (n, n))
+        assert rel_err(got, y_ref[0]) <= 1e-12
+
+
+def test_unsupported_cases_raise():
+    from pararealml_amd.constrained_problem import ConstrainedProblem
+
+    # ODEs and equations without fused kernels are refused, not run on the host
+    cp = ConstrainedProblem(pml.LorenzEquation())
+    ic = pml.ContinuousInitialCondition(cp, lambda _: np.ones(3))
+    ivp = pml.InitialValueProblem(cp, (0.0, 1.0), ic)
+    with pytest.raises(NotImplementedError):
+        FDMOperator(RK4(), ThreePointCentralDifferenceMethod(), 0.1).solve(ivp)
