@@ -145,6 +145,21 @@ def device_problem(cp) -> DeviceProblem:
     return DeviceProblem(name, tuple(params), n_fields, ndim, n0, n1, dx0, dx1, bc)
 
 
+_streams = {}
+
+
+def solver_stream():
+    """A dedicated (default-priority) stream per device for the solvers, so
+    that step loops can be captured in hipGraphs (capture is not possible on
+    the null stream)."""
+    import torch
+
+    dev_idx = torch.cuda.current_device()
+    if dev_idx not in _streams:
+        _streams[dev_idx] = torch.cuda.Stream(device=dev_idx)
+    return _streams[dev_idx]
+
+
 def integrator_id(integrator) -> int:
     from pararealml_amd.operators.fdm.numerical_integrator import (
         RK4,

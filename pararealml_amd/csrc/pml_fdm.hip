@@ -622,7 +622,8 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
   // same at every replay, and no snapshot may fall inside a replayed block.
   const int64_t fused_launches = n_steps / steps_per_launch;
   int64_t li = 0;
-  if (use_graph && !snapshots_hi && fused_launches >= 64) {
+  // Stream capture needs a created stream (not the null stream).
+  if (use_graph && st != nullptr && !snapshots_hi && fused_launches >= 64) {
     const int64_t block = 32;  // launches per graph (even)
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;

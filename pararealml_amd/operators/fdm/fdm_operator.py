@@ -102,27 +102,31 @@ class FDMOperator(Operator):
             )
         n_snap = n_steps // every
         y0 = ivp.initial_condition.discrete_y_0(True)
-
-        batch = dev.FDMBatch([problem], [0], self._precision, integ)
-        batch.load_host([y0])
         if n_snap == 0:
             return Solution(
                 ivp, t[1:], np.empty((0,) + cp.y_vertices_shape), vertex_oriented=True,
                 d_t=self._d_t,
             )
-        snaps = batch.new_snapshots(n_snap)
-        batch.advance(
-            self._d_t,
-            n_steps,
-            snaps,
-            every,
-            tile_config=self.tile_config_for(problem),
-            use_graph=self._use_graph,
-        )
-        flat = batch.snapshots_f64(snaps, n_snap)
-        y = problem.from_device_layout(
-            flat.cpu().numpy(), cp.mesh.vertices_shape
-        )
+
+        import torch
+
+        stream = dev.solver_stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            batch = dev.FDMBatch([problem], [0], self._precision, integ, stream)
+            batch.load_host([y0])
+            snaps = batch.new_snapshots(n_snap)
+            batch.advance(
+                self._d_t,
+                n_steps,
+                snaps,
+                every,
+                tile_config=self.tile_config_for(problem),
+                use_graph=self._use_graph,
+            )
+            flat = batch.snapshots_f64(snaps, n_snap).cpu().numpy()
+        flat = flat.reshape(n_snap, problem.n_fields, problem.n0, problem.n1)
+        y = problem.from_device_layout(flat, cp.mesh.vertices_shape)
         return Solution(
             ivp,
             t[every::every],
