@@ -1,0 +1,255 @@
+#!/usr/bin/env python
+"""Benchmark: batched FP32 FDM stepping on one MI355X.
+
+Workload (default, ``--workload ch256``): 256 Cahn-Hilliard initial
+conditions (d = 0.1, gamma = 0.01) on uneven square grids drawn uniformly
+from {64, 96, 128, 192, 256}^2 with spacing 0.1, zero-Neumann edges, c_0
+uniform in [-0.05, 0.05] from ``RandomState(541016860)``, mu_0 = c^3 - c -
+gamma lap(c); RK4 at d_t = 1e-4, all grids advanced together by the fused
+kernel in one launch per step (or per fused block of steps), the step loop
+captured in hipGraphs.  A "step" is one time step of the whole batch.
+
+Prints one JSON line with the throughput in Gcell-steps/s, the roofline of
+the stepping kernel (algorithmic bytes = 2 fields x (read + write) x 4 B =
+16 B per cell-step, against 8 TB/s), the FP32 error against an FP64 run of
+the same batch, and the CPU baseline (the NumPy oracle, FP64, one core, on a
+bounded sample of the same workload).
+"""
+
+import argparse
+import json
+import os
+import platform
+import re
+import subprocess
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# First entries of the reference's seed list (pararealml/utils/rand.py:7).
+SEEDS = (541016860, 434342745, 1462316, 746159945)
+HBM_PEAK_GBS = 8000.0
+METRIC = "FDM Gcell-steps/s (%HBM peak); Parareal time-to-1e-6, learned vs FDM coarse"
+
+
+def parse_args():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="ch256", choices=["ch256"])
+    ap.add_argument("--tile-config", type=int, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    return ap.parse_args()
+
+
+def host_info():
+    model = platform.processor() or ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                model = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return {"cpu_model": model, "cpu_count": os.cpu_count()}
+
+
+def ch_workload(n_ics=256):
+    """The 256-IC ragged Cahn-Hilliard batch (host arrays + problems)."""
+    import pararealml_amd as pml
+
+    rng = np.random.RandomState(SEEDS[0])
+    sizes = rng.choice([64, 96, 128, 192, 256], size=n_ics)
+    cps = {}
+    for n in sorted(set(int(s) for s in sizes)):
+        eq = pml.CahnHilliardEquation(2, 0.1, 0.01)
+        mesh = pml.Mesh([(0.0, 0.1 * (n - 1))] * 2, [0.1, 0.1])
+        bc = pml.NeumannBoundaryCondition(
+            lambda x, t: np.zeros((len(x), 2)), is_static=True
+        )
+        cps[n] = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    c0 = [rng.uniform(-0.05, 0.05, (int(n), int(n), 1)) for n in sizes]
+    return [int(s) for s in sizes], cps, c0
+
+
+def mu0_on_device(cps, sizes, c0):
+    """mu_0 = c^3 - c - gamma lap(c) with the device Laplacian."""
+    from pararealml_amd.operators.fdm import ThreePointCentralDifferenceMethod
+
+    diff = ThreePointCentralDifferenceMethod()
+    y0 = []
+    for n, c in zip(sizes, c0):
+        cp = cps[n]
+        lap = diff.laplacian(c, cp.mesh, cp.static_boundary_vertex_constraints[1][:, :1])
+        y0.append(np.concatenate([c, c**3 - c - 0.01 * lap], axis=-1))
+    return y0
+
+
+def cpu_baseline(cps, sizes, y0, budget_s):
+    """The NumPy oracle (FP64, the reference's op-by-op cost) on one grid of
+    each size of the batch, stepped for about ``budget_s`` seconds."""
+    from oracle import fdm_numpy as ora
+
+    picked = {}
+    for i, n in enumerate(sizes):
+        picked.setdefault(n, i)
+    cells = sum(n * n for n in picked)
+
+    def run(steps):
+        start = time.perf_counter()
+        for n, i in sorted(picked.items()):
+            ora.fdm_solve(cps[n], y0[i], (0.0, steps * 1e-4), 1e-4, "rk4", keep="last")
+        return time.perf_counter() - start
+
+    probe = run(2)
+    steps = int(max(2, min(400, 2 * budget_s / max(probe, 1e-3))))
+    elapsed = run(steps)
+    return {
+        "value": cells * steps / elapsed / 1e9,
+        "unit": "Gcell-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (
+            f"NumPy FP64 oracle (RK4, generic symbolic RHS), one "
+            f"{'/'.join(str(n) for n in sorted(picked))}^2 grid each ({cells} cells) "
+            f"x {steps} steps in {elapsed:.1f} s"
+        ),
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(
+            json.dumps(
+                {
+                    "error": "this build measures one GPU; multi-GPU runs are not part of it",
+                    "n_gpus": args.gpus,
+                }
+            )
+        )
+        sys.exit(2)
+
+    import torch
+
+    from pararealml_amd import _device as dev
+    from pararealml_amd import _native as nat
+
+    torch.cuda.set_device(0)
+    sizes, cps, c0 = ch_workload()
+    y0 = mu0_on_device(cps, sizes, c0)
+    size_list = sorted(cps)
+    problems = [dev.device_problem(cps[n]) for n in size_list]
+    grid_problem = [size_list.index(n) for n in sizes]
+    cells = sum(n * n for n in sizes)
+
+    stream = dev.solver_stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        batch = dev.FDMBatch(problems, grid_problem, "float32", nat.PML_INT_RK4, stream)
+        lib = batch.lib
+        cfg = args.tile_config
+        if cfg is None:
+            from pararealml_amd.autotune import tune
+
+            batch.load_host(y0)
+            cfg, timings = tune(batch, 1e-4, n_steps=64, repeats=2, store=False)
+            print(f"autotune: {timings} -> config {cfg}", file=sys.stderr)
+        cfg_name = lib.pml_fdm_config_name(batch.eq_id, batch.integrator, batch.prec, cfg).decode()
+        m = re.search(r"-s(\d+)$", cfg_name)
+        spl = int(m.group(1)) if m else 1
+
+        batch.load_host(y0)
+        batch.advance(1e-4, args.warmup, tile_config=cfg, use_graph=True)
+        batch.load_host(y0)  # the timed steps are the workload's first K steps
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        batch.advance(1e-4, args.steps, tile_config=cfg, use_graph=True)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        ev_ms = ev0.elapsed_time(ev1)
+
+        fp32_state = batch.state_f64().cpu().numpy()
+        err = None
+        if not args.no_fp64_check:
+            ref = dev.FDMBatch(problems, grid_problem, "float64", nat.PML_INT_RK4, stream)
+            ref.load_host(y0)
+            ref.advance(1e-4, args.steps)
+            ref_state = ref.state_f64().cpu().numpy()
+            errs = []
+            for k in range(len(sizes)):
+                a = batch.grid_view(fp32_state, k)
+                b = ref.grid_view(ref_state, k)
+                errs.append(float(np.max(np.abs(a - b)) / np.max(np.abs(b))))
+            err = max(errs)
+            del ref
+
+    launches = args.steps // spl + (args.steps % spl)
+    avg_launch_s = ev_ms / 1e3 / launches
+    bytes_per_launch = cells * 16 * spl
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    value = cells * args.steps / wall / 1e9
+
+    traffic = None
+    tpath = ROOT / "profiles" / "traffic_ch256_rk4_fp32.json"
+    if tpath.exists():
+        try:
+            tj = json.loads(tpath.read_text())
+            if tj.get("config") == cfg_name:
+                traffic = tj.get("hbm_bytes_per_launch")
+        except ValueError:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(cps, sizes, y0, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "Gcell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded uniform noise initial conditions)",
+        "config": {
+            "workload": "batched Cahn-Hilliard RK4 FP32: 256 ICs on uneven {64,96,128,192,256}^2 grids, d_t=1e-4",
+            "n_grids": len(sizes),
+            "cells": cells,
+            "kernel_variant": cfg_name,
+            "steps_per_launch": spl,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "fp32_error_vs_fp64": err,
+        "cpu_baseline": cpu,
+        "host": host_info(),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -106,16 +106,23 @@ int pml_device_sync(void *stream);
  *                       is written at element offset k*total_elems
  *   total_elems       : elements of one state plane (all grids, all fields)
  *   tile_config       : kernel variant (0 = default; see pml_fdm_configs)
- *   use_graph         : capture the step loop in hipGraphs when non-zero
+ *   flags             : PML_FLAG_GRAPH -- replay the step loop from a
+ *                       captured hipGraph (needs a non-null stream);
+ *                       PML_FLAG_CONSTRAIN_INPUT -- impose the Dirichlet
+ *                       values on the initial state before the first step
+ *                       (what the reference's initial conditions do)
  * After the call the current state is in state_hi/lo.
  */
+#define PML_FLAG_GRAPH 1
+#define PML_FLAG_CONSTRAIN_INPUT 2
+
 int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
                   int integrator_id, int precision, int ndim,
                   const pml_grid *grids, int n_grids, const double *bc_values,
                   void *state_hi, void *state_lo, void *work_hi, void *work_lo,
                   int64_t total_elems, double dt, int64_t n_steps,
                   void *snapshots_hi, void *snapshots_lo,
-                  int64_t snapshot_every, int tile_config, int use_graph,
+                  int64_t snapshot_every, int tile_config, int flags,
                   void *stream);
 
 /* Number of kernel variants for (eq, integrator, precision) and a short
@@ -189,6 +196,58 @@ int pml_restrict2x(int precision, const void *fine, void *coarse,
                    int64_t batch, int64_t m0, int64_t m1, void *stream);
 int pml_prolong2x(int precision, const void *coarse, void *fine,
                   int64_t batch, int64_t m0, int64_t m1, void *stream);
+
+/*
+ * Convolutional surrogate (no counterpart in the reference; it replaces the
+ * coarse propagator).  Tensors are NHWC BF16 unless stated; accumulation is
+ * FP32; 3x3 "same" convolutions with zero padding.
+ *   pml_conv64          : 64 -> 64 layer on the matrix cores.
+ *                         mode 0 (forward): wk = [9][64 co][64 ci] weights,
+ *                           bias FP32[64]; writes z_out (pre-activation,
+ *                           may be NULL) and a_out = GELU(z).
+ *                         mode 1 (input gradient): in = dL/dz of this
+ *                           layer, wk = [9][64 ci][64 co] with the taps
+ *                           flipped, zprev = the input's pre-activation;
+ *                           writes a_out = dL/dz of the previous layer.
+ *   pml_wgrad64         : dw[9][64 ci][64 co] += weight gradient (FP32).
+ *   pml_conv_small      : C -> 64 (C in 1..3) or 64 -> C layers, w =
+ *                         [9][ci][co]; forward writes z_out/a_out (64
+ *                         outputs) or y_out (FP32, C outputs); mode 1 is the
+ *                         input gradient of a 64 -> C layer.
+ *   pml_conv_small_wgrad: dw[9][ci][co] and db[co] += gradients (FP32).
+ *   pml_bias_grad64     : db[64] += sum over pixels of dz.
+ *   pml_adam            : Adam step on FP32 masters p/m/v with gradient g
+ *                         (step = 1, 2, ...); writes BF16 weights in the
+ *                         plain [9][ci][co], forward [9][co][ci] and
+ *                         flipped backward layouts (each may be NULL).
+ *   pml_surrogate_normalize: per-sample scale s = max|u| and x = u / s,
+ *                         u field-major [B][C][HW] FP32/FP64 -> x NHWC.
+ *   pml_surrogate_residual : out = s (x + y), back to field-major.
+ *   pml_surrogate_loss     : loss += sum (x + y - t/s)^2 (FP64 scalar) and
+ *                         dy = 2 (x + y - t/s) / n (BF16 NHWC).
+ */
+int pml_conv64(int mode, const void *in, const void *wk, const float *bias,
+               const void *zprev, void *z_out, void *a_out, int B, int H, int W,
+               void *stream);
+int pml_wgrad64(const void *x, const void *dz, float *dw, int B, int H, int W,
+                void *stream);
+int pml_conv_small(int mode, const void *in, const void *w, const float *bias,
+                   const void *zprev, void *z_out, void *a_out, float *y_out,
+                   int Ci, int Co, int B, int H, int W, void *stream);
+int pml_conv_small_wgrad(const void *x, const void *dz, float *dw, float *db,
+                         int Ci, int Co, int B, int H, int W, void *stream);
+int pml_bias_grad64(const void *dz, float *db, int64_t pixels, void *stream);
+int pml_adam(float *p, float *m, float *v, const float *g, int64_t n, float lr,
+             float beta1, float beta2, float eps, int step, void *plain,
+             void *wf, void *wb, int ci, int co, void *stream);
+int pml_surrogate_normalize(int precision, const void *u, void *x, float *s,
+                            int B, int C, int64_t HW, void *stream);
+int pml_surrogate_residual(int precision, const void *x, const float *y,
+                           const float *s, void *out, int B, int C, int64_t HW,
+                           void *stream);
+int pml_surrogate_loss(const void *x, const float *y, const float *s,
+                       const float *target, void *dy, double *loss, int B,
+                       int C, int64_t HW, void *stream);
 
 #ifdef __cplusplus
 }

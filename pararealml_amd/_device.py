@@ -301,10 +301,15 @@ class FDMBatch:
         snapshot_every: int = 0,
         tile_config: int = 0,
         use_graph: bool = False,
+        constrain_input: bool = False,
+        first_grid: int = 0,
+        n_grids: Optional[int] = None,
     ) -> None:
-        """Advances every grid by ``n_steps`` steps in place.  ``snapshots`` is
-        an optional ``(hi, lo)`` pair of device tensors receiving the batch
-        state after every ``snapshot_every``-th step."""
+        """Advances grids ``[first_grid, first_grid + n_grids)`` (default: all)
+        by ``n_steps`` steps in place.  ``snapshots`` is an optional
+        ``(hi, lo)`` pair of device tensors receiving the batch state after
+        every ``snapshot_every``-th step; ``constrain_input`` imposes the
+        Dirichlet values on the state before the first step."""
         snap_hi = snap_lo = None
         if snapshots is not None:
             snap_hi, snap_lo = snapshots
@@ -316,8 +321,8 @@ class FDMBatch:
                 self.integrator,
                 self.prec,
                 self.ndim,
-                self.grids,
-                len(self.grid_problem),
+                ctypes.byref(self.grids[first_grid]),
+                len(self.grid_problem) - first_grid if n_grids is None else n_grids,
                 self.bc.data_ptr(),
                 self._ptr(self.hi),
                 self._ptr(self.lo),
@@ -330,7 +335,8 @@ class FDMBatch:
                 self._ptr(snap_lo),
                 int(snapshot_every),
                 int(tile_config),
-                1 if use_graph else 0,
+                (nat.PML_FLAG_GRAPH if use_graph else 0)
+                | (nat.PML_FLAG_CONSTRAIN_INPUT if constrain_input else 0),
                 self._sptr(),
             )
         )

@@ -6,7 +6,7 @@ visible, every call raises.
 
 import ctypes
 import threading
-from ctypes import POINTER, c_char_p, c_double, c_int, c_int32, c_int64, c_void_p
+from ctypes import POINTER, c_char_p, c_double, c_float, c_int, c_int32, c_int64, c_void_p
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().parent / "libpml.so"
@@ -28,6 +28,9 @@ PML_EQ = {
     "wave": 4,
     "burgers": 5,
 }
+
+PML_FLAG_GRAPH = 1
+PML_FLAG_CONSTRAIN_INPUT = 2
 
 PML_INT_FE = 0
 PML_INT_MIDPOINT = 1
@@ -96,6 +99,39 @@ SIGNATURES = {
     ),
     "pml_prolong2x": (
         c_int, [c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p]
+    ),
+    "pml_conv64": (
+        c_int,
+        [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
+         c_int, c_void_p],
+    ),
+    "pml_wgrad64": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "pml_conv_small": (
+        c_int,
+        [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
+         c_int, c_int, c_int, c_int, c_void_p],
+    ),
+    "pml_conv_small_wgrad": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p],
+    ),
+    "pml_bias_grad64": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "pml_adam": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_float, c_float, c_float, c_float,
+         c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p],
+    ),
+    "pml_surrogate_normalize": (
+        c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int64, c_void_p]
+    ),
+    "pml_surrogate_residual": (
+        c_int,
+        [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int64, c_void_p],
+    ),
+    "pml_surrogate_loss": (
+        c_int,
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int64,
+         c_void_p],
     ),
 }
 

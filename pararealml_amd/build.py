@@ -28,6 +28,10 @@ COMMON_FLAGS = [
     "-Wall",
     "-Wno-unused-variable",
     "-Wno-unused-but-set-variable",
+    # No FMA contraction: every kernel variant then rounds identically, so
+    # results do not depend on the tuned variant, and FP64 results follow
+    # the unfused operation order of the NumPy reference.
+    "-ffp-contract=off",
     # Packed FP32 instructions are kept out of every kernel.
     "-Xclang",
     "-target-feature",

@@ -1,0 +1,7 @@
+// Fused stepping kernel instantiations: PrecF32C, shallow water / wave /
+// Burgers.
+#include "pml_fdm_kernels.h"
+
+namespace pml {
+PML_DEFINE_TABLE_B(fdm_table_f32c_b, PrecF32C)
+}  // namespace pml

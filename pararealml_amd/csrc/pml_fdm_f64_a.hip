@@ -1,0 +1,7 @@
+// Fused stepping kernel instantiations: PrecF64, diffusion / Cahn-Hilliard /
+// convection-diffusion.
+#include "pml_fdm_kernels.h"
+
+namespace pml {
+PML_DEFINE_TABLE_A(fdm_table_f64_a, PrecF64)
+}  // namespace pml

@@ -1,0 +1,7 @@
+// Fused stepping kernel instantiations: PrecF64, shallow water / wave /
+// Burgers.
+#include "pml_fdm_kernels.h"
+
+namespace pml {
+PML_DEFINE_TABLE_B(fdm_table_f64_b, PrecF64)
+}  // namespace pml

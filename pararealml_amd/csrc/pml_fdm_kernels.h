@@ -1,0 +1,719 @@
+// Fused finite-difference time stepping kernels for gfx950.
+//
+// Two kernel families advance a (possibly ragged) batch of 2-D grids:
+//
+// * fdm_stream -- single-stage steps (forward Euler, and any integrator for
+//   stage-invariant systems such as Cahn-Hilliard).  Barrier-free and
+//   LDS-free: each wavefront owns a 64-column strip of RW rows, each lane one
+//   column; the lane loads rows i0-1 .. i0+RW of its column into registers
+//   at once and takes its left/right neighbours from the adjacent lanes with
+//   cross-lane shuffles.  HBM sees one read and one write of the state per
+//   step; the two halo rows per strip are mostly served by L2.
+//
+// * fdm_fused -- any stage count, with STEPS steps fused per launch.  One
+//   workgroup loads an output tile plus a halo of H = stages * STEPS
+//   vertices into LDS once, evaluates every Runge-Kutta stage on a region
+//   that shrinks by one vertex per stage, and writes the output tile once.
+//
+// Boundary semantics restate pararealml/operators/fdm/numerical_differentiator.py:
+//   * first derivatives use zero halos and are then overwritten on the
+//     boundary by the Neumann values (1012-1057);
+//   * second derivatives use ghosts y[1] - 2dx g_lo / y[n-2] + 2dx g_hi where
+//     a Neumann value exists and 0 otherwise (1059-1095, 1188-1242);
+// and the Dirichlet constraints are re-imposed on every stage input and on
+// the step result (numerical_integrator.py:42-132).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "pml_common.h"
+#include "pml_equations.h"
+
+namespace pml {
+
+struct GridDev {
+  int n0, n1;
+  int64_t off;  // state element offset of field 0
+  int64_t bc;   // offset of the grid's boundary value block
+  double inv_dx0sq, inv_dx1sq;
+  double inv_2dx0, inv_2dx1;
+  double m2dx0, m2dx1;  // -2 dx (lower ghost multiplier)
+  double p2dx0, p2dx1;  // +2 dx (upper ghost multiplier)
+};
+
+// A work unit: an output tile (fdm_fused) or a strip of rows (fdm_stream)
+// with origin (i0, j0) in grid g.
+struct TileDev {
+  int g, i0, j0, pad;
+};
+
+struct FdmArgs {
+  const TileDev *tiles;
+  const GridDev *grids;
+  const double *bc;
+  const void *in_hi;
+  const void *in_lo;
+  void *out_hi;
+  void *out_lo;
+  void *snap_hi;
+  void *snap_lo;
+  double dt;
+  double p[6];
+  int n_tiles;
+  int ndim;
+  int constrain_input;  // apply the Dirichlet values to the loaded state
+};
+
+// Bijective block -> unit map that gives every XCD (blocks are dealt to the
+// 8 XCDs round robin) a contiguous range of units, so that units sharing
+// halo rows run behind the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int q = n >> 3, rem = n & 7;
+  const int x = b & 7, local = b >> 3;
+  return x * q + min(x, rem) + local;
+}
+
+template <class P>
+struct Store;
+template <>
+struct Store<PrecF64> {
+  __device__ static void load(const void *hi_p, const void *, int64_t i, double &hi, double &lo) {
+    hi = static_cast<const double *>(hi_p)[i];
+    lo = 0.0;
+  }
+  __device__ static void store(void *p_hi, void *, int64_t i, double hi, double) {
+    static_cast<double *>(p_hi)[i] = hi;
+  }
+};
+template <>
+struct Store<PrecF32> {
+  __device__ static void load(const void *hi_p, const void *, int64_t i, float &hi, float &lo) {
+    hi = static_cast<const float *>(hi_p)[i];
+    lo = 0.0f;
+  }
+  __device__ static void store(void *p_hi, void *, int64_t i, float hi, float) {
+    static_cast<float *>(p_hi)[i] = hi;
+  }
+};
+template <>
+struct Store<PrecF32C> {
+  __device__ static void load(const void *hi_p, const void *lo_p, int64_t i, float &hi,
+                              float &lo) {
+    hi = static_cast<const float *>(hi_p)[i];
+    lo = static_cast<const float *>(lo_p)[i];
+  }
+  __device__ static void store(void *p_hi, void *p_lo, int64_t i, float hi, float lo) {
+    static_cast<float *>(p_hi)[i] = hi;
+    static_cast<float *>(p_lo)[i] = lo;
+  }
+};
+
+// y <- y + d for the state precision.
+template <class P>
+__device__ __forceinline__ void state_add(typename P::C &hi, typename P::C &lo,
+                                          typename P::C d) {
+  if constexpr (P::kComp) {
+    comp_add(hi, lo, d);
+  } else {
+    hi = hi + d;
+  }
+}
+
+// Working (single-word) value of y + d without changing y.
+template <class P>
+__device__ __forceinline__ typename P::C state_plus(typename P::C hi, typename P::C lo,
+                                                    typename P::C d) {
+  if constexpr (P::kComp) {
+    return hi + (lo + d);
+  } else {
+    return hi + d;
+  }
+}
+
+template <class P>
+__device__ __forceinline__ void state_set(typename P::C &hi, typename P::C &lo, double v) {
+  if constexpr (P::kComp) {
+    split_f64(v, hi, lo);
+  } else {
+    hi = static_cast<typename P::C>(v);
+    lo = 0;
+  }
+}
+
+// Dirichlet value of field f at boundary vertex (gi, gj); NaN when free.
+__device__ __forceinline__ double dirichlet_at(const double *fb, int n0, int n1, int gi,
+                                               int gj, bool ax0) {
+  double v = NAN;
+  if (ax0) {
+    if (gi == 0) v = fb[gj];
+    else if (gi == n0 - 1) v = fb[n1 + gj];
+  }
+  if (v != v) {
+    if (gj == 0) v = fb[2 * n1 + gi];
+    else if (gj == n1 - 1) v = fb[2 * n1 + n0 + gi];
+  }
+  return v;
+}
+
+template <typename C>
+struct StencilK {
+  C inv_dx0sq, inv_dx1sq, inv_2dx0, inv_2dx1, m2dx0, m2dx1, p2dx0, p2dx1;
+  __device__ explicit StencilK(const GridDev &G)
+      : inv_dx0sq(static_cast<C>(G.inv_dx0sq)),
+        inv_dx1sq(static_cast<C>(G.inv_dx1sq)),
+        inv_2dx0(static_cast<C>(G.inv_2dx0)),
+        inv_2dx1(static_cast<C>(G.inv_2dx1)),
+        m2dx0(static_cast<C>(G.m2dx0)),
+        m2dx1(static_cast<C>(G.m2dx1)),
+        p2dx0(static_cast<C>(G.p2dx0)),
+        p2dx1(static_cast<C>(G.p2dx1)) {}
+};
+
+// Stencil quantities of one field at vertex (gi, gj) from its raw
+// neighbours yw = (gi, gj-1), ye = (gi, gj+1), yn = (gi-1, gj),
+// ys = (gi+1, gj); neighbours outside the grid are never used.  nb points at
+// the field's Neumann edge arrays.
+template <typename C, bool WL, bool WD>
+__device__ __forceinline__ void field_quant(Quant<C> &q, C yc, C yw, C ye, C yn, C ys,
+                                            bool lo1, bool hi1, bool lo0, bool hi0, bool ax0,
+                                            const double *nb, int gi, int gj, int n0, int n1,
+                                            const StencilK<C> &K) {
+  q.v = yc;
+  q.lap = 0;
+  q.d0 = 0;
+  q.d1 = 0;
+  if (!WL && !WD) return;
+  C lap1 = 0, d1 = 0, lap0 = 0, d0 = 0;
+  if (!(lo1 || hi1 || lo0 || hi0)) {
+    if (WL) lap1 = ((ye - C(2) * yc) + yw) * K.inv_dx1sq;
+    if (WD) d1 = (ye - yw) * K.inv_2dx1;
+    if (ax0) {
+      if (WL) lap0 = ((ys - C(2) * yc) + yn) * K.inv_dx0sq;
+      if (WD) d0 = (ys - yn) * K.inv_2dx0;
+    }
+  } else {
+    const C zw = lo1 ? C(0) : yw, ze = hi1 ? C(0) : ye;
+    const double g_lo1 = lo1 ? nb[2 * n1 + gi] : NAN;
+    const double g_hi1 = hi1 ? nb[2 * n1 + n0 + gi] : NAN;
+    if (WL) {
+      C gw = zw, ge = ze;
+      if (lo1) gw = (g_lo1 == g_lo1) ? ye + K.m2dx1 * static_cast<C>(g_lo1) : C(0);
+      if (hi1) ge = (g_hi1 == g_hi1) ? yw + K.p2dx1 * static_cast<C>(g_hi1) : C(0);
+      lap1 = ((ge - C(2) * yc) + gw) * K.inv_dx1sq;
+    }
+    if (WD) {
+      d1 = (ze - zw) * K.inv_2dx1;
+      if (lo1 && g_lo1 == g_lo1) d1 = static_cast<C>(g_lo1);
+      if (hi1 && g_hi1 == g_hi1) d1 = static_cast<C>(g_hi1);
+    }
+    if (ax0) {
+      const C zn = lo0 ? C(0) : yn, zs = hi0 ? C(0) : ys;
+      const double g_lo0 = lo0 ? nb[gj] : NAN;
+      const double g_hi0 = hi0 ? nb[n1 + gj] : NAN;
+      if (WL) {
+        C gn = zn, gs = zs;
+        if (lo0) gn = (g_lo0 == g_lo0) ? ys + K.m2dx0 * static_cast<C>(g_lo0) : C(0);
+        if (hi0) gs = (g_hi0 == g_hi0) ? yn + K.p2dx0 * static_cast<C>(g_hi0) : C(0);
+        lap0 = ((gs - C(2) * yc) + gn) * K.inv_dx0sq;
+      }
+      if (WD) {
+        d0 = (zs - zn) * K.inv_2dx0;
+        if (lo0 && g_lo0 == g_lo0) d0 = static_cast<C>(g_lo0);
+        if (hi0 && g_hi0 == g_hi0) d0 = static_cast<C>(g_hi0);
+      }
+    }
+  }
+  if (ax0) {
+    q.lap = lap0 + lap1;
+    q.d0 = d0;
+    q.d1 = d1;
+  } else {
+    q.lap = lap1;
+    q.d0 = d1;  // the single physical axis lies along axis 1
+    q.d1 = 0;
+  }
+}
+
+template <class Eq, int f>
+struct Need {
+  static constexpr bool kL = (Eq::kLap >> f) & 1u;
+  static constexpr bool kD = (Eq::kD >> f) & 1u;
+};
+
+// All fields' quantities; Y(f, dir) yields the raw neighbour of field f.
+template <class Eq, typename C, typename Getter>
+__device__ __forceinline__ void all_quants(Quant<C> *q, const Getter &Y, bool lo1, bool hi1,
+                                           bool lo0, bool hi0, bool ax0, const double *bcg,
+                                           int64_t fstride, int gi, int gj, int n0, int n1,
+                                           const StencilK<C> &K) {
+  const int64_t noff = 2 * static_cast<int64_t>(n0 + n1);
+#define PML_FQ(F_)                                                                        \
+  if constexpr (F_ < Eq::F) {                                                             \
+    field_quant<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD>(                                   \
+        q[F_], Y(F_, 0), Y(F_, 1), Y(F_, 2), Y(F_, 3), Y(F_, 4), lo1, hi1, lo0, hi0, ax0, \
+        bcg + F_ * fstride + noff, gi, gj, n0, n1, K);                                    \
+  }
+  PML_FQ(0)
+  PML_FQ(1)
+  PML_FQ(2)
+#undef PML_FQ
+}
+
+// ---------------------------------------------------------------------------
+// fdm_stream: single-stage steps, one wavefront per 64-column strip of RW
+// rows.  Every lane issues all its loads up front (RW + 2 rows of its column
+// per field, plus the strip-edge halo values on lanes 0 and 63), then
+// computes its RW cells from registers and cross-lane shuffles.
+template <class Eq, class Integ, class P, int RW>
+__global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
+  using C = typename P::C;
+  constexpr int F = Eq::F;
+  static_assert(Eq::kInvariant || Integ::kStages == 1, "single-stage steps only");
+  const int lane = threadIdx.x & 63;
+  const int unit = xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  if (unit >= a.n_tiles) return;  // whole wavefront: no barriers follow
+  const TileDev U = a.tiles[unit];
+  const GridDev G = a.grids[U.g];
+  const int n0 = G.n0, n1 = G.n1;
+  const int j = U.j0 + lane;
+  const bool act = j < n1;
+  const bool ax0 = a.ndim == 2;
+  const int plane = n0 * n1;
+  const double *bcg = a.bc + G.bc;
+  const int fstride = 4 * (n0 + n1);
+  const StencilK<C> K(G);
+  C pc[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) pc[i] = static_cast<C>(a.p[i]);
+  const C dt = static_cast<C>(a.dt);
+  const int i0 = U.i0;
+  const int hj = lane == 0 ? j - 1 : j + 1;  // halo column of lanes 0 / 63
+  const bool halo_lane = (lane == 0 || lane == 63) && hj >= 0 && hj < n1;
+
+  using S = typename P::C;
+  const S *in_hi = static_cast<const S *>(a.in_hi) + G.off;
+  const S *in_lo = P::kComp ? static_cast<const S *>(a.in_lo) + G.off : nullptr;
+
+  // rows i0-1 .. i0+RW of this lane's column; lo parts of rows i0..i0+RW-1;
+  // halo values of rows i0..i0+RW-1.
+  C v[RW + 2][F], vl[RW][F], h[RW][F];
+#pragma unroll
+  for (int r = 0; r < RW + 2; ++r) {
+    const int i = i0 - 1 + r;
+    const bool in_row = i >= 0 && i < n0;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      C x = 0;
+      if (act && in_row) x = in_hi[f * plane + i * n1 + j];
+      v[r][f] = x;
+      if (r >= 1 && r <= RW) {
+        C xl = 0, hv = 0;
+        if (P::kComp && act && in_row) xl = in_lo[f * plane + i * n1 + j];
+        if (halo_lane && in_row) hv = in_hi[f * plane + i * n1 + hj];
+        vl[r - 1][f] = xl;
+        h[r - 1][f] = hv;
+      }
+    }
+  }
+  if (a.constrain_input) {
+    // Impose the Dirichlet values on every loaded boundary vertex.
+#pragma unroll
+    for (int r = 0; r < RW + 2; ++r) {
+      const int i = i0 - 1 + r;
+      if (i < 0 || i >= n0) continue;
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        if (act && (j == 0 || j == n1 - 1 || (ax0 && (i == 0 || i == n0 - 1)))) {
+          const double dv = dirichlet_at(bcg + f * fstride, n0, n1, i, j, ax0);
+          if (dv == dv) {
+            if (r >= 1 && r <= RW) state_set<P>(v[r][f], vl[r - 1][f], dv);
+            else v[r][f] = static_cast<C>(dv);
+          }
+        }
+        if (r >= 1 && r <= RW && halo_lane &&
+            (hj == 0 || hj == n1 - 1 || (ax0 && (i == 0 || i == n0 - 1)))) {
+          const double dv = dirichlet_at(bcg + f * fstride, n0, n1, i, hj, ax0);
+          if (dv == dv) h[r - 1][f] = static_cast<C>(dv);
+        }
+      }
+    }
+  }
+
+  S *out_hi = static_cast<S *>(a.out_hi) + G.off;
+  S *out_lo = P::kComp ? static_cast<S *>(a.out_lo) + G.off : nullptr;
+  S *sn_hi = a.snap_hi ? static_cast<S *>(a.snap_hi) + G.off : nullptr;
+  S *sn_lo = (a.snap_hi && P::kComp) ? static_cast<S *>(a.snap_lo) + G.off : nullptr;
+  const bool lo1 = j == 0, hi1 = j == n1 - 1;
+#pragma unroll
+  for (int r = 1; r <= RW; ++r) {
+    const int i = i0 - 1 + r;
+    C lf[F], rt[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      lf[f] = __shfl_up(v[r][f], 1, 64);
+      rt[f] = __shfl_down(v[r][f], 1, 64);
+      if (lane == 0) lf[f] = h[r - 1][f];
+      if (lane == 63) rt[f] = h[r - 1][f];
+    }
+    if (!act || i >= n0) continue;
+    const bool lo0 = ax0 && i == 0, hi0 = ax0 && i == n0 - 1;
+    Quant<C> q[F];
+    auto Y = [&](int f, int d) -> C {
+      switch (d) {
+        case 0: return v[r][f];
+        case 1: return lf[f];
+        case 2: return rt[f];
+        case 3: return v[r - 1][f];
+        default: return v[r + 1][f];
+      }
+    };
+    all_quants<Eq, C>(q, Y, lo1, hi1, lo0, hi0, ax0, bcg, fstride, i, j, n0, n1, K);
+    C rr[F], ra[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) rr[f] = ra[f] = 0;
+    Eq::template rhs<C>(pc, q, rr);
+    if (Eq::kAlg != 0u) Eq::template alg<C>(pc, q, ra);
+    const bool edge = lo1 || hi1 || lo0 || hi0;
+    const int e = i * n1 + j;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      C hv = v[r][f], lv = vl[r - 1][f];
+      if ((Eq::kAlg >> f) & 1u) {
+        hv = ra[f];
+        lv = 0;
+      } else {
+        const C kk = dt * rr[f];
+        C d = kk;
+        if (Eq::kInvariant && Integ::kStages == 4) d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
+        state_add<P>(hv, lv, d);
+      }
+      if (edge) {
+        const double dv = dirichlet_at(bcg + f * fstride, n0, n1, i, j, ax0);
+        if (dv == dv) state_set<P>(hv, lv, dv);
+      }
+      out_hi[f * plane + e] = hv;
+      if (P::kComp) out_lo[f * plane + e] = lv;
+      if (sn_hi) {
+        sn_hi[f * plane + e] = hv;
+        if (P::kComp) sn_lo[f * plane + e] = lv;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fdm_fused: LDS tiles with shrinking stage regions, STEPS steps per launch.
+template <class Eq, class Integ, class P, int OX, int OY, int STEPS, int NT>
+__global__ __launch_bounds__(NT) void fdm_fused(FdmArgs a) {
+  using C = typename P::C;
+  constexpr int F = Eq::F;
+  constexpr int S = Eq::kInvariant ? 1 : Integ::kStages;
+  constexpr int H = S * STEPS;
+  constexpr int EX = OX + 2 * H;
+  constexpr int EY = OY + 2 * H;
+  constexpr int CELLS = EX * EY;
+  constexpr int PER = (CELLS + NT - 1) / NT;
+
+  __shared__ C L[F * CELLS];
+
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= a.n_tiles) return;
+  const TileDev T = a.tiles[tile];
+  const GridDev G = a.grids[T.g];
+  const int n0 = G.n0, n1 = G.n1;
+  const int64_t plane = static_cast<int64_t>(n0) * n1;
+  const int bi = T.i0 - H, bj = T.j0 - H;
+  const bool ax0 = a.ndim == 2;
+  const double *bcg = a.bc + G.bc;
+  const int64_t fstride = 4 * static_cast<int64_t>(n0 + n1);
+  const StencilK<C> K(G);
+
+  C pc[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) pc[i] = static_cast<C>(a.p[i]);
+  const C dt = static_cast<C>(a.dt);
+  const C hdt = static_cast<C>(a.dt * 0.5);
+
+  C yh[PER][F], yl[PER][F], kacc[PER][F], kn[PER][F], an[PER][F];
+
+  // ---- load the extended tile -------------------------------------------
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int idx = threadIdx.x + k * NT;
+    if (idx < CELLS) {
+      const int r = idx / EX, c = idx - (idx / EX) * EX;
+      const int gi = bi + r, gj = bj + c;
+      const bool in = gi >= 0 && gi < n0 && gj >= 0 && gj < n1;
+      const bool edge = gj == 0 || gj == n1 - 1 || (ax0 && (gi == 0 || gi == n0 - 1));
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        C hi = 0, lo = 0;
+        if (in) {
+          Store<P>::load(a.in_hi, a.in_lo, G.off + f * plane + static_cast<int64_t>(gi) * n1 + gj,
+                         hi, lo);
+          if (a.constrain_input && edge) {
+            const double dv = dirichlet_at(bcg + f * fstride, n0, n1, gi, gj, ax0);
+            if (dv == dv) state_set<P>(hi, lo, dv);
+          }
+        }
+        yh[k][f] = hi;
+        yl[k][f] = lo;
+        L[f * CELLS + idx] = hi;
+      }
+    }
+  }
+  __syncthreads();
+
+  for (int step = 0; step < STEPS; ++step) {
+#pragma unroll
+    for (int stage = 0; stage < S; ++stage) {
+      const int hh = H - (step * S + stage + 1);
+      const int rlo = H - hh, rhi = H + OY + hh, clo = H - hh, chi = H + OX + hh;
+      // ---- evaluate the right-hand side on the stage region ------------
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int idx = threadIdx.x + k * NT;
+        const int r = idx / EX, c = idx - (idx / EX) * EX;
+        const int gi = bi + r, gj = bj + c;
+        if (idx < CELLS && r >= rlo && r < rhi && c >= clo && c < chi && gi >= 0 &&
+            gi < n0 && gj >= 0 && gj < n1) {
+          const bool lo1 = gj == 0, hi1 = gj == n1 - 1;
+          const bool lo0 = ax0 && gi == 0, hi0 = ax0 && gi == n0 - 1;
+          Quant<C> q[F];
+          // Out-of-grid neighbours are never read (edge flags select ghosts).
+          auto Y = [&](int f, int d) -> C {
+            const C *Lf = L + f * CELLS;
+            switch (d) {
+              case 0: return Lf[idx];
+              case 1: return lo1 ? C(0) : Lf[idx - 1];
+              case 2: return hi1 ? C(0) : Lf[idx + 1];
+              case 3: return (ax0 && !lo0) ? Lf[idx - EX] : C(0);
+              default: return (ax0 && !hi0) ? Lf[idx + EX] : C(0);
+            }
+          };
+          all_quants<Eq, C>(q, Y, lo1, hi1, lo0, hi0, ax0, bcg, fstride, gi, gj, n0, n1, K);
+          C rr[F];
+#pragma unroll
+          for (int f = 0; f < F; ++f) rr[f] = 0;
+          Eq::template rhs<C>(pc, q, rr);
+          const C w = (!Eq::kInvariant && Integ::kStages == 2 && stage == 0) ? hdt : dt;
+#pragma unroll
+          for (int f = 0; f < F; ++f) kn[k][f] = w * rr[f];
+          if (Eq::kAlg != 0u && stage == 0) {
+            C ra[F];
+#pragma unroll
+            for (int f = 0; f < F; ++f) ra[f] = 0;
+            Eq::template alg<C>(pc, q, ra);
+#pragma unroll
+            for (int f = 0; f < F; ++f) an[k][f] = ra[f];
+          }
+        }
+      }
+      __syncthreads();
+      // ---- stage update ------------------------------------------------
+#pragma unroll
+      for (int k = 0; k < PER; ++k) {
+        const int idx = threadIdx.x + k * NT;
+        const int r = idx / EX, c = idx - (idx / EX) * EX;
+        const int gi = bi + r, gj = bj + c;
+        if (idx < CELLS && r >= rlo && r < rhi && c >= clo && c < chi && gi >= 0 &&
+            gi < n0 && gj >= 0 && gj < n1) {
+          const bool edge = gj == 0 || gj == n1 - 1 || (ax0 && (gi == 0 || gi == n0 - 1));
+          const bool last = stage == S - 1;
+#pragma unroll
+          for (int f = 0; f < F; ++f) {
+            const bool alg = (Eq::kAlg >> f) & 1u;
+            C nv;  // new LDS value (stage input, or the step result)
+            if (alg) {
+              if (last) {
+                yh[k][f] = an[k][f];
+                yl[k][f] = 0;
+              }
+              nv = yh[k][f];
+            } else if (Eq::kInvariant) {
+              const C kk = kn[k][f];
+              C d = kk;
+              if (Integ::kStages == 4) d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
+              state_add<P>(yh[k][f], yl[k][f], d);
+              nv = yh[k][f];
+            } else if (Integ::kStages == 1) {
+              state_add<P>(yh[k][f], yl[k][f], kn[k][f]);
+              nv = yh[k][f];
+            } else if (Integ::kStages == 2) {
+              if (stage == 0) {
+                nv = state_plus<P>(yh[k][f], yl[k][f], kn[k][f]);
+              } else {
+                state_add<P>(yh[k][f], yl[k][f], kn[k][f]);
+                nv = yh[k][f];
+              }
+            } else {  // RK4
+              const C kk = kn[k][f];
+              if (stage == 0) {
+                kacc[k][f] = kk;
+                nv = state_plus<P>(yh[k][f], yl[k][f], kk / C(2));
+              } else if (stage == 1) {
+                kacc[k][f] = kacc[k][f] + C(2) * kk;
+                nv = state_plus<P>(yh[k][f], yl[k][f], kk / C(2));
+              } else if (stage == 2) {
+                kacc[k][f] = kacc[k][f] + C(2) * kk;
+                nv = state_plus<P>(yh[k][f], yl[k][f], kk);
+              } else {
+                kacc[k][f] = kacc[k][f] + kk;
+                state_add<P>(yh[k][f], yl[k][f], kacc[k][f] / C(6));
+                nv = yh[k][f];
+              }
+            }
+            if (edge) {
+              const double dv = dirichlet_at(bcg + f * fstride, n0, n1, gi, gj, ax0);
+              if (dv == dv) {
+                if (last || alg) {
+                  state_set<P>(yh[k][f], yl[k][f], dv);
+                  nv = yh[k][f];
+                } else {
+                  nv = static_cast<C>(dv);
+                }
+              }
+            }
+            L[f * CELLS + idx] = nv;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- write the output tile ----------------------------------------------
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const int idx = threadIdx.x + k * NT;
+    const int r = idx / EX, c = idx - (idx / EX) * EX;
+    const int gi = bi + r, gj = bj + c;
+    if (idx < CELLS && r >= H && r < H + OY && c >= H && c < H + OX && gi < n0 && gj < n1) {
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        const int64_t e = G.off + f * plane + static_cast<int64_t>(gi) * n1 + gj;
+        Store<P>::store(a.out_hi, a.out_lo, e, yh[k][f], yl[k][f]);
+        if (a.snap_hi) Store<P>::store(a.snap_hi, a.snap_lo, e, yh[k][f], yl[k][f]);
+      }
+    }
+  }
+}
+
+// ===========================================================================
+// Kernel variant tables.
+
+using KernelFn = void (*)(FdmArgs);
+
+enum VariantKind { kTileKind = 0, kStreamKind = 1 };
+
+struct Variant {
+  KernelFn fn;
+  KernelFn fn_single;  // one step per launch, same unit shape
+  int kind;            // kTileKind or kStreamKind
+  int ox, oy;          // unit shape: output tile (tile kind) or 64 x RW strip
+  int steps, nt;
+  const char *name;
+};
+
+#define PML_TILE(EQ, IN, PR, OX, OY, ST, NT, NAME)                                     \
+  Variant {                                                                             \
+    &fdm_fused<EQ, IN, PR, OX, OY, ST, NT>, &fdm_fused<EQ, IN, PR, OX, OY, 1, NT>,      \
+        kTileKind, OX, OY, ST, NT, NAME                                                 \
+  }
+#define PML_STREAM(EQ, IN, PR, RW, NAME)                                                \
+  Variant {                                                                             \
+    &fdm_stream<EQ, IN, PR, RW>, &fdm_stream<EQ, IN, PR, RW>, kStreamKind, 64, RW, 1, 256, \
+        NAME                                                                            \
+  }
+
+// Variants per (equation, integrator, precision); the first is the default.
+template <class EQ, class IN, class PR>
+const std::vector<Variant> &variants_for() {
+  static const std::vector<Variant> v = [] {
+    std::vector<Variant> out;
+    constexpr int S = EQ::kInvariant ? 1 : IN::kStages;
+    constexpr bool wide = sizeof(typename PR::C) == 8;
+    if constexpr (S == 1) {
+      out.push_back(PML_STREAM(EQ, IN, PR, 8, "stream-rw8"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 4, "stream-rw4"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 16, "stream-rw16"));
+      if constexpr (wide) {
+        out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
+      } else {
+        out.push_back(PML_TILE(EQ, IN, PR, 64, 16, 1, 256, "tile-ox64-oy16-s1"));
+        if constexpr (EQ::F <= 2) {
+          out.push_back(PML_TILE(EQ, IN, PR, 64, 16, 2, 256, "tile-ox64-oy16-s2"));
+          out.push_back(PML_TILE(EQ, IN, PR, 64, 32, 4, 512, "tile-ox64-oy32-s4"));
+        }
+      }
+    } else if constexpr (wide) {
+      out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
+    } else {
+      out.push_back(PML_TILE(EQ, IN, PR, 64 - 2 * S, 16, 1, 256, "tile-ox64m-oy16-s1"));
+      out.push_back(PML_TILE(EQ, IN, PR, 64 - 2 * S, 32, 1, 256, "tile-ox64m-oy32-s1"));
+    }
+    return out;
+  }();
+  return v;
+}
+
+template <class EQ, class PR>
+const std::vector<Variant> *variants_by_int(int integ) {
+  switch (integ) {
+    case PML_INT_FE: return &variants_for<EQ, IntFE, PR>();
+    case PML_INT_MIDPOINT: return &variants_for<EQ, IntMidpoint, PR>();
+    case PML_INT_RK4: return &variants_for<EQ, IntRK4, PR>();
+    default: return nullptr;
+  }
+}
+
+// Defined in the per-precision translation units (pml_fdm_<prec>_<group>.hip).
+// Each returns the variant table for (eq, integrator, ndim) and the field
+// count, or nullptr when that unit does not cover the equation.
+#define PML_DECLARE_TABLE(NAME) \
+  const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields);
+PML_DECLARE_TABLE(fdm_table_f32_a)
+PML_DECLARE_TABLE(fdm_table_f32_b)
+PML_DECLARE_TABLE(fdm_table_f64_a)
+PML_DECLARE_TABLE(fdm_table_f64_b)
+PML_DECLARE_TABLE(fdm_table_f32c_a)
+PML_DECLARE_TABLE(fdm_table_f32c_b)
+#undef PML_DECLARE_TABLE
+
+// Group a: diffusion, Cahn-Hilliard, convection-diffusion.
+// Group b: shallow water, wave, Burgers.
+#define PML_DEFINE_TABLE_A(NAME, PR)                                                   \
+  const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields) {       \
+    (void)ndim;                                                                        \
+    switch (eq) {                                                                      \
+      case PML_EQ_DIFFUSION: *n_fields = 1; return variants_by_int<EqDiffusion, PR>(integ); \
+      case PML_EQ_CAHN_HILLIARD:                                                       \
+        *n_fields = 2;                                                                 \
+        return variants_by_int<EqCahnHilliard, PR>(integ);                             \
+      case PML_EQ_CONVECTION_DIFFUSION:                                                \
+        *n_fields = 1;                                                                 \
+        return variants_by_int<EqConvectionDiffusion, PR>(integ);                      \
+      default: return nullptr;                                                         \
+    }                                                                                  \
+  }
+#define PML_DEFINE_TABLE_B(NAME, PR)                                                   \
+  const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields) {       \
+    switch (eq) {                                                                      \
+      case PML_EQ_SHALLOW_WATER:                                                       \
+        *n_fields = 3;                                                                 \
+        return ndim == 2 ? variants_by_int<EqShallowWater, PR>(integ) : nullptr;       \
+      case PML_EQ_WAVE: *n_fields = 2; return variants_by_int<EqWave, PR>(integ);      \
+      case PML_EQ_BURGERS:                                                             \
+        if (ndim == 1) {                                                               \
+          *n_fields = 1;                                                               \
+          return variants_by_int<EqBurgers1, PR>(integ);                               \
+        }                                                                              \
+        *n_fields = 2;                                                                 \
+        return variants_by_int<EqBurgers2, PR>(integ);                                 \
+      default: return nullptr;                                                         \
+    }                                                                                  \
+  }
+
+}  // namespace pml

@@ -1,0 +1,224 @@
+"""Parareal on one GPU.
+
+``PararealOperator(f, g, termination_condition, max_iterations)`` keeps the
+reference's interface and algorithm (pararealml/operators/parareal/
+parareal_operator.py:13-197) with the time slices as the batch axis of one
+device tensor instead of MPI ranks:
+
+1. the slice length is ``(t1 - t0) / n_slices`` and must be a multiple of
+   both operators' time steps (the reference's ``ValueError`` checks);
+2. the coarse operator sweeps the slices once from the initial value;
+3. each iteration i runs the fine solves of slices i..N-1 together (slices
+   before i have unchanged initial values, so their fine results would be
+   identical), forms the FP64 corrections ``F_end - G_end``, then sweeps
+   slices i..N-1 serially: ``G_end[j] = g(U[j])`` for j > i and
+   ``U[j+1] = G_end[j] + corr[j]``, accumulating the per-field squared
+   updates;
+4. it stops when, for every field, the maximum over slices of the RMS
+   update is below the tolerance (or a callable ``termination_condition``
+   says so), or after ``min(n_slices, max_iterations)`` iterations;
+5. the output is each slice's last fine trajectory shifted by
+   ``U[r+1] - F_r[-1]``.
+
+The border points U, the coarse end points, the corrections and the update
+norms are FP64 in HBM; the fine and coarse states use their operators'
+precisions.  ``n_slices`` replaces the MPI communicator size;
+``output="endpoints"`` returns only the slice end points (the full fine
+trajectory is infeasible for long runs).  The iteration count and timings
+of the last solve are in ``last_iterations`` and ``last_stats``.
+"""
+
+import sys
+import time
+from typing import Callable, Optional, Sequence, Union
+
+import numpy as np
+
+from pararealml_amd import _device as dev
+from pararealml_amd import _native as nat
+from pararealml_amd.initial_value_problem import InitialValueProblem
+from pararealml_amd.operator import Operator, discretize_time_domain
+from pararealml_amd.solution import Solution
+
+TerminationCondition = Union[
+    float, Sequence[float], Callable[[np.ndarray, np.ndarray], bool]
+]
+
+
+class PararealOperator(Operator):
+    """A parallel-in-time solver based on the Parareal algorithm."""
+
+    def __init__(
+        self,
+        f: Operator,
+        g,
+        termination_condition: TerminationCondition = None,
+        max_iterations: int = sys.maxsize,
+        *,
+        n_slices: int = 1,
+        output: str = "trajectory",
+    ):
+        super().__init__(f.d_t, f.vertex_oriented)
+        if n_slices < 1:
+            raise ValueError("n_slices must be at least 1")
+        if output not in ("trajectory", "endpoints"):
+            raise ValueError("output must be 'trajectory' or 'endpoints'")
+        self._f = f
+        self._g = g
+        self._termination_condition = termination_condition
+        self._max_iterations = max_iterations
+        self._n_slices = int(n_slices)
+        self._output = output
+        self.last_iterations: Optional[int] = None
+        self.last_stats: dict = {}
+
+    @property
+    def n_slices(self) -> int:
+        return self._n_slices
+
+    # ------------------------------------------------------------ termination
+    def _tolerances(self, y_dim: int) -> np.ndarray:
+        tc = self._termination_condition
+        if isinstance(tc, Sequence):
+            if len(tc) != y_dim:
+                raise ValueError(
+                    f"length of update tolerances ({len(tc)}) must match number "
+                    f"of y dimensions ({y_dim})"
+                )
+            return np.array(tc, dtype=float)
+        return np.array([tc] * y_dim, dtype=float)
+
+    # ------------------------------------------------------------------ solve
+    def solve(self, ivp: InitialValueProblem, parallel_enabled: bool = True) -> Solution:
+        if not parallel_enabled:
+            return self._f.solve(ivp)
+        import torch
+
+        f, g = self._f, self._g
+        n = self._n_slices
+        t0, t1 = ivp.t_interval
+        delta = (t1 - t0) / n
+        if not np.isclose(delta, f.d_t * round(delta / f.d_t)):
+            raise ValueError(
+                f"fine operator time step size ({f.d_t}) must be a divisor of "
+                f"sub-IVP time slice length ({delta})"
+            )
+        if not np.isclose(delta, g.d_t * round(delta / g.d_t)):
+            raise ValueError(
+                f"coarse operator time step size ({g.d_t}) must be a divisor of "
+                f"sub-IVP time slice length ({delta})"
+            )
+        cp = ivp.constrained_problem
+        tc = self._termination_condition
+        if not callable(tc):
+            tols = self._tolerances(cp.differential_equation.y_dimension)
+        f_steps = int(round(delta / f.d_t))
+        g_steps = int(round(delta / g.d_t))
+        borders = np.linspace(t0, t1, n + 1)
+
+        stream = dev.solver_stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        timings = {"fine_s": 0.0, "coarse_s": 0.0, "correction_s": 0.0}
+        with torch.cuda.stream(stream):
+            fine = f.slice_propagator(
+                cp, n, f_steps, stream, record_trajectory=self._output == "trajectory"
+            )
+            coarse = g.slice_propagator(cp, n, g_steps, stream)
+            problem = fine.problem
+            elems = problem.elems
+            plane = problem.plane
+            n_fields = problem.n_fields
+            lib = fine.batch.lib
+            sp = fine._sptr()
+            f64 = dict(dtype=torch.float64, device="cuda")
+
+            y0 = ivp.initial_condition.discrete_y_0(True)
+            u = torch.empty((n + 1, elems), **f64)
+            u[0] = torch.from_numpy(problem.to_device_layout(y0).ravel()).to("cuda")
+            g_end = torch.empty((n, elems), **f64)
+            f_end = torch.empty((n, elems), **f64)
+            corr = torch.empty((n, elems), **f64)
+            sumsq = torch.zeros((n, n_fields), **f64)
+
+            def sync_time(key, fn):
+                stream.synchronize()
+                start = time.perf_counter()
+                fn()
+                stream.synchronize()
+                timings[key] += time.perf_counter() - start
+
+            # initial coarse sweep
+            def first_sweep():
+                for j in range(n):
+                    coarse.run(u[j : j + 1], j, j + 1, g_end[j : j + 1])
+                    u[j + 1].copy_(g_end[j])
+
+            sync_time("coarse_s", first_sweep)
+
+            iterations = 0
+            norms_hist = []
+            for i in range(min(n, self._max_iterations)):
+                iterations = i + 1
+                sync_time("fine_s", lambda: fine.run(u[i:n], i, n, f_end[i:n]))
+
+                def correct():
+                    nat.check(
+                        lib.pml_parareal_correction(
+                            nat.PML_F64, f_end[i].data_ptr(), None, elems,
+                            g_end[i].data_ptr(), corr[i].data_ptr(), elems, n - i, sp,
+                        )
+                    )
+
+                sync_time("correction_s", correct)
+                old = u[1:].clone() if callable(tc) else None
+                sumsq.zero_()
+
+                def sweep():
+                    for j in range(i, n):
+                        if j > i:
+                            coarse.run(u[j : j + 1], j, j + 1, g_end[j : j + 1])
+                        nat.check(
+                            lib.pml_parareal_update(
+                                g_end[j].data_ptr(), corr[j].data_ptr(),
+                                u[j + 1].data_ptr(), elems, n_fields,
+                                sumsq[j].data_ptr(), sp,
+                            )
+                        )
+
+                sync_time("coarse_s", sweep)
+                if callable(tc):
+                    layout = lambda t: problem.from_device_layout(  # noqa: E731
+                        t.cpu().numpy().reshape(n, n_fields, problem.n0, problem.n1),
+                        cp.mesh.vertices_shape,
+                    )
+                    if tc(layout(old), layout(u[1:])):
+                        break
+                else:
+                    rms = np.sqrt(sumsq.cpu().numpy() / plane)
+                    max_norms = rms.max(axis=0)
+                    norms_hist.append(max_norms.tolist())
+                    if np.all(max_norms < tols):
+                        break
+
+            self.last_iterations = iterations
+            if self._output == "endpoints":
+                shifted = (f_end + (u[1:] - f_end)).cpu().numpy()
+                y = problem.from_device_layout(
+                    shifted.reshape(n, n_fields, problem.n0, problem.n1),
+                    cp.mesh.vertices_shape,
+                )
+                t = borders[1:]
+                d_t_out = delta
+            else:
+                traj = fine.trajectory_f64()  # (steps, n, elems)
+                shift = (u[1:] - traj[-1]).unsqueeze(0)
+                traj = (traj + shift).permute(1, 0, 2).reshape(n * f_steps, elems)
+                y = problem.from_device_layout(
+                    traj.cpu().numpy().reshape(n * f_steps, n_fields, problem.n0, problem.n1),
+                    cp.mesh.vertices_shape,
+                )
+                t = discretize_time_domain(ivp.t_interval, f.d_t)[1:]
+                d_t_out = f.d_t
+        torch.cuda.current_stream().wait_stream(stream)
+        self.last_stats = dict(timings, iterations=iterations, update_norms=norms_hist)
+        return Solution(ivp, t, y, vertex_oriented=self._vertex_oriented, d_t=d_t_out)

@@ -1,0 +1,153 @@
+"""Device slice propagators used by the one-GPU Parareal driver.
+
+A propagator advances initial values given as FP64 device tensors in the
+fine problem's field-major layout over whole time slices, for a range of
+slices at once:
+
+* ``FDMSlicePropagator`` -- an ``FDMOperator`` on the problem's own grid;
+  all requested slices form one batch advanced by a single
+  ``pml_fdm_solve`` call (the fine solves of every slice in one launch per
+  step).  Optionally records every step's state for the trajectory output.
+* ``CoarseGridPropagator`` -- an ``FDMOperator`` on the 2x-coarsened grid:
+  injection to the coarse grid, stepping there, bilinear prolongation back
+  (``pml_restrict2x`` / ``pml_prolong2x``).  The grid transfer has no
+  counterpart in the reference; injection followed by prolongation is exact
+  on the coarse points, so a chain of per-slice coarse solves equals one
+  continuous coarse solve, as the reference's initial coarse sweep assumes.
+
+Every slice's initial value has the problem's static Dirichlet values
+imposed before its first step, as ``DiscreteInitialCondition`` does for
+each sub-problem of the reference's Parareal (parareal_operator.py:153-163).
+"""
+
+import ctypes
+from typing import Optional
+
+import numpy as np
+
+from pararealml_amd import _device as dev
+from pararealml_amd import _native as nat
+
+
+class FDMSlicePropagator:
+    """Slices of one FDM problem, batched."""
+
+    def __init__(self, op, cp, n_slices: int, slice_steps: int, stream,
+                 record_trajectory: bool = False):
+        import torch
+
+        self.op = op
+        self.problem = op.device_problem(cp)
+        self.n_slices = n_slices
+        self.steps = slice_steps
+        self.stream = stream
+        self.batch = dev.FDMBatch(
+            [self.problem], [0] * n_slices, op.precision, dev.integrator_id(op.integrator),
+            stream,
+        )
+        self.elems = self.problem.elems
+        self.tile_config = op.tile_config_for(self.problem)
+        self.snaps = None
+        if record_trajectory:
+            self.snaps = self.batch.new_snapshots(slice_steps)
+        self._esz = 8 if self.batch.prec == nat.PML_F64 else 4
+        self.torch = torch
+
+    def _sptr(self):
+        return ctypes.c_void_p(self.stream.cuda_stream)
+
+    def run(self, u_f64, first: int, last: int, out_f64):
+        """Advances slices [first, last) from ``u_f64[k]`` (k = 0 .. last -
+        first - 1, FP64, shape (count, elems)) and writes the end states to
+        ``out_f64`` (same shape)."""
+        count = last - first
+        if count <= 0:
+            return
+        b = self.batch
+        off = first * self.elems
+        lib = b.lib
+        hi = b.hi.data_ptr() + off * self._esz
+        lo = None if b.lo is None else b.lo.data_ptr() + off * self._esz
+        nat.check(lib.pml_from_f64(b.prec, u_f64.data_ptr(), hi, lo, count * self.elems,
+                                   self._sptr()))
+        b.advance(
+            self.op.d_t,
+            self.steps,
+            snapshots=self.snaps,
+            snapshot_every=1 if self.snaps is not None else 0,
+            tile_config=self.tile_config,
+            use_graph=True,
+            constrain_input=True,
+            first_grid=first,
+            n_grids=count,
+        )
+        nat.check(lib.pml_to_f64(b.prec, hi, lo, out_f64.data_ptr(), count * self.elems,
+                                 self._sptr()))
+
+    def trajectory_f64(self):
+        """Recorded states, FP64, shape (steps, n_slices, elems)."""
+        if self.snaps is None:
+            raise RuntimeError("trajectory was not recorded")
+        flat = self.batch.snapshots_f64(self.snaps, self.steps)
+        return flat.view(self.steps, self.n_slices, self.elems)
+
+
+class CoarseGridPropagator:
+    """An FDM operator on the 2x-coarsened vertex grid of the problem."""
+
+    def __init__(self, op, cp, n_slices: int, slice_steps: int, stream):
+        import torch
+
+        from pararealml_amd.constrained_problem import ConstrainedProblem
+        from pararealml_amd.mesh import Mesh
+
+        mesh = cp.mesh
+        if mesh.dimensions != 2:
+            raise NotImplementedError("grid coarsening covers 2-D problems")
+        if any((n - 1) % 2 for n in mesh.vertices_shape):
+            raise ValueError(
+                "coarsening needs an odd number of vertices (2m - 1) along every axis"
+            )
+        coarse_mesh = Mesh(mesh.x_intervals, [2.0 * d for d in mesh.d_x],
+                           mesh.coordinate_system_type)
+        self.coarse_cp = ConstrainedProblem(
+            cp.differential_equation, coarse_mesh, cp.boundary_conditions
+        )
+        self.fine_shape = mesh.vertices_shape
+        self.m0, self.m1 = coarse_mesh.vertices_shape
+        self.inner = FDMSlicePropagator(op, self.coarse_cp, 1, slice_steps, stream)
+        self.n_fields = cp.differential_equation.y_dimension
+        self.stream = stream
+        self.c_in = torch.empty(self.inner.elems, dtype=torch.float64, device="cuda")
+        self.c_out = torch.empty(self.inner.elems, dtype=torch.float64, device="cuda")
+        self.elems = self.n_fields * int(np.prod(self.fine_shape))
+
+    def run(self, u_f64, first: int, last: int, out_f64):
+        lib = self.inner.batch.lib
+        sp = ctypes.c_void_p(self.stream.cuda_stream)
+        for k in range(last - first):
+            nat.check(lib.pml_restrict2x(nat.PML_F64, u_f64[k].data_ptr(), self.c_in.data_ptr(),
+                                         self.n_fields, self.m0, self.m1, sp))
+            self.inner.run(self.c_in.view(1, -1), 0, 1, self.c_out.view(1, -1))
+            nat.check(lib.pml_prolong2x(nat.PML_F64, self.c_out.data_ptr(), out_f64[k].data_ptr(),
+                                        self.n_fields, self.m0, self.m1, sp))
+
+
+class CoarseGridFDMOperator:
+    """Marks an ``FDMOperator`` to be run on the 2x-coarsened grid when used
+    as a Parareal coarse propagator (a capability the reference does not
+    have: its coarse operators share the fine grid)."""
+
+    def __init__(self, fdm_operator):
+        self.fdm = fdm_operator
+
+    @property
+    def d_t(self) -> float:
+        return self.fdm.d_t
+
+    @property
+    def vertex_oriented(self) -> Optional[bool]:
+        return True
+
+    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False):
+        return CoarseGridPropagator(self.fdm, cp, n_slices, slice_steps, stream)

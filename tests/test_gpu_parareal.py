@@ -1,0 +1,162 @@
+"""Parity of the one-GPU Parareal driver with the oracle's serial emulation
+of the reference's SPMD algorithm, and the textbook invariants."""
+
+import numpy as np
+import pytest
+
+import pararealml_amd as pml
+from oracle import fdm_numpy as ora
+from pararealml_amd.operators.fdm import (
+    RK4,
+    FDMOperator,
+    ForwardEulerMethod,
+    ThreePointCentralDifferenceMethod,
+)
+from pararealml_amd.operators.parareal import CoarseGridFDMOperator, PararealOperator
+from tests.helpers import cahn_hilliard, zero_bc
+
+pytestmark = pytest.mark.gpu
+
+DIFF = ThreePointCentralDifferenceMethod()
+
+
+def rel_err(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def diffusion_problem(n=33, d=0.5):
+    eq = pml.DiffusionEquation(2, d)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / (n - 1)] * 2)
+    bc = zero_bc(1)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.4, 0.55]), 0.01 * np.eye(2))])
+    return cp, ic
+
+
+def oracle_parareal(cp, ic, t_interval, n_slices, f_dt, g_dt, f_int, g_int, tol,
+                    coarse_grid=False, max_iterations=10**9):
+    y0 = ic.discrete_y_0(True)
+    y_cons = cp.static_y_vertex_constraints
+
+    def constrain(y):
+        return ora.apply_constraints_last_axis(y_cons, y)
+
+    def f_solve(y, a, b):
+        return ora.fdm_solve(cp, y, (a, b), f_dt, f_int)[1]
+
+    if coarse_grid:
+        mesh = cp.mesh
+        cmesh = pml.Mesh(mesh.x_intervals, [2 * d for d in mesh.d_x])
+        ccp = pml.ConstrainedProblem(cp.differential_equation, cmesh, cp.boundary_conditions)
+
+        def g_solve(y, a, b):
+            yc = ora.apply_constraints_last_axis(ccp.static_y_vertex_constraints,
+                                                 ora.restrict_injection(y).copy())
+            traj = ora.fdm_solve(ccp, yc, (a, b), g_dt, g_int)[1]
+            return np.stack([ora.prolong_bilinear(s) for s in traj])
+    else:
+        def g_solve(y, a, b):
+            return ora.fdm_solve(cp, y, (a, b), g_dt, g_int)[1]
+
+    return ora.parareal_solve(y0, t_interval, n_slices, f_solve, g_solve, f_dt, g_dt,
+                              constrain, tol, max_iterations)
+
+
+def test_parareal_matches_oracle_emulation_fp64():
+    print("case: parareal 4 slices fp64", flush=True)
+    cp, ic = diffusion_problem()
+    t = (0.0, 0.04)
+    f = FDMOperator(RK4(), DIFF, 1e-4)
+    g = FDMOperator(ForwardEulerMethod(), DIFF, 5e-4)
+    p = PararealOperator(f, g, 1e-6, n_slices=4)
+    sol = p.solve(pml.InitialValueProblem(cp, t, ic))
+    t_ref, y_ref, it_ref, _ = oracle_parareal(cp, ic, t, 4, 1e-4, 5e-4, "rk4", "fe", 1e-6)
+    assert p.last_iterations == it_ref
+    np.testing.assert_allclose(sol.t_coordinates, t_ref)
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+def test_parareal_n_iterations_equals_serial_fine():
+    """With a tolerance that is never met, N iterations reproduce the serial
+    fine solution (Parareal's finite-termination property)."""
+    print("case: parareal finite termination", flush=True)
+    cp, ic = diffusion_problem(25)
+    t = (0.0, 0.02)
+    f = FDMOperator(RK4(), DIFF, 1e-4)
+    g = FDMOperator(ForwardEulerMethod(), DIFF, 1e-3)
+    p = PararealOperator(f, g, 0.0, n_slices=5)
+    ivp = pml.InitialValueProblem(cp, t, ic)
+    sol = p.solve(ivp)
+    assert p.last_iterations == 5
+    fine = f.solve(ivp).discrete_y()
+    assert rel_err(sol.discrete_y(), fine) <= 1e-12
+
+
+def test_parareal_coarse_equal_to_fine_converges_in_one_iteration():
+    print("case: parareal g == f", flush=True)
+    cp, ic = diffusion_problem(25)
+    t = (0.0, 0.02)
+    f = FDMOperator(RK4(), DIFF, 1e-4)
+    p = PararealOperator(f, f, 1e-12, n_slices=4)
+    ivp = pml.InitialValueProblem(cp, t, ic)
+    sol = p.solve(ivp)
+    assert p.last_iterations == 1
+    assert rel_err(sol.discrete_y(), f.solve(ivp).discrete_y()) <= 1e-12
+
+
+def test_parareal_single_slice_and_serial_mode():
+    cp, ic = diffusion_problem(17)
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.01), ic)
+    f = FDMOperator(RK4(), DIFF, 1e-4)
+    g = FDMOperator(ForwardEulerMethod(), DIFF, 1e-3)
+    fine = f.solve(ivp).discrete_y()
+    p = PararealOperator(f, g, 1e-6)
+    np.testing.assert_allclose(p.solve(ivp).discrete_y(), fine, rtol=0, atol=1e-14)
+    np.testing.assert_array_equal(p.solve(ivp, parallel_enabled=False).discrete_y(), fine)
+
+
+def test_parareal_coarse_grid_fp64_and_compensated_fine():
+    """2x-coarsened coarse operator; compensated-FP32 fine solves judged
+    against the FP64 oracle emulation at a stated 1e-6 relative tolerance."""
+    print("case: parareal coarse grid", flush=True)
+    cp, ic = diffusion_problem(33)
+    t = (0.0, 0.04)
+    g = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), DIFF, 5e-4))
+    t_ref, y_ref, it_ref, _ = oracle_parareal(cp, ic, t, 4, 1e-4, 5e-4, "rk4", "fe", 1e-7,
+                                              coarse_grid=True)
+    f64 = PararealOperator(FDMOperator(RK4(), DIFF, 1e-4), g, 1e-7, n_slices=4)
+    sol = f64.solve(pml.InitialValueProblem(cp, t, ic))
+    assert f64.last_iterations == it_ref
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+    f32c = PararealOperator(FDMOperator(RK4(), DIFF, 1e-4, precision="float32c"), g, 1e-7,
+                            n_slices=4, output="endpoints")
+    sol = f32c.solve(pml.InitialValueProblem(cp, t, ic))
+    assert abs(f32c.last_iterations - it_ref) <= 1
+    ends = y_ref.reshape(4, -1, *y_ref.shape[1:])[:, -1]
+    assert rel_err(sol.discrete_y(), ends) <= 1e-6
+
+
+def test_parareal_cahn_hilliard_multi_field_tolerances():
+    print("case: parareal cahn-hilliard", flush=True)
+    cp, ic = cahn_hilliard(20, seed=4)
+    t = (0.0, 0.004)
+    f = FDMOperator(RK4(), DIFF, 1e-4)
+    g = FDMOperator(ForwardEulerMethod(), DIFF, 5e-4)
+    p = PararealOperator(f, g, [1e-8, 1e-6], n_slices=4)
+    sol = p.solve(pml.InitialValueProblem(cp, t, ic))
+    t_ref, y_ref, it_ref, _ = oracle_parareal(cp, ic, t, 4, 1e-4, 5e-4, "rk4", "fe",
+                                              [1e-8, 1e-6])
+    assert p.last_iterations == it_ref
+    assert rel_err(sol.discrete_y(), y_ref) <= 1e-12
+
+
+def test_parareal_validation_errors():
+    cp, ic = diffusion_problem(17)
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.01), ic)
+    f = FDMOperator(RK4(), DIFF, 3e-4)
+    g = FDMOperator(ForwardEulerMethod(), DIFF, 1e-3)
+    with pytest.raises(ValueError):
+        PararealOperator(f, g, 1e-6, n_slices=4).solve(ivp)
+    f = FDMOperator(RK4(), DIFF, 1e-4)
+    with pytest.raises(ValueError):
+        PararealOperator(f, g, [1e-6, 1e-6], n_slices=2).solve(ivp)
