@@ -181,6 +181,13 @@ int pml_parareal_update(const double *g_new, const double *corr,
                         double *u_next, int64_t n, int n_fields,
                         double *sumsq, void *stream);
 
+/* Imposes the Dirichlet edge values of one grid's boundary block (pml_grid
+ * layout, FP64) on `batch` consecutive FP64 field-major copies of the grid
+ * (what the reference's constraint application does to a state,
+ * pararealml/constraint.py:104-131). */
+int pml_apply_dirichlet(double *state, const double *bc_values, int n_fields,
+                        int n0, int n1, int ndim, int64_t batch, void *stream);
+
 /* Conversions between FP64 and the state precisions (device to device). */
 int pml_from_f64(int precision, const double *src, void *dst_hi, void *dst_lo,
                  int64_t n, void *stream);

@@ -100,6 +100,9 @@ SIGNATURES = {
     "pml_prolong2x": (
         c_int, [c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p]
     ),
+    "pml_apply_dirichlet": (
+        c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int64, c_void_p]
+    ),
     "pml_conv64": (
         c_int,
         [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,

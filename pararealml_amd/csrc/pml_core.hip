@@ -137,9 +137,54 @@ __global__ void k_prolong(const T *coarse, T *fine, int64_t batch, int64_t m0, i
   }
 }
 
+// Dirichlet values of a 2-D grid's edge arrays (the pml_grid boundary block
+// layout) imposed on FP64 states of `batch` copies of the grid.
+__global__ void k_apply_dirichlet(double *u, const double *bc, int n_fields, int n0, int n1,
+                                  int ndim, int64_t batch) {
+  const int64_t per = 2 * static_cast<int64_t>(n0 + n1);  // perimeter slots
+  const int64_t total = batch * n_fields * per;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / (n_fields * per);
+    const int64_t r = t - b * n_fields * per;
+    const int f = static_cast<int>(r / per);
+    const int64_t k = r - f * per;
+    const double *fb = bc + static_cast<int64_t>(f) * 4 * (n0 + n1);
+    int gi, gj;
+    if (k < n1) { gi = 0; gj = static_cast<int>(k); }
+    else if (k < 2 * n1) { gi = n0 - 1; gj = static_cast<int>(k - n1); }
+    else if (k < 2 * n1 + n0) { gi = static_cast<int>(k - 2 * n1); gj = 0; }
+    else { gi = static_cast<int>(k - 2 * n1 - n0); gj = n1 - 1; }
+    if (ndim == 1 && k < 2 * n1) continue;  // 1-D: only the line's two ends
+    double v = NAN;
+    if (ndim == 2) {
+      if (gi == 0) v = fb[gj];
+      else if (gi == n0 - 1) v = fb[n1 + gj];
+    }
+    if (v != v) {
+      if (gj == 0) v = fb[2 * n1 + gi];
+      else if (gj == n1 - 1) v = fb[2 * n1 + n0 + gi];
+    }
+    if (v == v)
+      u[(b * n_fields + f) * static_cast<int64_t>(n0) * n1 + static_cast<int64_t>(gi) * n1 + gj] = v;
+  }
+}
+
 }  // namespace pml
 
 using namespace pml;
+
+extern "C" int pml_apply_dirichlet(double *state, const double *bc_values, int n_fields, int n0,
+                                   int n1, int ndim, int64_t batch, void *stream) {
+  if (!state || !bc_values || n_fields <= 0 || n0 <= 0 || n1 <= 0 || batch <= 0 ||
+      (ndim != 1 && ndim != 2))
+    return fail(PML_E_INVALID, "bad Dirichlet arguments");
+  const int64_t total = batch * n_fields * 2 * static_cast<int64_t>(n0 + n1);
+  hipLaunchKernelGGL(k_apply_dirichlet, dim3(blocks_for(total, 256)), dim3(256), 0,
+                     as_stream(stream), state, bc_values, n_fields, n0, n1, ndim, batch);
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
 
 extern "C" const char *pml_last_error(void) { return t_error.c_str(); }
 

@@ -1,0 +1,16 @@
+"""Learned coarse propagator: a BF16 convolutional surrogate trained on the
+device FDM solver's trajectories."""
+
+from pararealml_amd.operators.ml.conv_surrogate import ConvSurrogate
+from pararealml_amd.operators.ml.learned_coarse import (
+    LearnedCoarseOperator,
+    gaussian_mixture_states,
+    generate_pairs,
+)
+
+__all__ = [
+    "ConvSurrogate",
+    "LearnedCoarseOperator",
+    "gaussian_mixture_states",
+    "generate_pairs",
+]

@@ -1,0 +1,240 @@
+"""The BF16 MFMA surrogate kernels against the NumPy oracle (same BF16
+rounding points, FP64 accumulation there vs FP32 on the device), training,
+and the learned coarse propagator with its FDM fallback."""
+
+import numpy as np
+import pytest
+
+from oracle import surrogate_numpy as so
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / max(np.linalg.norm(np.ravel(b)), 1e-300))
+
+
+def random_params(c, seed):
+    rng = np.random.RandomState(seed)
+    params = so.init_params(c, c, seed=seed)
+    w5 = so.bf16(rng.normal(0, 0.05, params[-1][0].shape))
+    params[-1] = (w5, rng.normal(0, 0.01, c))
+    return [(so.bf16(w), rng.normal(0, 0.05, b.shape)) for w, b in params]
+
+
+@pytest.mark.parametrize("shape", [(2, 13, 37), (1, 32, 32), (3, 9, 70)])
+def test_conv64_forward_matches_oracle(shape):
+    import torch
+
+    from pararealml_amd import _native as nat
+
+    print(f"case: conv64 forward {shape}", flush=True)
+    b, h, w = shape
+    rng = np.random.RandomState(0)
+    x = so.bf16(rng.randn(b, h, w, 64))
+    wt = so.bf16(rng.normal(0, 0.1, (3, 3, 64, 64)))
+    bias = rng.normal(0, 0.1, 64)
+    z_ref = so.conv3x3(x, wt, bias)
+    lib = nat.device_lib()
+    xt = torch.tensor(x, dtype=torch.bfloat16, device="cuda").contiguous()
+    wf = torch.tensor(wt.reshape(9, 64, 64), dtype=torch.bfloat16, device="cuda")
+    wf = wf.permute(0, 2, 1).contiguous()
+    bt = torch.tensor(bias, dtype=torch.float32, device="cuda")
+    z = torch.empty_like(xt)
+    a = torch.empty_like(xt)
+    nat.check(lib.pml_conv64(0, xt.data_ptr(), wf.data_ptr(), bt.data_ptr(), None,
+                             z.data_ptr(), a.data_ptr(), b, h, w, None))
+    torch.cuda.synchronize()
+    zg = z.float().cpu().numpy()
+    ag = a.float().cpu().numpy()
+    # z is stored in BF16: one BF16 ulp (2^-8 relative) of tolerance
+    assert np.max(np.abs(zg - z_ref)) <= 2**-7 * np.max(np.abs(z_ref))
+    assert rel(zg, z_ref) <= 4e-3
+    assert rel(ag, so.gelu(z_ref)) <= 4e-3
+
+
+def test_conv64_input_gradient_matches_oracle():
+    import torch
+
+    from pararealml_amd import _native as nat
+
+    print("case: conv64 input gradient", flush=True)
+    b, h, w = 2, 11, 40
+    rng = np.random.RandomState(1)
+    dz = so.bf16(rng.randn(b, h, w, 64))
+    zprev = so.bf16(rng.randn(b, h, w, 64))
+    wt = so.bf16(rng.normal(0, 0.1, (3, 3, 64, 64)))
+    ref = so.conv3x3_input_grad(dz, wt) * so.gelu_grad(zprev)
+    lib = nat.device_lib()
+    wb = torch.tensor(wt.reshape(9, 64, 64), dtype=torch.bfloat16, device="cuda").flip(0).contiguous()
+    dzt = torch.tensor(dz, dtype=torch.bfloat16, device="cuda")
+    zpt = torch.tensor(zprev, dtype=torch.bfloat16, device="cuda")
+    out = torch.empty_like(dzt)
+    nat.check(lib.pml_conv64(1, dzt.data_ptr(), wb.data_ptr(), None, zpt.data_ptr(), None,
+                             out.data_ptr(), b, h, w, None))
+    torch.cuda.synchronize()
+    assert rel(out.float().cpu().numpy(), ref) <= 4e-3
+
+
+def test_wgrad64_matches_oracle():
+    import torch
+
+    from pararealml_amd import _native as nat
+
+    print("case: wgrad64", flush=True)
+    b, h, w = 3, 10, 45
+    rng = np.random.RandomState(2)
+    x = so.bf16(rng.randn(b, h, w, 64))
+    dz = so.bf16(rng.randn(b, h, w, 64))
+    ref = so.conv3x3_weight_grad(x, dz).reshape(9, 64, 64)
+    lib = nat.device_lib()
+    dw = torch.zeros(9 * 64 * 64, dtype=torch.float32, device="cuda")
+    xt = torch.tensor(x, dtype=torch.bfloat16, device="cuda")
+    dzt = torch.tensor(dz, dtype=torch.bfloat16, device="cuda")
+    nat.check(lib.pml_wgrad64(xt.data_ptr(), dzt.data_ptr(), dw.data_ptr(), b, h, w, None))
+    torch.cuda.synchronize()
+    assert rel(dw.cpu().numpy().reshape(9, 64, 64), ref) <= 1e-5
+
+
+@pytest.mark.parametrize("c", [1, 3])
+def test_network_forward_and_gradients_match_oracle(c):
+    import torch
+
+    from pararealml_amd.operators.ml import ConvSurrogate
+
+    print(f"case: network gradients c={c}", flush=True)
+    params = random_params(c, seed=4 + c)
+    rng = np.random.RandomState(7)
+    u = rng.rand(2, c, 20, 35) + 0.1
+    target = u + 0.01 * rng.randn(*u.shape)
+    net = ConvSurrogate(c)
+    net.set_params(params)
+    ut = torch.tensor(u, dtype=torch.float32, device="cuda")
+    tt = torch.tensor(target, dtype=torch.float32, device="cuda")
+    loss = net.loss_and_grads(ut, tt)
+    torch.cuda.synchronize()
+    # oracle on the same normalisation (per-sample max |u| in FP32)
+    u32 = u.astype(np.float32).astype(np.float64)
+    s = np.abs(u32).reshape(2, -1).max(axis=1)
+    x = np.moveaxis(u32 / s[:, None, None, None], 1, -1)
+    t = np.moveaxis(target.astype(np.float32).astype(np.float64) / s[:, None, None, None], 1, -1)
+    loss_ref, grads = so.loss_and_grads(params, x, t)
+    assert abs(loss - loss_ref) <= 1e-3 * loss_ref
+    for layer, (gw_ref, gb_ref) in enumerate(grads):
+        gw = net.gw[layer].cpu().numpy().reshape(gw_ref.shape)
+        gb = net.gb[layer].cpu().numpy()
+        assert rel(gw, gw_ref) <= 1e-2, (layer, rel(gw, gw_ref))
+        assert rel(gb, gb_ref) <= 1e-2, (layer, rel(gb, gb_ref))
+    # forward prediction
+    pred = net.predict(ut.double()).cpu().numpy()
+    ref = np.moveaxis(so.predict(params, np.moveaxis(u32, 1, -1)), -1, 1)
+    assert rel(pred, ref) <= 5e-3
+
+
+def test_adam_step_matches_oracle():
+    import torch
+
+    from pararealml_amd.operators.ml import ConvSurrogate
+
+    print("case: adam", flush=True)
+    params = random_params(1, seed=9)
+    net = ConvSurrogate(1)
+    net.set_params(params)
+    rng = np.random.RandomState(3)
+    grads = []
+    for layer, (w, b) in enumerate(params):
+        gw = rng.randn(*w.shape).astype(np.float32).astype(np.float64)
+        gb = rng.randn(*b.shape).astype(np.float32).astype(np.float64)
+        net.gw[layer].copy_(torch.tensor(gw.ravel(), dtype=torch.float32))
+        net.gb[layer].copy_(torch.tensor(gb, dtype=torch.float32))
+        grads.append((gw, gb))
+    for _ in range(3):
+        net.adam_step(1e-3)
+    masters = [(w.astype(np.float32).astype(np.float64), b.astype(np.float32).astype(np.float64))
+               for w, b in params]
+    zeros = lambda: [(np.zeros_like(w), np.zeros_like(b)) for w, b in params]  # noqa: E731
+    state = (masters, zeros(), zeros(), 0)
+    for _ in range(3):
+        _, state = so.adam_step(params, grads, state, 1e-3)
+    for (gw, gb), (rw, rb) in zip(net.params(), state[0]):
+        np.testing.assert_allclose(gw, rw, rtol=1e-5, atol=2e-6)
+        np.testing.assert_allclose(gb, rb, rtol=1e-5, atol=2e-6)
+
+
+def test_training_learns_a_diffusion_step():
+    """Training on (u, heat-step(u)) pairs reduces the loss well below the
+    identity map's and the trained network beats the identity on held-out
+    states."""
+    import torch
+
+    from pararealml_amd.operators.ml import ConvSurrogate
+
+    print("case: training", flush=True)
+    rng = np.random.RandomState(11)
+    n, h = 96, 33
+
+    def heat(u, k=3):
+        for _ in range(k):
+            p = np.pad(u, ((0, 0), (0, 0), (1, 1), (1, 1)))
+            u = u + 0.2 * (p[:, :, 2:, 1:-1] + p[:, :, :-2, 1:-1] + p[:, :, 1:-1, 2:] + p[:, :, 1:-1, :-2] - 4 * u)
+        return u
+
+    g = np.linspace(0, 1, h)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    u = np.zeros((n, 1, h, h))
+    for k in range(n):
+        for _ in range(3):
+            m = rng.uniform(0.2, 0.8, 2)
+            v = rng.uniform(0.005, 0.02)
+            u[k, 0] += np.exp(-((X - m[0]) ** 2 + (Y - m[1]) ** 2) / (2 * v))
+    target = heat(u)
+    ut = torch.tensor(u[:80], dtype=torch.float32, device="cuda")
+    tt = torch.tensor(target[:80], dtype=torch.float32, device="cuda")
+    net = ConvSurrogate(1, seed=0)
+    losses = net.fit(ut, tt, steps=300, batch=16, lr=2e-3)
+    assert np.mean(losses[-20:]) < 0.2 * np.mean(losses[:5])
+    held = torch.tensor(u[80:], dtype=torch.float64, device="cuda")
+    pred = net.predict(held).cpu().numpy()
+    err_net = np.linalg.norm(pred - target[80:])
+    err_id = np.linalg.norm(u[80:] - target[80:])
+    assert err_net < 0.5 * err_id, (err_net, err_id)
+
+
+def test_learned_coarse_fallback_on_non_finite_output():
+    """A surrogate producing NaN falls back to the FDM coarse operator, so
+    the Parareal result equals the FDM-coarse run."""
+    import torch
+
+    import pararealml_amd as pml
+    from pararealml_amd.operators.fdm import (
+        RK4,
+        FDMOperator,
+        ForwardEulerMethod,
+        ThreePointCentralDifferenceMethod,
+    )
+    from pararealml_amd.operators.ml import ConvSurrogate, LearnedCoarseOperator
+    from pararealml_amd.operators.parareal import CoarseGridFDMOperator, PararealOperator
+    from tests.helpers import zero_bc
+
+    print("case: learned coarse fallback", flush=True)
+    diff = ThreePointCentralDifferenceMethod()
+    eq = pml.DiffusionEquation(2, 0.5)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1 / 32, 1 / 32])
+    bc = zero_bc(1)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.5, 0.45]), 0.01 * np.eye(2))])
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.02), ic)
+    f = FDMOperator(RK4(), diff, 1e-4)
+    coarse_fdm = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), diff, 5e-4))
+    net = ConvSurrogate(1)
+    net.b32[0].fill_(float("nan"))
+    net._refresh_bf16()
+    g = LearnedCoarseOperator(net, 1.25e-3, coarsen=True, fallback=coarse_fdm)
+    p_learned = PararealOperator(f, g, 1e-7, n_slices=4, output="endpoints")
+    y_l = p_learned.solve(ivp).discrete_y()
+    assert g.fallbacks > 0
+    p_fdm = PararealOperator(f, coarse_fdm, 1e-7, n_slices=4, output="endpoints")
+    y_f = p_fdm.solve(ivp).discrete_y()
+    assert p_learned.last_iterations == p_fdm.last_iterations
+    np.testing.assert_allclose(y_l, y_f, rtol=0, atol=1e-12)
+    torch.cuda.synchronize()
