@@ -42,7 +42,17 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="ch256", choices=["ch256"])
+    ap.add_argument("--workload", default="ch256", choices=["ch256", "parareal", "copy"])
+    # Parareal workload (2-D diffusion, learned vs FDM coarse)
+    ap.add_argument("--pr-n", type=int, default=256, help="cells per axis (vertices n + 1)")
+    ap.add_argument("--pr-slices", type=int, default=128)
+    ap.add_argument("--pr-fine-steps", type=int, default=40000, help="fine RK4 steps per slice")
+    ap.add_argument("--pr-coarse-steps", type=int, default=800, help="coarse FE steps per slice")
+    ap.add_argument("--pr-substeps", type=int, default=8, help="network applications per slice")
+    ap.add_argument("--pr-traj", type=int, default=256, help="training trajectories")
+    ap.add_argument("--pr-train-steps", type=int, default=3000)
+    ap.add_argument("--pr-target", type=float, default=1e-6)
+    ap.add_argument("--pr-fine-precision", default="float32c")
     ap.add_argument("--tile-config", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
@@ -125,6 +135,201 @@ def cpu_baseline(cps, sizes, y0, budget_s):
     }
 
 
+def parareal_workload(args):
+    """Time-to-target of learned-coarse Parareal, FDM-coarse Parareal and
+    serial fine FDM on one GPU, all judged against the FP64 serial fine
+    solution at the slice ends.
+
+    2-D diffusion (d = 0.5) on [0, 1]^2 with (n + 1)^2 vertices, zero
+    Dirichlet edges, initial value = sum of 4 normalised Gaussians with
+    means in [0.2, 0.8]^2 and variances in [0.005, 0.02] from
+    RandomState(434342745); t in [0, 1] split into S slices; fine RK4 with
+    the configured steps per slice; FDM coarse = forward Euler on the
+    2x-coarsened grid; learned coarse = the conv surrogate on the
+    2x-coarsened grid, K applications per slice, trained on trajectories of
+    the device FDM solver from other seeded Gaussian mixtures, with the FDM
+    coarse operator as fallback."""
+    import torch
+
+    import pararealml_amd as pml
+    from pararealml_amd.operators.fdm import (
+        RK4,
+        FDMOperator,
+        ForwardEulerMethod,
+        ThreePointCentralDifferenceMethod,
+    )
+    from pararealml_amd.operators.ml import (
+        ConvSurrogate,
+        LearnedCoarseOperator,
+        gaussian_mixture_states,
+        generate_pairs,
+    )
+    from pararealml_amd.operators.parareal import CoarseGridFDMOperator, PararealOperator
+
+    torch.cuda.set_device(0)
+    n, S = args.pr_n, args.pr_slices
+    target = args.pr_target
+    diff = ThreePointCentralDifferenceMethod()
+    eq = pml.DiffusionEquation(2, 0.5)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / n, 1.0 / n])
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    y0 = gaussian_mixture_states(cp, 1, seed=SEEDS[1])[0]
+    ic = pml.DiscreteInitialCondition(cp, y0, True)
+    ivp = pml.InitialValueProblem(cp, (0.0, 1.0), ic)
+    dt_f = 1.0 / (S * args.pr_fine_steps)
+    dt_c = 1.0 / (S * args.pr_coarse_steps)
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+
+    def timed_serial(precision):
+        op = FDMOperator(RK4(), diff, dt_f, precision=precision,
+                         snapshot_every=args.pr_fine_steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol = op.solve(ivp)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, sol.discrete_y()
+
+    log("parareal: FP64 serial fine reference")
+    t_ref, ref = timed_serial("float64")
+    ref_flat = ref.reshape(S, -1)
+    ref_dev = torch.from_numpy(ref_flat).cuda()
+    ref_norm = torch.linalg.norm(ref_dev, dim=1)
+
+    def err_of(end_points):
+        e = torch.linalg.norm(end_points - ref_dev, dim=1) / ref_norm
+        return float(e.max())
+
+    log("parareal: serial fine at the fine precision")
+    t_serial, y_serial = timed_serial(args.pr_fine_precision)
+    err_serial = float(np.max(np.linalg.norm(y_serial.reshape(S, -1) - ref_flat, axis=1)
+                              / np.linalg.norm(ref_flat, axis=1)))
+
+    def run_parareal(g, name):
+        history = []
+        prob_box = {}
+
+        def monitor(i, u):
+            e = err_of(u[1:])
+            history.append((i + 1, e))
+            return e <= target
+
+        f = FDMOperator(RK4(), diff, dt_f, precision=args.pr_fine_precision)
+        p = PararealOperator(f, g, 0.0, n_slices=S, output="endpoints", monitor=monitor)
+        log(f"parareal: {name}")
+        p.solve(ivp)
+        st = p.last_stats
+        hit = next((k for k, (it, e) in enumerate(history) if e <= target), None)
+        wall_to_target = st["iteration_wall_s"][hit] if hit is not None else None
+        del prob_box
+        return {
+            "iterations_to_target": history[hit][0] if hit is not None else None,
+            "time_to_target_s": wall_to_target,
+            "errors": [e for _, e in history],
+            "iteration_wall_s": st["iteration_wall_s"],
+            "fine_s": st["fine_s"],
+            "coarse_s": st["coarse_s"],
+            "correction_s": st["correction_s"],
+        }
+
+    g_fdm = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), diff, dt_c))
+    fdm_res = run_parareal(g_fdm, "FDM coarse")
+
+    # learned coarse: data from the device FDM solver, then training
+    K = args.pr_substeps
+    sub = 1.0 / (S * K)
+    data_steps = int(np.ceil(sub / (0.9 * 2.78 * (1.0 / n) ** 2 / (8 * 0.5))))
+    data_op = FDMOperator(RK4(), diff, sub / data_steps, precision="float32")
+    log(f"parareal: generating {args.pr_traj} training trajectories")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y0s = gaussian_mixture_states(cp, args.pr_traj, seed=SEEDS[2])
+    snaps = generate_pairs(data_op, cp, y0s, 1.0, sub, coarsen=True)
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    inputs = snaps[:-1].reshape(-1, *snaps.shape[2:])
+    targets = snaps[1:].reshape(-1, *snaps.shape[2:])
+    net = ConvSurrogate(1, seed=0)
+    log(f"parareal: training on {inputs.shape[0]} pairs")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = net.fit(inputs, targets, steps=args.pr_train_steps, batch=32, lr=1e-3,
+                     log_every=max(1, args.pr_train_steps // 10))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del snaps, inputs, targets
+    torch.cuda.empty_cache()
+    g_learned = LearnedCoarseOperator(net, sub, coarsen=True, fallback=g_fdm)
+    learned_res = run_parareal(g_learned, "learned coarse")
+
+    value = learned_res["time_to_target_s"]
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "s (learned-coarse Parareal time to target error)",
+        "n_gpus": 1,
+        "steps": S * args.pr_fine_steps,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": args.pr_fine_precision,
+        "data": "synthetic (seeded Gaussian-mixture initial conditions)",
+        "config": {
+            "workload": (
+                f"Parareal 2-D diffusion d=0.5, {n + 1}^2 vertices, Dirichlet 0, t in [0,1], "
+                f"{S} slices, fine RK4 {args.pr_fine_steps} steps/slice, FDM coarse FE "
+                f"{args.pr_coarse_steps} steps/slice on the 2x grid, learned coarse {K} "
+                f"network steps/slice on the 2x grid"
+            ),
+            "target_rel_error": target,
+        },
+        "parareal": {
+            "serial_fine_fp64_s": t_ref,
+            "serial_fine_s": t_serial,
+            "serial_fine_error": err_serial,
+            "fdm_coarse": fdm_res,
+            "learned_coarse": learned_res,
+            "learned_fallbacks": g_learned.fallbacks,
+            "training": {
+                "trajectories": args.pr_traj,
+                "pairs": int((S * K) * args.pr_traj),
+                "data_generation_s": t_data,
+                "train_steps": args.pr_train_steps,
+                "train_s": t_train,
+                "loss_first": float(np.mean(losses[:10])),
+                "loss_last": float(np.mean(losses[-10:])),
+            },
+        },
+        "host": host_info(),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def copy_workload(args):
+    """A plain device-to-device copy of 1 GiB, repeated: the calibration
+    reference for the FETCH_SIZE / WRITE_SIZE counters (each dispatch moves
+    exactly 1 GiB in and 1 GiB out)."""
+    import torch
+
+    torch.cuda.set_device(0)
+    n = (1 << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    reps = max(1, args.steps // 100)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    print(json.dumps({"workload": "copy", "bytes_per_dispatch": 2 * 4 * n,
+                      "achieved_GBps": 2 * 4 * n / dt / 1e9}), flush=True)
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -137,6 +342,13 @@ def main():
             )
         )
         sys.exit(2)
+
+    if args.workload == "parareal":
+        parareal_workload(args)
+        return
+    if args.workload == "copy":
+        copy_workload(args)
+        return
 
     import torch
 

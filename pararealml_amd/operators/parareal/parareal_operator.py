@@ -25,7 +25,10 @@ norms are FP64 in HBM; the fine and coarse states use their operators'
 precisions.  ``n_slices`` replaces the MPI communicator size;
 ``output="endpoints"`` returns only the slice end points (the full fine
 trajectory is infeasible for long runs).  The iteration count and timings
-of the last solve are in ``last_iterations`` and ``last_stats``.
+of the last solve are in ``last_iterations`` and ``last_stats``; an optional
+``monitor(iteration, u)`` is called after every iteration with the FP64
+border points ``u`` (device tensor, (N + 1, elems), field-major), its time is
+excluded from the timings, and a True return ends the iterations.
 """
 
 import sys
@@ -57,6 +60,7 @@ class PararealOperator(Operator):
         *,
         n_slices: int = 1,
         output: str = "trajectory",
+        monitor: Optional[Callable] = None,
     ):
         super().__init__(f.d_t, f.vertex_oriented)
         if n_slices < 1:
@@ -69,6 +73,7 @@ class PararealOperator(Operator):
         self._max_iterations = max_iterations
         self._n_slices = int(n_slices)
         self._output = output
+        self._monitor = monitor
         self.last_iterations: Optional[int] = None
         self.last_stats: dict = {}
 
@@ -157,6 +162,7 @@ class PararealOperator(Operator):
 
             iterations = 0
             norms_hist = []
+            iter_wall = []
             for i in range(min(n, self._max_iterations)):
                 iterations = i + 1
                 sync_time("fine_s", lambda: fine.run(u[i:n], i, n, f_end[i:n]))
@@ -186,6 +192,9 @@ class PararealOperator(Operator):
                         )
 
                 sync_time("coarse_s", sweep)
+                iter_wall.append(sum(timings.values()))
+                if self._monitor is not None and self._monitor(i, u):
+                    break
                 if callable(tc):
                     layout = lambda t: problem.from_device_layout(  # noqa: E731
                         t.cpu().numpy().reshape(n, n_fields, problem.n0, problem.n1),
@@ -220,5 +229,7 @@ class PararealOperator(Operator):
                 t = discretize_time_domain(ivp.t_interval, f.d_t)[1:]
                 d_t_out = f.d_t
         torch.cuda.current_stream().wait_stream(stream)
-        self.last_stats = dict(timings, iterations=iterations, update_norms=norms_hist)
+        self.last_stats = dict(
+            timings, iterations=iterations, update_norms=norms_hist, iteration_wall_s=iter_wall
+        )
         return Solution(ivp, t, y, vertex_oriented=self._vertex_oriented, d_t=d_t_out)

@@ -1,0 +1,155 @@
+"""NumericalDifferentiator on the device: the reference's known answers and
+FP64 parity with the oracle on random fields (1-D, 2-D, 3-D), through the
+C ABI stencils and the Jacobi sweep."""
+
+import numpy as np
+import pytest
+
+import pararealml_amd as pml
+from oracle import fdm_numpy as ora
+from pararealml_amd.constraint import Constraint
+from pararealml_amd.operators.fdm import ThreePointCentralDifferenceMethod
+from tests.helpers import dbc_from_spec
+
+pytestmark = pytest.mark.gpu
+
+DIFF = ThreePointCentralDifferenceMethod()
+
+
+def _mesh(k):
+    return pml.Mesh([tuple(iv) for iv in k["x_intervals"]], k["d_x"])
+
+
+@pytest.mark.parametrize(
+    "name", ["gradient_2d", "gradient_1d_constrained", "gradient_2d_constrained"]
+)
+def test_device_gradient_kats(kats, name):
+    k = kats[name]
+    y = np.array(k["y"])
+    dbc = dbc_from_spec(k.get("dbc"), len(k["d_x"]), y.shape[-1])
+    out = DIFF.gradient(y, _mesh(k), k["x_axis"], dbc)
+    np.testing.assert_allclose(out, k["expected"], rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize(
+    "name", ["hessian_2d", "hessian_1d_constrained", "hessian_2d_constrained", "hessian_mixed"]
+)
+def test_device_hessian_kats(kats, name):
+    k = kats[name]
+    y = np.array(k["y"])
+    dbc = dbc_from_spec(k.get("dbc"), len(k["d_x"]), y.shape[-1])
+    out = DIFF.hessian(y, _mesh(k), k["x_axis1"], k["x_axis2"], dbc)
+    np.testing.assert_allclose(out, k["expected"], rtol=0, atol=1e-12)
+
+
+def test_device_divergence_curl_laplacian_kats(kats):
+    for name in ("divergence_2d", "divergence_3d"):
+        k = kats[name]
+        np.testing.assert_allclose(DIFF.divergence(np.array(k["y"]), _mesh(k)), k["expected"], atol=1e-12)
+    k = kats["curl_2d"]
+    np.testing.assert_allclose(DIFF.curl(np.array(k["y"]), _mesh(k)), k["expected"], atol=1e-12)
+    k = kats["curl_3d"]
+    exp = np.array(k["expected"])
+    for i in range(3):
+        np.testing.assert_allclose(DIFF.curl(np.array(k["y"]), _mesh(k), i), exp[..., i : i + 1], atol=1e-12)
+    k = kats["laplacian_2d"]
+    np.testing.assert_allclose(DIFF.laplacian(np.array(k["y"]), _mesh(k)), k["expected"], atol=1e-12)
+    y = np.array(k["y"])
+    np.testing.assert_allclose(DIFF.vector_laplacian(y, _mesh(k), 1), DIFF.laplacian(y[..., 1:], _mesh(k)))
+
+
+def test_device_stencils_match_oracle_on_random_fields():
+    rng = np.random.RandomState(0)
+    mesh = pml.Mesh([(0.0, 1.2), (0.0, 2.0), (0.0, 0.6)], [0.1, 0.25, 0.2])
+    y = rng.randn(*mesh.vertices_shape, 3)
+    dbc = np.empty((3, 3), dtype=object)
+    # random partial derivative constraints on two axes
+    for a, f in ((0, 1), (2, 0)):
+        shape = list(mesh.vertices_shape) + [1]
+        shape[a] = 1
+        mask = rng.rand(*shape) > 0.4
+        dbc[a, f] = (Constraint(rng.randn(mask.sum()), mask), None)
+    for axis in range(3):
+        np.testing.assert_allclose(DIFF.gradient(y, mesh, axis, dbc), ora.gradient(y, mesh.d_x, axis, dbc), atol=1e-11)
+        for axis2 in range(3):
+            np.testing.assert_allclose(
+                DIFF.hessian(y, mesh, axis, axis2, dbc), ora.hessian(y, mesh.d_x, axis, axis2, dbc), atol=1e-9
+            )
+    np.testing.assert_allclose(DIFF.laplacian(y, mesh, dbc), ora.laplacian(y, mesh.d_x, dbc), atol=1e-9)
+    np.testing.assert_allclose(DIFF.divergence(y, mesh, dbc), ora.divergence(y, mesh.d_x, dbc), atol=1e-11)
+    for i in range(3):
+        np.testing.assert_allclose(DIFF.curl(y, mesh, i, dbc), ora.curl(y, mesh.d_x, i, dbc), atol=1e-11)
+    # FP32 inputs stay FP32
+    y32 = y.astype(np.float32)
+    out = DIFF.laplacian(y32, mesh)
+    assert out.dtype == np.float32
+    np.testing.assert_allclose(out, ora.laplacian(y, mesh.d_x), rtol=1e-4, atol=1e-3)
+
+
+def test_device_accepts_torch_tensors():
+    import torch
+
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [0.1, 0.1])
+    y = np.random.rand(11, 11, 1)
+    t = torch.tensor(y, device="cuda")
+    out = DIFF.laplacian(t, mesh)
+    assert out.is_cuda
+    np.testing.assert_allclose(out.cpu().numpy(), ora.laplacian(y, mesh.d_x), atol=1e-10)
+
+
+@pytest.mark.parametrize("case", ["2d", "1d_derivative", "2d_derivative"])
+def test_device_anti_laplacian_round_trip(case):
+    """The reference's anti-Laplacian tests
+    (test_numerical_differentiator.py:527-667)."""
+    diff = ThreePointCentralDifferenceMethod(1e-12)
+    rng = np.random.RandomState(3)
+    if case == "1d_derivative":
+        y = rng.random_sample((20, 2))
+        mesh = pml.Mesh([(0.0, 0.95)], [0.05])
+        v0 = np.full((20, 1), np.nan)
+        v0[0] = 1.0
+        v1 = np.full((20, 1), np.nan)
+        v1[0], v1[-1] = -2.0, 2.0
+        cons = []
+        for v in (v0, v1):
+            m = ~np.isnan(v)
+            cons.append(Constraint(v[m], m))
+        cons[0].apply(y[..., :1])
+        cons[1].apply(y[..., 1:])
+        dbc = np.array([[(None, Constraint(np.full(1, -3.0), np.ones(1, dtype=bool))), None]], dtype=object)
+    else:
+        y = rng.random_sample((20, 20, 2))
+        mesh = pml.Mesh([(0.0, 0.95), (0.0, 0.475)], [0.05, 0.025])
+        v = np.full((20, 20, 1), np.nan)
+        v[0], v[-1] = 1.0, 2.0
+        v[:, 0], v[:, -1] = 3.0, 42.0
+        m = ~np.isnan(v)
+        c = Constraint(v[m], m)
+        c.apply(y[..., :1])
+        c.apply(y[..., 1:])
+        cons = [c, c]
+        dbc = None
+        if case == "2d_derivative":
+            dbc = np.array(
+                [
+                    [(None, Constraint(np.full(20, -3.0), np.ones((1, 20, 1), dtype=bool))), None],
+                    [(None, Constraint(np.full(20, 0.0), np.ones((20, 1, 1), dtype=bool))), None],
+                ],
+                dtype=object,
+            )
+    lap = diff.laplacian(y, mesh, dbc)
+    sol = diff.anti_laplacian(lap, mesh, cons, dbc)
+    np.testing.assert_allclose(diff.laplacian(sol, mesh, dbc), lap, atol=1e-6)
+    np.testing.assert_allclose(sol, y, atol=1e-8)
+
+
+def test_device_differentiator_validation():
+    mesh = pml.Mesh([(0.0, 1.0)], [1.0 / 3.0])
+    with pytest.raises(ValueError):
+        DIFF.gradient(np.arange(1.0, 5.0), mesh, 0)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 2.0), (0.0, 1.0)], [1.0, 1.0, 1.0])
+    with pytest.raises(ValueError):
+        DIFF.gradient(np.arange(0.0, 12.0).reshape((2, 3, 2)), mesh, 0)
+    polar = pml.Mesh([(1.0, 5.0), (0.0, 2 * np.pi)], [2.0, np.pi], pml.CoordinateSystem.POLAR)
+    with pytest.raises(NotImplementedError):
+        DIFF.laplacian(np.zeros((3, 3, 1)), polar)
