@@ -446,6 +446,7 @@ def main():
             "n_grids": len(sizes),
             "cells": cells,
             "kernel_variant": cfg_name,
+            "tile_config": cfg,
             "steps_per_launch": spl,
         },
         "roofline": {

@@ -177,7 +177,6 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
     a.out_lo = nxt_lo;
     a.snap_hi = nullptr;
     a.snap_lo = nullptr;
-    a.constrain_input = (constrain && done == 0) ? 1 : 0;
     if (snapshots_hi && (done + steps) % snapshot_every == 0) {
       a.snap_hi = static_cast<char *>(snapshots_hi) + snap_idx * total_elems * esz;
       a.snap_lo = snapshots_lo ? static_cast<char *>(snapshots_lo) + snap_idx * total_elems * esz
@@ -192,9 +191,25 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
     return PML_OK;
   };
 
+  // Dirichlet values imposed on the input state in place (it is overwritten
+  // by the result anyway).
+  if (constrain && n_steps > 0) {
+    const dim3 cgrid(n_grids, n_fields);
+    if (precision == PML_F64)
+      hipLaunchKernelGGL(fdm_constrain<PrecF64>, cgrid, dim3(256), 0, st, a.grids, bc_values,
+                         state_hi, state_lo, ndim);
+    else if (precision == PML_F32C)
+      hipLaunchKernelGGL(fdm_constrain<PrecF32C>, cgrid, dim3(256), 0, st, a.grids, bc_values,
+                         state_hi, state_lo, ndim);
+    else
+      hipLaunchKernelGGL(fdm_constrain<PrecF32>, cgrid, dim3(256), 0, st, a.grids, bc_values,
+                         state_hi, state_lo, ndim);
+    PML_LAUNCH_CHECK();
+  }
+
   const int64_t fused_launches = n_steps / steps_per_launch;
   int64_t li = 0;
-  // The first launch may constrain its input; it always runs eagerly.
+  // The first launch runs eagerly (it also warms the unit tables).
   if (fused_launches > 0) {
     rc = launch(can_fuse ? V.fn : V.fn_single, steps_per_launch);
     if (rc) return rc;

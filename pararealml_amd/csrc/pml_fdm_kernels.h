@@ -5,9 +5,10 @@
 // * fdm_stream -- single-stage steps (forward Euler, and any integrator for
 //   stage-invariant systems such as Cahn-Hilliard).  Barrier-free and
 //   LDS-free: each wavefront owns a 64-column strip of RW rows, each lane one
-//   column; the lane loads rows i0-1 .. i0+RW of its column into registers
-//   at once and takes its left/right neighbours from the adjacent lanes with
-//   cross-lane shuffles.  HBM sees one read and one write of the state per
+//   column; the lane walks down its column through a register ring with
+//   rows prefetched ahead, takes its left/right neighbours from the adjacent
+//   lanes (DPP wave shifts) and reads and writes the state through per-row
+//   buffer resources.  HBM sees one read and one write of the state per
 //   step; the two halo rows per strip are mostly served by L2.
 //
 // * fdm_fused -- any stage count, with STEPS steps fused per launch.  One
@@ -63,7 +64,6 @@ struct FdmArgs {
   double p[6];
   int n_tiles;
   int ndim;
-  int constrain_input;  // apply the Dirichlet values to the loaded state
 };
 
 // Bijective block -> unit map that gives every XCD (blocks are dealt to the
@@ -263,16 +263,145 @@ __device__ __forceinline__ void all_quants(Quant<C> *q, const Getter &Y, bool lo
 
 // ---------------------------------------------------------------------------
 // fdm_stream: single-stage steps, one wavefront per 64-column strip of RW
-// rows.  Every lane issues all its loads up front (RW + 2 rows of its column
-// per field, plus the strip-edge halo values on lanes 0 and 63), then
-// computes its RW cells from registers and cross-lane shuffles.
-template <class Eq, class Integ, class P, int RW>
+// rows, one lane per column.  The lane walks down its column through a ring
+// of PD + 3 row slots (rows i-1 .. i+1 in use, rows up to i+1+PD in flight);
+// the row loop is unrolled by the ring length so that no slot is ever
+// copied (a copy would wait for the prefetch).  Left and right neighbours
+// come from the adjacent lanes by DPP wave shifts; lanes 0 and 63 load the
+// strip's halo columns and feed them in as the shifted-in value.  Rows are
+// handled by one of three wave-uniform bodies:
+//   * interior strip (no grid column edge in the strip): plain stencils;
+//   * edge strip: the row's axis-1 edge values are the same for every lane
+//     (scalar loads) and are selected per lane, branch-free;
+//   * first / last grid row: the generic per-vertex boundary code.
+// Out-of-grid rows are never used (their loads are clamped into the grid).
+
+// Per-row, per-field boundary values of the axis-1 edges (NaN = none).
+template <typename C>
+struct RowEdges {
+  C n_lo, n_hi;       // Neumann values at (i, 0) and (i, n1-1)
+  double d_lo, d_hi;  // Dirichlet values at (i, 0) and (i, n1-1)
+};
+
+// Stencil quantities of one field away from the first and last rows:
+// identical arithmetic to field_quant.  kCols selects the column-edge
+// handling (lo1 / hi1 with the row's edge values E).
+template <typename C, bool WL, bool WD, bool kCols>
+__device__ __forceinline__ void field_quant_row(Quant<C> &q, C yc, C yw, C ye, C yn, C ys,
+                                                bool lo1, bool hi1, bool ax0,
+                                                const RowEdges<C> &E, const StencilK<C> &K) {
+  q.v = yc;
+  q.lap = 0;
+  q.d0 = 0;
+  q.d1 = 0;
+  if (!WL && !WD) return;
+  C lap1 = 0, d1 = 0, lap0 = 0, d0 = 0;
+  if (WL) {
+    C gw = yw, ge = ye;
+    if (kCols) {
+      gw = lo1 ? (E.n_lo == E.n_lo ? ye + K.m2dx1 * E.n_lo : C(0)) : yw;
+      ge = hi1 ? (E.n_hi == E.n_hi ? yw + K.p2dx1 * E.n_hi : C(0)) : ye;
+    }
+    lap1 = ((ge - C(2) * yc) + gw) * K.inv_dx1sq;
+    if (ax0) lap0 = ((ys - C(2) * yc) + yn) * K.inv_dx0sq;
+  }
+  if (WD) {
+    if (kCols) {
+      const C zw = lo1 ? C(0) : yw, ze = hi1 ? C(0) : ye;
+      d1 = (ze - zw) * K.inv_2dx1;
+      d1 = (lo1 && E.n_lo == E.n_lo) ? E.n_lo : d1;
+      d1 = (hi1 && E.n_hi == E.n_hi) ? E.n_hi : d1;
+    } else {
+      d1 = (ye - yw) * K.inv_2dx1;
+    }
+    if (ax0) d0 = (ys - yn) * K.inv_2dx0;
+  }
+  if (ax0) {
+    q.lap = lap0 + lap1;
+    q.d0 = d0;
+    q.d1 = d1;
+  } else {
+    q.lap = lap1;
+    q.d0 = d1;  // the single physical axis lies along axis 1
+    q.d1 = 0;
+  }
+}
+
+// Lane l receives lane l-1's x (lane 0: fill), or lane l+1's (lane 63: fill).
+template <typename C>
+__device__ __forceinline__ C wave_shr1(C x, C fill);
+template <typename C>
+__device__ __forceinline__ C wave_shl1(C x, C fill);
+// DPP wave_shr:1 (0x138) / wave_shl:1 (0x130), bound_ctrl off: the lane
+// whose source is outside the wavefront keeps `fill`.
+template <>
+__device__ __forceinline__ float wave_shr1<float>(float x, float fill) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
+                                                               __builtin_bit_cast(int, x), 0x138,
+                                                               0xf, 0xf, false));
+}
+template <>
+__device__ __forceinline__ float wave_shl1<float>(float x, float fill) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, fill),
+                                                               __builtin_bit_cast(int, x), 0x130,
+                                                               0xf, 0xf, false));
+}
+template <>
+__device__ __forceinline__ double wave_shr1<double>(double x, double fill) {
+  const long long r = __builtin_amdgcn_update_dpp(__builtin_bit_cast(long long, fill),
+                                                  __builtin_bit_cast(long long, x), 0x138, 0xf,
+                                                  0xf, false);
+  return __builtin_bit_cast(double, r);
+}
+template <>
+__device__ __forceinline__ double wave_shl1<double>(double x, double fill) {
+  const long long r = __builtin_amdgcn_update_dpp(__builtin_bit_cast(long long, fill),
+                                                  __builtin_bit_cast(long long, x), 0x130, 0xf,
+                                                  0xf, false);
+  return __builtin_bit_cast(double, r);
+}
+
+// Raw buffer access: a resource over `bytes` bytes from p (gfx9 word 3:
+// 32-bit data format, no stride); loads outside the range return 0 and
+// stores outside it are dropped.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, static_cast<int>(bytes),
+                                           0x00020000);
+}
+template <typename C>
+__device__ __forceinline__ C buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <>
+__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+template <>
+__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+template <typename C>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, C x);
+template <>
+__device__ __forceinline__ void buf_store<float>(__amdgpu_buffer_rsrc_t r, unsigned off, float x) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, off, 0, 0);
+}
+template <>
+__device__ __forceinline__ void buf_store<double>(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                                  double x) {
+  using u2 = unsigned __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, off, 0, 0);
+}
+
+template <class Eq, class Integ, class P, int RW, int PD, bool FULL>
 __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
   using C = typename P::C;
   constexpr int F = Eq::F;
+  constexpr int NW = PD + 3;  // ring slots
   static_assert(Eq::kInvariant || Integ::kStages == 1, "single-stage steps only");
   const int lane = threadIdx.x & 63;
-  const int unit = xcd_tile(blockIdx.x, gridDim.x) * 4 + (threadIdx.x >> 6);
+  // The unit is the same for the whole wavefront: keep it (and everything
+  // derived from it) in scalar registers.
+  const int unit = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * 4 +
+                                                  static_cast<int>(threadIdx.x >> 6));
   if (unit >= a.n_tiles) return;  // whole wavefront: no barriers follow
   const TileDev U = a.tiles[unit];
   const GridDev G = a.grids[U.g];
@@ -280,7 +409,7 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
   const int j = U.j0 + lane;
   const bool act = j < n1;
   const bool ax0 = a.ndim == 2;
-  const int plane = n0 * n1;
+  const unsigned plane = static_cast<unsigned>(n0) * n1;
   const double *bcg = a.bc + G.bc;
   const int fstride = 4 * (n0 + n1);
   const StencilK<C> K(G);
@@ -288,98 +417,60 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
 #pragma unroll
   for (int i = 0; i < 6; ++i) pc[i] = static_cast<C>(a.p[i]);
   const C dt = static_cast<C>(a.dt);
-  const int i0 = U.i0;
-  const int hj = lane == 0 ? j - 1 : j + 1;  // halo column of lanes 0 / 63
-  const bool halo_lane = (lane == 0 || lane == 63) && hj >= 0 && hj < n1;
-
-  using S = typename P::C;
-  const S *in_hi = static_cast<const S *>(a.in_hi) + G.off;
-  const S *in_lo = P::kComp ? static_cast<const S *>(a.in_lo) + G.off : nullptr;
-
-  // rows i0-1 .. i0+RW of this lane's column; lo parts of rows i0..i0+RW-1;
-  // halo values of rows i0..i0+RW-1.
-  C v[RW + 2][F], vl[RW][F], h[RW][F];
-#pragma unroll
-  for (int r = 0; r < RW + 2; ++r) {
-    const int i = i0 - 1 + r;
-    const bool in_row = i >= 0 && i < n0;
-#pragma unroll
-    for (int f = 0; f < F; ++f) {
-      C x = 0;
-      if (act && in_row) x = in_hi[f * plane + i * n1 + j];
-      v[r][f] = x;
-      if (r >= 1 && r <= RW) {
-        C xl = 0, hv = 0;
-        if (P::kComp && act && in_row) xl = in_lo[f * plane + i * n1 + j];
-        if (halo_lane && in_row) hv = in_hi[f * plane + i * n1 + hj];
-        vl[r - 1][f] = xl;
-        h[r - 1][f] = hv;
-      }
-    }
-  }
-  if (a.constrain_input) {
-    // Impose the Dirichlet values on every loaded boundary vertex.
-#pragma unroll
-    for (int r = 0; r < RW + 2; ++r) {
-      const int i = i0 - 1 + r;
-      if (i < 0 || i >= n0) continue;
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        if (act && (j == 0 || j == n1 - 1 || (ax0 && (i == 0 || i == n0 - 1)))) {
-          const double dv = dirichlet_at(bcg + f * fstride, n0, n1, i, j, ax0);
-          if (dv == dv) {
-            if (r >= 1 && r <= RW) state_set<P>(v[r][f], vl[r - 1][f], dv);
-            else v[r][f] = static_cast<C>(dv);
-          }
-        }
-        if (r >= 1 && r <= RW && halo_lane &&
-            (hj == 0 || hj == n1 - 1 || (ax0 && (i == 0 || i == n0 - 1)))) {
-          const double dv = dirichlet_at(bcg + f * fstride, n0, n1, i, hj, ax0);
-          if (dv == dv) h[r - 1][f] = static_cast<C>(dv);
-        }
-      }
-    }
-  }
-
-  S *out_hi = static_cast<S *>(a.out_hi) + G.off;
-  S *out_lo = P::kComp ? static_cast<S *>(a.out_lo) + G.off : nullptr;
-  S *sn_hi = a.snap_hi ? static_cast<S *>(a.snap_hi) + G.off : nullptr;
-  S *sn_lo = (a.snap_hi && P::kComp) ? static_cast<S *>(a.snap_lo) + G.off : nullptr;
+  const int i_beg = U.i0;
+  const int i_end = min(U.i0 + RW, n0);
+  const bool edge_strip = U.j0 == 0 || U.j0 + 64 >= n1;  // wave-uniform
   const bool lo1 = j == 0, hi1 = j == n1 - 1;
-#pragma unroll
-  for (int r = 1; r <= RW; ++r) {
-    const int i = i0 - 1 + r;
-    C lf[F], rt[F];
+  const int hj = lane == 0 ? j - 1 : j + 1;
+  const int jj = act ? j : 0;  // column for the per-lane boundary lookups
+
+  const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
+  const C *in_lo = P::kComp ? static_cast<const C *>(a.in_lo) + G.off : nullptr;
+  C *out_hi = static_cast<C *>(a.out_hi) + G.off;
+  C *out_lo = P::kComp ? static_cast<C *>(a.out_lo) + G.off : nullptr;
+  C *sn_hi = a.snap_hi ? static_cast<C *>(a.snap_hi) + G.off : out_hi;
+  C *sn_lo = (a.snap_hi && P::kComp) ? static_cast<C *>(a.snap_lo) + G.off : out_lo;
+
+  // Ring slot s holds row r with r = s (mod NW): value, lo part, halo value.
+  C w[NW][F], wl[NW][F], wh[NW][F];
+  // Lane byte offsets into a row: the lane's column and its halo column
+  // (lanes 0 / 63 only).  Offsets outside the row read 0 (and store
+  // nothing) through the row's buffer range.
+  const unsigned jb = static_cast<unsigned>(j) * static_cast<unsigned>(sizeof(C));
+  const unsigned hb = static_cast<unsigned>(lane == 0 || lane == 63 ? hj : j) *
+                      static_cast<unsigned>(sizeof(C));
+  const unsigned row_bytes = static_cast<unsigned>(n1) * static_cast<unsigned>(sizeof(C));
+  const unsigned snap_bytes = a.snap_hi ? row_bytes : 0u;
+  // Row r of the input; rows outside [0, n0) and past `last` read as 0
+  // without touching memory (empty buffer range), so that the load is
+  // unconditional.
+  auto load = [&](int r, int last, C *v, C *vl, C *h) {
+    const bool ok = r >= 0 && r < n0 && r <= last;
+    const size_t ro = static_cast<size_t>(min(max(r, 0), n0 - 1)) * n1;
+    const unsigned nb = ok ? row_bytes : 0u;
 #pragma unroll
     for (int f = 0; f < F; ++f) {
-      lf[f] = __shfl_up(v[r][f], 1, 64);
-      rt[f] = __shfl_down(v[r][f], 1, 64);
-      if (lane == 0) lf[f] = h[r - 1][f];
-      if (lane == 63) rt[f] = h[r - 1][f];
+      const auto rs = row_rsrc(in_hi + static_cast<size_t>(f) * plane + ro, nb);
+      v[f] = buf_load<C>(rs, jb);
+      if (P::kComp) vl[f] = buf_load<C>(row_rsrc(in_lo + static_cast<size_t>(f) * plane + ro, nb), jb);
+      h[f] = buf_load<C>(rs, hb);
     }
-    if (!act || i >= n0) continue;
-    const bool lo0 = ax0 && i == 0, hi0 = ax0 && i == n0 - 1;
-    Quant<C> q[F];
-    auto Y = [&](int f, int d) -> C {
-      switch (d) {
-        case 0: return v[r][f];
-        case 1: return lf[f];
-        case 2: return rt[f];
-        case 3: return v[r - 1][f];
-        default: return v[r + 1][f];
-      }
-    };
-    all_quants<Eq, C>(q, Y, lo1, hi1, lo0, hi0, ax0, bcg, fstride, i, j, n0, n1, K);
+  };
+
+  // Writes row i: update, Dirichlet values, store.
+  auto finish = [&](int i, const Quant<C> *q, const C *ct, const C *ctl, const double *dv) {
     C rr[F], ra[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) rr[f] = ra[f] = 0;
     Eq::template rhs<C>(pc, q, rr);
     if (Eq::kAlg != 0u) Eq::template alg<C>(pc, q, ra);
-    const bool edge = lo1 || hi1 || lo0 || hi0;
-    const int e = i * n1 + j;
+    const size_t ro = static_cast<size_t>(i) * n1;
+    auto put = [&](C *base, unsigned nb, int f, C x) {
+      buf_store<C>(row_rsrc(base + static_cast<size_t>(f) * plane + ro, nb), jb, x);
+    };
 #pragma unroll
     for (int f = 0; f < F; ++f) {
-      C hv = v[r][f], lv = vl[r - 1][f];
+      C hv = ct[f], lv = P::kComp ? ctl[f] : C(0);
       if ((Eq::kAlg >> f) & 1u) {
         hv = ra[f];
         lv = 0;
@@ -389,16 +480,160 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
         if (Eq::kInvariant && Integ::kStages == 4) d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
         state_add<P>(hv, lv, d);
       }
-      if (edge) {
-        const double dv = dirichlet_at(bcg + f * fstride, n0, n1, i, j, ax0);
-        if (dv == dv) state_set<P>(hv, lv, dv);
+      if (dv && dv[f] == dv[f]) state_set<P>(hv, lv, dv[f]);
+      put(out_hi, row_bytes, f, hv);
+      if (P::kComp) put(out_lo, row_bytes, f, lv);
+      // Snapshot stores go to an empty range when no snapshot is taken.
+      put(sn_hi, snap_bytes, f, hv);
+      if (P::kComp) put(sn_lo, snap_bytes, f, lv);
+    }
+  };
+
+  // The ring computes every row of the strip with the column-edge rules
+  // only; the first and last grid rows (2-D) are computed again below with
+  // the full boundary code, and their second store (same lane, later in
+  // program order) is the one that remains.  The step body has no branch
+  // but the wave-uniform strip kind, so the waits for the prefetched rows
+  // are counted exactly.
+  // i_beg is a multiple of RW, and RW of NW: row i_beg + k sits in slot
+  // k mod NW at every pass.
+  static_assert(RW % NW == 0, "rows per strip must be a multiple of the ring length");
+  // Rows i_beg-1 .. i_beg+PD into the ring.
+#pragma unroll
+  for (int k = -1; k <= PD; ++k) {
+    const int s = (k + NW) % NW;
+    load(i_beg + k, i_end, w[s], wl[s], wh[s]);
+  }
+  // FULL unrolls the passes too (no loop back-edge).
+#pragma unroll(FULL ? RW / NW : 1)
+  for (int p = 0; p < RW / NW; ++p) {
+    const int ib = i_beg + p * NW;
+    if (ib >= i_end) break;
+#pragma unroll
+    for (int k = 0; k < NW; ++k) {
+      const int i = ib + k;
+      if (i >= i_end) break;
+      // Prefetch row i+1+PD into the slot of row i-2 (rows past i_end read
+      // as 0).
+      const int sp = (k + PD + 1) % NW, su = (k + NW - 1) % NW, sc = k % NW, sd = (k + 1) % NW;
+      load(i + 1 + PD, i_end, w[sp], wl[sp], wh[sp]);
+      C lf[F], rt[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        lf[f] = wave_shr1<C>(w[sc][f], wh[sc][f]);
+        rt[f] = wave_shl1<C>(w[sc][f], wh[sc][f]);
       }
-      out_hi[f * plane + e] = hv;
-      if (P::kComp) out_lo[f * plane + e] = lv;
-      if (sn_hi) {
-        sn_hi[f * plane + e] = hv;
-        if (P::kComp) sn_lo[f * plane + e] = lv;
+      Quant<C> q[F];
+      if (edge_strip) {
+        RowEdges<C> E[F];
+        double dv[F];
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          // The boundary block is read-only during the launch: read the
+          // row's (uniform) edge values through the scalar cache.
+          using cdbl = const __attribute__((address_space(4))) double;
+          cdbl *fb = (cdbl *)(bcg + f * fstride);
+          cdbl *nb = fb + 2 * (n0 + n1);
+          E[f].d_lo = fb[2 * n1 + i];
+          E[f].d_hi = fb[2 * n1 + n0 + i];
+          E[f].n_lo = static_cast<C>(nb[2 * n1 + i]);
+          E[f].n_hi = static_cast<C>(nb[2 * n1 + n0 + i]);
+          dv[f] = lo1 ? E[f].d_lo : (hi1 ? E[f].d_hi : static_cast<double>(NAN));
+        }
+#define PML_FQR(F_)                                                                        \
+  if constexpr (F_ < F) {                                                                  \
+    field_quant_row<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD, true>(                          \
+        q[F_], w[sc][F_], lf[F_], rt[F_], w[su][F_], w[sd][F_], lo1, hi1, ax0, E[F_], K);  \
+  }
+        PML_FQR(0)
+        PML_FQR(1)
+        PML_FQR(2)
+#undef PML_FQR
+        finish(i, q, w[sc], wl[sc], dv);
+      } else {
+        const RowEdges<C> E{};
+#define PML_FQR(F_)                                                                        \
+  if constexpr (F_ < F) {                                                                  \
+    field_quant_row<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD, false>(                         \
+        q[F_], w[sc][F_], lf[F_], rt[F_], w[su][F_], w[sd][F_], false, false, ax0, E, K);  \
+  }
+        PML_FQR(0)
+        PML_FQR(1)
+        PML_FQR(2)
+#undef PML_FQR
+        finish(i, q, w[sc], wl[sc], nullptr);
       }
+    }
+  }
+
+  // First and last grid rows (2-D): the generic per-vertex boundary code,
+  // with their rows loaded again (cache hits).
+  if (!ax0) return;
+#pragma unroll 1
+  for (int e = 0; e < 2; ++e) {
+    const int i = e == 0 ? 0 : n0 - 1;
+    if (i < i_beg || i >= i_end) continue;
+    C u[F], c[F], cl[F], d[F], ch[F], dummy[F];
+    load(i - 1, n0, u, dummy, dummy);
+    load(i, n0, c, cl, ch);
+    load(i + 1, n0, d, dummy, dummy);
+    C lf[F], rt[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      lf[f] = wave_shr1<C>(c[f], ch[f]);
+      rt[f] = wave_shl1<C>(c[f], ch[f]);
+    }
+    auto Y = [&](int f, int dir) -> C {
+      switch (dir) {
+        case 0: return c[f];
+        case 1: return lf[f];
+        case 2: return rt[f];
+        case 3: return u[f];
+        default: return d[f];
+      }
+    };
+    Quant<C> q[F];
+    all_quants<Eq, C>(q, Y, lo1, hi1, i == 0, i == n0 - 1, ax0, bcg, fstride, i, jj, n0, n1, K);
+    double dv[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) dv[f] = dirichlet_at(bcg + f * fstride, n0, n1, i, jj, ax0);
+    finish(i, q, c, cl, dv);
+  }
+}
+
+// Imposes the Dirichlet values on a state in place: one workgroup per
+// (grid, field) walks the grid's edge vertices.
+template <class P>
+__global__ __launch_bounds__(256) void fdm_constrain(const GridDev *grids, const double *bc,
+                                                     void *st_hi, void *st_lo, int ndim) {
+  using C = typename P::C;
+  const GridDev G = grids[blockIdx.x];
+  const int f = blockIdx.y;
+  const int n0 = G.n0, n1 = G.n1;
+  const bool ax0 = ndim == 2;
+  const double *fb = bc + G.bc + static_cast<int64_t>(f) * 4 * (n0 + n1);
+  const int64_t base = G.off + static_cast<int64_t>(f) * n0 * n1;
+  const int n_edge = 2 * (n0 + n1);
+  for (int t = threadIdx.x; t < n_edge; t += blockDim.x) {
+    int i, jc;
+    if (t < n1) {
+      i = 0;
+      jc = t;
+    } else if (t < 2 * n1) {
+      i = n0 - 1;
+      jc = t - n1;
+    } else if (t < 2 * n1 + n0) {
+      i = t - 2 * n1;
+      jc = 0;
+    } else {
+      i = t - 2 * n1 - n0;
+      jc = n1 - 1;
+    }
+    const double dv = dirichlet_at(fb, n0, n1, i, jc, ax0);
+    if (dv == dv) {
+      C hi, lo;
+      state_set<P>(hi, lo, dv);
+      Store<P>::store(st_hi, st_lo, base + static_cast<int64_t>(i) * n1 + jc, hi, lo);
     }
   }
 }
@@ -446,17 +681,12 @@ __global__ __launch_bounds__(NT) void fdm_fused(FdmArgs a) {
       const int r = idx / EX, c = idx - (idx / EX) * EX;
       const int gi = bi + r, gj = bj + c;
       const bool in = gi >= 0 && gi < n0 && gj >= 0 && gj < n1;
-      const bool edge = gj == 0 || gj == n1 - 1 || (ax0 && (gi == 0 || gi == n0 - 1));
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         C hi = 0, lo = 0;
         if (in) {
           Store<P>::load(a.in_hi, a.in_lo, G.off + f * plane + static_cast<int64_t>(gi) * n1 + gj,
                          hi, lo);
-          if (a.constrain_input && edge) {
-            const double dv = dirichlet_at(bcg + f * fstride, n0, n1, gi, gj, ax0);
-            if (dv == dv) state_set<P>(hi, lo, dv);
-          }
         }
         yh[k][f] = hi;
         yl[k][f] = lo;
@@ -622,10 +852,10 @@ struct Variant {
     &fdm_fused<EQ, IN, PR, OX, OY, ST, NT>, &fdm_fused<EQ, IN, PR, OX, OY, 1, NT>,      \
         kTileKind, OX, OY, ST, NT, NAME                                                 \
   }
-#define PML_STREAM(EQ, IN, PR, RW, NAME)                                                \
+#define PML_STREAM(EQ, IN, PR, RW, PD, FULL, NAME)                                      \
   Variant {                                                                             \
-    &fdm_stream<EQ, IN, PR, RW>, &fdm_stream<EQ, IN, PR, RW>, kStreamKind, 64, RW, 1, 256, \
-        NAME                                                                            \
+    &fdm_stream<EQ, IN, PR, RW, PD, FULL>, &fdm_stream<EQ, IN, PR, RW, PD, FULL>, kStreamKind, \
+        64, RW, 1, 256, NAME                                                            \
   }
 
 // Variants per (equation, integrator, precision); the first is the default.
@@ -636,9 +866,15 @@ const std::vector<Variant> &variants_for() {
     constexpr int S = EQ::kInvariant ? 1 : IN::kStages;
     constexpr bool wide = sizeof(typename PR::C) == 8;
     if constexpr (S == 1) {
-      out.push_back(PML_STREAM(EQ, IN, PR, 8, "stream-rw8"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 4, "stream-rw4"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 16, "stream-rw16"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, false, "stream-rw8-pd1"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 4, 1, false, "stream-rw4-pd1"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 16, 1, false, "stream-rw16-pd1"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 10, 2, false, "stream-rw10-pd2"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 20, 2, false, "stream-rw20-pd2"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 32, 1, false, "stream-rw32-pd1"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, "stream-rw8-pd1-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 16, 1, true, "stream-rw16-pd1-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 10, 2, true, "stream-rw10-pd2-full"));
       if constexpr (wide) {
         out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
       } else {
