@@ -229,9 +229,12 @@ int pml_prolong2x(int precision, const void *coarse, void *fine,
  *                         flipped backward layouts (each may be NULL).
  *   pml_surrogate_normalize: per-sample scale s = max|u| and x = u / s,
  *                         u field-major [B][C][HW] FP32/FP64 -> x NHWC.
- *   pml_surrogate_residual : out = s (x + y), back to field-major.
- *   pml_surrogate_loss     : loss += sum (x + y - t/s)^2 (FP64 scalar) and
- *                         dy = 2 (x + y - t/s) / n (BF16 NHWC).
+ *   pml_surrogate_residual : out = u + s y, u and out field-major FP32/FP64
+ *                         (the identity part never passes through BF16;
+ *                         out may alias u).
+ *   pml_surrogate_loss     : loss += sum (y - (t - u)/s)^2 (FP64 scalar) and
+ *                         dy = 2 (y - (t - u)/s) / n (BF16 NHWC); u and t
+ *                         field-major FP32.
  */
 int pml_conv64(int mode, const void *in, const void *wk, const float *bias,
                const void *zprev, void *z_out, void *a_out, int B, int H, int W,
@@ -249,10 +252,10 @@ int pml_adam(float *p, float *m, float *v, const float *g, int64_t n, float lr,
              void *wf, void *wb, int ci, int co, void *stream);
 int pml_surrogate_normalize(int precision, const void *u, void *x, float *s,
                             int B, int C, int64_t HW, void *stream);
-int pml_surrogate_residual(int precision, const void *x, const float *y,
+int pml_surrogate_residual(int precision, const void *u, const float *y,
                            const float *s, void *out, int B, int C, int64_t HW,
                            void *stream);
-int pml_surrogate_loss(const void *x, const float *y, const float *s,
+int pml_surrogate_loss(const float *u, const float *y, const float *s,
                        const float *target, void *dy, double *loss, int B,
                        int C, int64_t HW, void *stream);
 

@@ -12,8 +12,10 @@ be checked:
   C_in -> 64 -> 64 -> 64 -> 64 -> 64 -> C_out, tanh-form GELU after the
   first five, no activation after the last;
 * the input is normalised by the per-sample scale s = max |u| and the
-  prediction is ``s * (x + net(x))`` (a residual step);
-* the loss is the mean over pixels and channels of ``(x + net(x) - t/s)^2``.
+  prediction is ``u + s * net(x)`` (an increment step; the identity part is
+  not rounded);
+* the loss is the mean over pixels and channels of ``(net(x) - d)^2`` with
+  the normalised increment ``d = (t - u) / s``.
 
 Arithmetic is FP64, with BF16 round-to-nearest-even applied exactly where the
 device stores BF16 (weights, activations, pre-activations, activation
@@ -128,14 +130,13 @@ def forward(params, x, rounding=True):
     return y, (cache, zs)
 
 
-def loss_and_grads(params, x, target, rounding=True):
-    """MSE loss of ``x + net(x)`` against ``target`` (already normalised) and
+def loss_and_grads(params, x, inc, rounding=True):
+    """MSE loss of ``net(x)`` against the normalised increment ``inc`` and
     the parameter gradients, following the device's rounding points."""
     r = bf16 if rounding else (lambda v: v)
     y, (acts, zs) = forward(params, x, rounding)
-    pred = acts[0] + y
-    n = pred.size
-    diff = pred - target
+    n = y.size
+    diff = y - inc
     loss = float(np.mean(diff**2))
     dz = r(2.0 * diff / n)
     grads = [None] * N_LAYERS
@@ -175,8 +176,8 @@ def adam_step(params, grads, state, lr, beta1=0.9, beta2=0.999, eps=1e-8):
 
 
 def predict(params, u):
-    """One surrogate step on raw states u (B, H, W, C)."""
+    """One surrogate step on raw states u (B, H, W, C): u + s net(u / s)."""
     s = np.maximum(np.abs(u).reshape(u.shape[0], -1).max(axis=1), 1e-30)
     x = u / s[:, None, None, None]
-    y, (acts, _) = forward(params, x)
-    return s[:, None, None, None] * (acts[0] + y)
+    y, _ = forward(params, x)
+    return u + s[:, None, None, None] * y

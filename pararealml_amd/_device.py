@@ -246,6 +246,7 @@ class FDMBatch:
             self.lo, self.work_lo = mk(), mk()
         else:
             self.lo = self.work_lo = None
+        self._consume(self.bc, self.hi, self.work_hi, self.lo, self.work_lo)
         self._params = (ctypes.c_double * 6)(*self.params, *([0.0] * (6 - len(self.params))))
 
     # ------------------------------------------------------------------ I/O
@@ -256,8 +257,32 @@ class FDMBatch:
     def _sptr(self):
         return ctypes.c_void_p(self.stream.cuda_stream)
 
+    # The batch's kernels run on ``self.stream``; tensors handed in or out
+    # are ordered against the caller's current stream (a no-op when the two
+    # are the same stream).
+    def _consume(self, *tensors) -> None:
+        import torch
+
+        cur = torch.cuda.current_stream()
+        if cur != self.stream:
+            self.stream.wait_stream(cur)
+            for t in tensors:
+                if t is not None:
+                    t.record_stream(self.stream)
+
+    def _produce(self, *tensors) -> None:
+        import torch
+
+        cur = torch.cuda.current_stream()
+        if cur != self.stream:
+            cur.wait_stream(self.stream)
+            for t in tensors:
+                if t is not None:
+                    t.record_stream(self.stream)
+
     def load_f64(self, flat_f64) -> None:
         """Sets the state from a device FP64 tensor of ``total`` elements."""
+        self._consume(flat_f64)
         nat.check(
             self.lib.pml_from_f64(
                 self.prec, flat_f64.data_ptr(), self._ptr(self.hi), self._ptr(self.lo),
@@ -271,12 +296,14 @@ class FDMBatch:
 
         if out is None:
             out = torch.empty(self.total, dtype=torch.float64, device=self.device)
+        self._consume(out)
         nat.check(
             self.lib.pml_to_f64(
                 self.prec, self._ptr(self.hi), self._ptr(self.lo), out.data_ptr(),
                 self.total, self._sptr(),
             )
         )
+        self._produce(out)
         return out
 
     def load_host(self, ys: Sequence[np.ndarray]) -> None:
@@ -313,6 +340,7 @@ class FDMBatch:
         snap_hi = snap_lo = None
         if snapshots is not None:
             snap_hi, snap_lo = snapshots
+            self._consume(snap_hi, snap_lo)
         nat.check(
             self.lib.pml_fdm_solve(
                 self.eq_id,
@@ -357,12 +385,14 @@ class FDMBatch:
 
         hi, lo = snaps
         out = torch.empty(n * self.total, dtype=torch.float64, device=self.device)
+        self._consume(out)
         nat.check(
             self.lib.pml_to_f64(
                 self.prec, hi.data_ptr(), self._ptr(lo), out.data_ptr(), n * self.total,
                 self._sptr(),
             )
         )
+        self._produce(out)
         return out.view(n, self.total)
 
     def grid_view(self, flat, k: int):

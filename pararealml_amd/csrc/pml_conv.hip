@@ -486,9 +486,11 @@ __global__ __launch_bounds__(256) void k_normalize(const T *u, bf16 *x, float *s
   }
 }
 
-// Prediction u_next = s (x + y) back to field-major FP64 / FP32.
+// Prediction u_next = u + s y (field-major FP64 / FP32): the identity part
+// stays in the state's precision, only the learned increment passes through
+// the BF16 network.  out may alias u.
 template <typename T>
-__global__ __launch_bounds__(256) void k_residual(const bf16 *x, const float *y, const float *s,
+__global__ __launch_bounds__(256) void k_residual(const T *u, const float *y, const float *s,
                                                   T *out, int C, int64_t HW, int B) {
   const int64_t total = static_cast<int64_t>(B) * C * HW;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total;
@@ -498,13 +500,14 @@ __global__ __launch_bounds__(256) void k_residual(const bf16 *x, const float *y,
     const int c = static_cast<int>(r / HW);
     const int64_t px = r - c * HW;
     const int64_t e = (b * HW + px) * C + c;
-    out[i] = static_cast<T>(s[b] * (static_cast<float>(x[e]) + y[e]));
+    out[i] = u[i] + static_cast<T>(s[b]) * static_cast<T>(y[e]);
   }
 }
 
-// MSE loss of x + y against target / s (target field-major FP32) and the
-// BF16 gradient 2 (x + y - t) / n of the network output.
-__global__ __launch_bounds__(256) void k_loss(const bf16 *x, const float *y, const float *s,
+// MSE loss of the network output y against the normalised increment
+// (target - u) / s (u, target field-major FP32) and the BF16 gradient
+// 2 (y - (target - u) / s) / n of the network output.
+__global__ __launch_bounds__(256) void k_loss(const float *u, const float *y, const float *s,
                                               const float *target, bf16 *dy, double *loss, int C,
                                               int64_t HW, int B) {
   const int64_t total = static_cast<int64_t>(B) * C * HW;
@@ -517,7 +520,7 @@ __global__ __launch_bounds__(256) void k_loss(const bf16 *x, const float *y, con
     const int c = static_cast<int>(r / HW);
     const int64_t px = r - c * HW;
     const int64_t e = (b * HW + px) * C + c;
-    const float d = (static_cast<float>(x[e]) + y[e]) - target[i] / s[b];
+    const float d = y[e] - (target[i] - u[i]) / s[b];
     acc += static_cast<double>(d) * d;
     dy[e] = static_cast<bf16>(d * inv_n);
   }
@@ -726,33 +729,33 @@ extern "C" int pml_surrogate_normalize(int precision, const void *u, void *x, fl
   return PML_OK;
 }
 
-extern "C" int pml_surrogate_residual(int precision, const void *x, const float *y,
+extern "C" int pml_surrogate_residual(int precision, const void *u, const float *y,
                                       const float *s, void *out, int B, int C, int64_t HW,
                                       void *stream) {
-  if (B <= 0 || C <= 0 || HW <= 0 || !x || !y || !s || !out)
+  if (B <= 0 || C <= 0 || HW <= 0 || !u || !y || !s || !out)
     return fail(PML_E_INVALID, "bad residual arguments");
   hipStream_t st = as_stream(stream);
   const int nb = n_blocks(static_cast<int64_t>(B) * C * HW);
   if (precision == PML_F64)
     hipLaunchKernelGGL(k_residual<double>, dim3(nb), dim3(256), 0, st,
-                       static_cast<const bf16 *>(x), y, s, static_cast<double *>(out), C, HW, B);
+                       static_cast<const double *>(u), y, s, static_cast<double *>(out), C, HW, B);
   else if (precision == PML_F32)
     hipLaunchKernelGGL(k_residual<float>, dim3(nb), dim3(256), 0, st,
-                       static_cast<const bf16 *>(x), y, s, static_cast<float *>(out), C, HW, B);
+                       static_cast<const float *>(u), y, s, static_cast<float *>(out), C, HW, B);
   else
-    return fail(PML_E_INVALID, "residual writes FP32 or FP64");
+    return fail(PML_E_INVALID, "residual works in FP32 or FP64");
   PML_LAUNCH_CHECK();
   return PML_OK;
 }
 
-extern "C" int pml_surrogate_loss(const void *x, const float *y, const float *s,
+extern "C" int pml_surrogate_loss(const float *u, const float *y, const float *s,
                                   const float *target, void *dy, double *loss, int B, int C,
                                   int64_t HW, void *stream) {
-  if (B <= 0 || C <= 0 || HW <= 0 || !x || !y || !s || !target || !dy || !loss)
+  if (B <= 0 || C <= 0 || HW <= 0 || !u || !y || !s || !target || !dy || !loss)
     return fail(PML_E_INVALID, "bad loss arguments");
   hipLaunchKernelGGL(k_loss, dim3(n_blocks(static_cast<int64_t>(B) * C * HW, 2048)), dim3(256),
-                     0, as_stream(stream), static_cast<const bf16 *>(x), y, s, target,
-                     static_cast<bf16 *>(dy), loss, C, HW, B);
+                     0, as_stream(stream), u, y, s, target, static_cast<bf16 *>(dy), loss, C, HW,
+                     B);
   PML_LAUNCH_CHECK();
   return PML_OK;
 }

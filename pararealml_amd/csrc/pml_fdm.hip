@@ -113,7 +113,25 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
   const bool use_graph = flags & PML_FLAG_GRAPH;
   const bool constrain = flags & PML_FLAG_CONSTRAIN_INPUT;
 
-  const Variant &V = (*vars)[tile_config];
+  // Two-column stream variants move two values per lane instruction: every
+  // grid's row length and state offset must be even and the planes aligned
+  // to two values; otherwise the default variant (bitwise-identical
+  // results) runs instead.
+  const size_t esz_v = precision == PML_F64 ? 8 : 4;
+  auto cpl_ok = [&](const Variant &v) {
+    if (v.cpl == 1) return true;
+    const uintptr_t align = static_cast<uintptr_t>(v.cpl) * esz_v;
+    for (const void *ptr : {static_cast<const void *>(state_hi), static_cast<const void *>(state_lo),
+                            static_cast<const void *>(work_hi), static_cast<const void *>(work_lo),
+                            static_cast<const void *>(snapshots_hi),
+                            static_cast<const void *>(snapshots_lo)})
+      if (reinterpret_cast<uintptr_t>(ptr) % align) return false;
+    if ((total_elems * static_cast<int64_t>(esz_v)) % static_cast<int64_t>(align)) return false;
+    for (int g = 0; g < n_grids; ++g)
+      if (grids[g].n1 % v.cpl || grids[g].state_offset % v.cpl) return false;
+    return true;
+  };
+  const Variant &V = cpl_ok((*vars)[tile_config]) ? (*vars)[tile_config] : (*vars)[0];
   // Fused steps per launch must divide the snapshot interval.
   const bool can_fuse = V.steps > 1 && (!snapshots_hi || snapshot_every % V.steps == 0);
   const int steps_per_launch = can_fuse ? V.steps : 1;
@@ -183,6 +201,7 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
                                : nullptr;
       ++snap_idx;
     }
+    if (a.snap_hi && V.kind == kStreamKind) fn = V.fn_snap;
     hipLaunchKernelGGL(fn, grid_dim, block_dim, 0, st, a);
     PML_LAUNCH_CHECK();
     std::swap(cur_hi, nxt_hi);

@@ -368,35 +368,55 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t row_rsrc(const void *p, unsign
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, static_cast<int>(bytes),
                                            0x00020000);
 }
-template <typename C>
-__device__ __forceinline__ C buf_load(__amdgpu_buffer_rsrc_t r, unsigned off);
-template <>
-__device__ __forceinline__ float buf_load<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+
+// N consecutive values of type C (4 * N * sizeof(C) / 4 bytes, one buffer
+// instruction) at byte offset `off`.
+template <typename C, int N>
+struct Vec {
+  C v[N];
+};
+template <typename C, int N>
+__device__ __forceinline__ Vec<C, N> buf_load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  constexpr int kBytes = static_cast<int>(sizeof(C)) * N;
+  if constexpr (kBytes == 4) {
+    return __builtin_bit_cast(Vec<C, N>, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  } else if constexpr (kBytes == 8) {
+    return __builtin_bit_cast(Vec<C, N>, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  } else {
+    static_assert(kBytes == 16, "4, 8 or 16 bytes per lane");
+    return __builtin_bit_cast(Vec<C, N>, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  }
 }
-template <>
-__device__ __forceinline__ double buf_load<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
-  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
-}
-template <typename C>
-__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off, C x);
-template <>
-__device__ __forceinline__ void buf_store<float>(__amdgpu_buffer_rsrc_t r, unsigned off, float x) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, off, 0, 0);
-}
-template <>
-__device__ __forceinline__ void buf_store<double>(__amdgpu_buffer_rsrc_t r, unsigned off,
-                                                  double x) {
+template <typename C, int N>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                          const Vec<C, N> &x) {
+  constexpr int kBytes = static_cast<int>(sizeof(C)) * N;
   using u2 = unsigned __attribute__((ext_vector_type(2)));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, off, 0, 0);
+  using u4 = unsigned __attribute__((ext_vector_type(4)));
+  if constexpr (kBytes == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, off, 0, 0);
+  } else if constexpr (kBytes == 8) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, off, 0, 0);
+  } else {
+    static_assert(kBytes == 16, "4, 8 or 16 bytes per lane");
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, x), r, off, 0, 0);
+  }
 }
 
-template <class Eq, class Integ, class P, int RW, int PD, bool FULL>
+// CPL columns per lane: the strip is 64 CPL columns wide, lane l owns
+// columns j0 + CPL l .. j0 + CPL l + CPL - 1 and moves them with one
+// CPL-wide buffer instruction per row and field (CPL = 2 needs every grid's
+// row length and state offset even, which the host checks).  SNAP compiles
+// the snapshot stores in (launches that write a snapshot) or out.
+template <class Eq, class Integ, class P, int RW, int PD, bool FULL, int CPL, bool SNAP>
 __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
   using C = typename P::C;
+  using V = Vec<C, CPL>;
   constexpr int F = Eq::F;
   constexpr int NW = PD + 3;  // ring slots
+  constexpr int SW = 64 * CPL;  // strip width
   static_assert(Eq::kInvariant || Integ::kStages == 1, "single-stage steps only");
+  static_assert(CPL == 1 || CPL == 2, "one or two columns per lane");
   const int lane = threadIdx.x & 63;
   // The unit is the same for the whole wavefront: keep it (and everything
   // derived from it) in scalar registers.
@@ -406,8 +426,7 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
   const TileDev U = a.tiles[unit];
   const GridDev G = a.grids[U.g];
   const int n0 = G.n0, n1 = G.n1;
-  const int j = U.j0 + lane;
-  const bool act = j < n1;
+  const int jl = U.j0 + CPL * lane;  // the lane's first column
   const bool ax0 = a.ndim == 2;
   const unsigned plane = static_cast<unsigned>(n0) * n1;
   const double *bcg = a.bc + G.bc;
@@ -419,73 +438,102 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
   const C dt = static_cast<C>(a.dt);
   const int i_beg = U.i0;
   const int i_end = min(U.i0 + RW, n0);
-  const bool edge_strip = U.j0 == 0 || U.j0 + 64 >= n1;  // wave-uniform
-  const bool lo1 = j == 0, hi1 = j == n1 - 1;
-  const int hj = lane == 0 ? j - 1 : j + 1;
-  const int jj = act ? j : 0;  // column for the per-lane boundary lookups
+  const bool edge_strip = U.j0 == 0 || U.j0 + SW >= n1;  // wave-uniform
+  bool lo1[CPL], hi1[CPL];
+  int jj[CPL];  // columns for the per-lane boundary lookups (clamped)
+#pragma unroll
+  for (int c = 0; c < CPL; ++c) {
+    lo1[c] = jl + c == 0;
+    hi1[c] = jl + c == n1 - 1;
+    jj[c] = min(jl + c, n1 - 1);
+  }
 
   const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
   const C *in_lo = P::kComp ? static_cast<const C *>(a.in_lo) + G.off : nullptr;
   C *out_hi = static_cast<C *>(a.out_hi) + G.off;
   C *out_lo = P::kComp ? static_cast<C *>(a.out_lo) + G.off : nullptr;
-  C *sn_hi = a.snap_hi ? static_cast<C *>(a.snap_hi) + G.off : out_hi;
-  C *sn_lo = (a.snap_hi && P::kComp) ? static_cast<C *>(a.snap_lo) + G.off : out_lo;
+  C *sn_hi = SNAP ? static_cast<C *>(a.snap_hi) + G.off : nullptr;
+  C *sn_lo = (SNAP && P::kComp) ? static_cast<C *>(a.snap_lo) + G.off : nullptr;
 
-  // Ring slot s holds row r with r = s (mod NW): value, lo part, halo value.
-  C w[NW][F], wl[NW][F], wh[NW][F];
-  // Lane byte offsets into a row: the lane's column and its halo column
-  // (lanes 0 / 63 only).  Offsets outside the row read 0 (and store
-  // nothing) through the row's buffer range.
-  const unsigned jb = static_cast<unsigned>(j) * static_cast<unsigned>(sizeof(C));
-  const unsigned hb = static_cast<unsigned>(lane == 0 || lane == 63 ? hj : j) *
-                      static_cast<unsigned>(sizeof(C));
+  // Ring slot s holds row r with r = s (mod NW): values, lo parts, halo.
+  V w[NW][F], wl[NW][F];
+  C wh[NW][F];
+  // Lane byte offsets into a row: the lane's columns and its halo column
+  // (lanes 0 / 63: the columns left / right of the strip).  Offsets outside
+  // the row read 0 (and store nothing) through the row's buffer range.
+  const unsigned jb = static_cast<unsigned>(jl) * static_cast<unsigned>(sizeof(C));
+  const int hcol = lane == 0 ? U.j0 - 1 : (lane == 63 ? U.j0 + SW : jl);
+  const unsigned hb = static_cast<unsigned>(hcol) * static_cast<unsigned>(sizeof(C));
   const unsigned row_bytes = static_cast<unsigned>(n1) * static_cast<unsigned>(sizeof(C));
-  const unsigned snap_bytes = a.snap_hi ? row_bytes : 0u;
   // Row r of the input; rows outside [0, n0) and past `last` read as 0
   // without touching memory (empty buffer range), so that the load is
   // unconditional.
-  auto load = [&](int r, int last, C *v, C *vl, C *h) {
+  auto load = [&](int r, int last, V *v, V *vl, C *h) {
     const bool ok = r >= 0 && r < n0 && r <= last;
     const size_t ro = static_cast<size_t>(min(max(r, 0), n0 - 1)) * n1;
     const unsigned nb = ok ? row_bytes : 0u;
 #pragma unroll
     for (int f = 0; f < F; ++f) {
       const auto rs = row_rsrc(in_hi + static_cast<size_t>(f) * plane + ro, nb);
-      v[f] = buf_load<C>(rs, jb);
-      if (P::kComp) vl[f] = buf_load<C>(row_rsrc(in_lo + static_cast<size_t>(f) * plane + ro, nb), jb);
-      h[f] = buf_load<C>(rs, hb);
+      v[f] = buf_load<C, CPL>(rs, jb);
+      if (P::kComp)
+        vl[f] = buf_load<C, CPL>(row_rsrc(in_lo + static_cast<size_t>(f) * plane + ro, nb), jb);
+      h[f] = buf_load<C, 1>(rs, hb).v[0];
+    }
+  };
+
+  // Neighbours of the lane's cells along axis 1: west[c], east[c].
+  auto lateral = [&](const V *x, const C *h, C (*west)[CPL], C (*east)[CPL]) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      west[f][0] = wave_shr1<C>(x[f].v[CPL - 1], h[f]);
+      east[f][CPL - 1] = wave_shl1<C>(x[f].v[0], h[f]);
+#pragma unroll
+      for (int c = 1; c < CPL; ++c) {
+        west[f][c] = x[f].v[c - 1];
+        east[f][c - 1] = x[f].v[c];
+      }
     }
   };
 
   // Writes row i: update, Dirichlet values, store.
-  auto finish = [&](int i, const Quant<C> *q, const C *ct, const C *ctl, const double *dv) {
-    C rr[F], ra[F];
+  auto finish = [&](int i, Quant<C> (*q)[F], const V *ct, const V *ctl, double (*dv)[F]) {
+    V ho[F], lo[F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) rr[f] = ra[f] = 0;
-    Eq::template rhs<C>(pc, q, rr);
-    if (Eq::kAlg != 0u) Eq::template alg<C>(pc, q, ra);
+    for (int c = 0; c < CPL; ++c) {
+      C rr[F], ra[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) rr[f] = ra[f] = 0;
+      Eq::template rhs<C>(pc, q[c], rr);
+      if (Eq::kAlg != 0u) Eq::template alg<C>(pc, q[c], ra);
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        C hv = ct[f].v[c], lv = P::kComp ? ctl[f].v[c] : C(0);
+        if ((Eq::kAlg >> f) & 1u) {
+          hv = ra[f];
+          lv = 0;
+        } else {
+          const C kk = dt * rr[f];
+          C d = kk;
+          if (Eq::kInvariant && Integ::kStages == 4)
+            d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
+          state_add<P>(hv, lv, d);
+        }
+        if (dv && dv[c][f] == dv[c][f]) state_set<P>(hv, lv, dv[c][f]);
+        ho[f].v[c] = hv;
+        lo[f].v[c] = lv;
+      }
+    }
     const size_t ro = static_cast<size_t>(i) * n1;
-    auto put = [&](C *base, unsigned nb, int f, C x) {
-      buf_store<C>(row_rsrc(base + static_cast<size_t>(f) * plane + ro, nb), jb, x);
-    };
 #pragma unroll
     for (int f = 0; f < F; ++f) {
-      C hv = ct[f], lv = P::kComp ? ctl[f] : C(0);
-      if ((Eq::kAlg >> f) & 1u) {
-        hv = ra[f];
-        lv = 0;
-      } else {
-        const C kk = dt * rr[f];
-        C d = kk;
-        if (Eq::kInvariant && Integ::kStages == 4) d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
-        state_add<P>(hv, lv, d);
+      const size_t fo = static_cast<size_t>(f) * plane + ro;
+      buf_store<C, CPL>(row_rsrc(out_hi + fo, row_bytes), jb, ho[f]);
+      if (P::kComp) buf_store<C, CPL>(row_rsrc(out_lo + fo, row_bytes), jb, lo[f]);
+      if (SNAP) {
+        buf_store<C, CPL>(row_rsrc(sn_hi + fo, row_bytes), jb, ho[f]);
+        if (P::kComp) buf_store<C, CPL>(row_rsrc(sn_lo + fo, row_bytes), jb, lo[f]);
       }
-      if (dv && dv[f] == dv[f]) state_set<P>(hv, lv, dv[f]);
-      put(out_hi, row_bytes, f, hv);
-      if (P::kComp) put(out_lo, row_bytes, f, lv);
-      // Snapshot stores go to an empty range when no snapshot is taken.
-      put(sn_hi, snap_bytes, f, hv);
-      if (P::kComp) put(sn_lo, snap_bytes, f, lv);
     }
   };
 
@@ -517,16 +565,12 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
       // as 0).
       const int sp = (k + PD + 1) % NW, su = (k + NW - 1) % NW, sc = k % NW, sd = (k + 1) % NW;
       load(i + 1 + PD, i_end, w[sp], wl[sp], wh[sp]);
-      C lf[F], rt[F];
-#pragma unroll
-      for (int f = 0; f < F; ++f) {
-        lf[f] = wave_shr1<C>(w[sc][f], wh[sc][f]);
-        rt[f] = wave_shl1<C>(w[sc][f], wh[sc][f]);
-      }
-      Quant<C> q[F];
+      C west[F][CPL], east[F][CPL];
+      lateral(w[sc], wh[sc], west, east);
+      Quant<C> q[CPL][F];
       if (edge_strip) {
         RowEdges<C> E[F];
-        double dv[F];
+        double dv[CPL][F];
 #pragma unroll
         for (int f = 0; f < F; ++f) {
           // The boundary block is read-only during the launch: read the
@@ -538,29 +582,39 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
           E[f].d_hi = fb[2 * n1 + n0 + i];
           E[f].n_lo = static_cast<C>(nb[2 * n1 + i]);
           E[f].n_hi = static_cast<C>(nb[2 * n1 + n0 + i]);
-          dv[f] = lo1 ? E[f].d_lo : (hi1 ? E[f].d_hi : static_cast<double>(NAN));
+#pragma unroll
+          for (int c = 0; c < CPL; ++c)
+            dv[c][f] = lo1[c] ? E[f].d_lo : (hi1[c] ? E[f].d_hi : static_cast<double>(NAN));
         }
-#define PML_FQR(F_)                                                                        \
-  if constexpr (F_ < F) {                                                                  \
-    field_quant_row<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD, true>(                          \
-        q[F_], w[sc][F_], lf[F_], rt[F_], w[su][F_], w[sd][F_], lo1, hi1, ax0, E[F_], K);  \
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+#define PML_FQR(F_)                                                                         \
+  if constexpr (F_ < F) {                                                                   \
+    field_quant_row<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD, true>(                           \
+        q[c][F_], w[sc][F_].v[c], west[F_][c], east[F_][c], w[su][F_].v[c], w[sd][F_].v[c], \
+        lo1[c], hi1[c], ax0, E[F_], K);                                                     \
   }
-        PML_FQR(0)
-        PML_FQR(1)
-        PML_FQR(2)
+          PML_FQR(0)
+          PML_FQR(1)
+          PML_FQR(2)
 #undef PML_FQR
+        }
         finish(i, q, w[sc], wl[sc], dv);
       } else {
         const RowEdges<C> E{};
-#define PML_FQR(F_)                                                                        \
-  if constexpr (F_ < F) {                                                                  \
-    field_quant_row<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD, false>(                         \
-        q[F_], w[sc][F_], lf[F_], rt[F_], w[su][F_], w[sd][F_], false, false, ax0, E, K);  \
+#pragma unroll
+        for (int c = 0; c < CPL; ++c) {
+#define PML_FQR(F_)                                                                         \
+  if constexpr (F_ < F) {                                                                   \
+    field_quant_row<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD, false>(                          \
+        q[c][F_], w[sc][F_].v[c], west[F_][c], east[F_][c], w[su][F_].v[c], w[sd][F_].v[c], \
+        false, false, ax0, E, K);                                                           \
   }
-        PML_FQR(0)
-        PML_FQR(1)
-        PML_FQR(2)
+          PML_FQR(0)
+          PML_FQR(1)
+          PML_FQR(2)
 #undef PML_FQR
+        }
         finish(i, q, w[sc], wl[sc], nullptr);
       }
     }
@@ -573,31 +627,33 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
   for (int e = 0; e < 2; ++e) {
     const int i = e == 0 ? 0 : n0 - 1;
     if (i < i_beg || i >= i_end) continue;
-    C u[F], c[F], cl[F], d[F], ch[F], dummy[F];
-    load(i - 1, n0, u, dummy, dummy);
-    load(i, n0, c, cl, ch);
-    load(i + 1, n0, d, dummy, dummy);
-    C lf[F], rt[F];
+    V u[F], cr[F], cl[F], d[F], dummy[F];
+    C ch[F], hdummy[F];
+    load(i - 1, n0, u, dummy, hdummy);
+    load(i, n0, cr, cl, ch);
+    load(i + 1, n0, d, dummy, hdummy);
+    C west[F][CPL], east[F][CPL];
+    lateral(cr, ch, west, east);
+    Quant<C> q[CPL][F];
+    double dv[CPL][F];
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      lf[f] = wave_shr1<C>(c[f], ch[f]);
-      rt[f] = wave_shl1<C>(c[f], ch[f]);
+    for (int c = 0; c < CPL; ++c) {
+      auto Y = [&](int f, int dir) -> C {
+        switch (dir) {
+          case 0: return cr[f].v[c];
+          case 1: return west[f][c];
+          case 2: return east[f][c];
+          case 3: return u[f].v[c];
+          default: return d[f].v[c];
+        }
+      };
+      all_quants<Eq, C>(q[c], Y, lo1[c], hi1[c], i == 0, i == n0 - 1, ax0, bcg, fstride, i,
+                        jj[c], n0, n1, K);
+#pragma unroll
+      for (int f = 0; f < F; ++f)
+        dv[c][f] = dirichlet_at(bcg + f * fstride, n0, n1, i, jj[c], ax0);
     }
-    auto Y = [&](int f, int dir) -> C {
-      switch (dir) {
-        case 0: return c[f];
-        case 1: return lf[f];
-        case 2: return rt[f];
-        case 3: return u[f];
-        default: return d[f];
-      }
-    };
-    Quant<C> q[F];
-    all_quants<Eq, C>(q, Y, lo1, hi1, i == 0, i == n0 - 1, ax0, bcg, fstride, i, jj, n0, n1, K);
-    double dv[F];
-#pragma unroll
-    for (int f = 0; f < F; ++f) dv[f] = dirichlet_at(bcg + f * fstride, n0, n1, i, jj, ax0);
-    finish(i, q, c, cl, dv);
+    finish(i, q, cr, cl, dv);
   }
 }
 
@@ -841,21 +897,25 @@ enum VariantKind { kTileKind = 0, kStreamKind = 1 };
 struct Variant {
   KernelFn fn;
   KernelFn fn_single;  // one step per launch, same unit shape
+  KernelFn fn_snap;    // fn for launches that write a snapshot
   int kind;            // kTileKind or kStreamKind
-  int ox, oy;          // unit shape: output tile (tile kind) or 64 x RW strip
+  int ox, oy;          // unit shape: output tile (tile kind) or 64 CPL x RW strip
   int steps, nt;
+  int cpl;             // columns per lane (stream kind)
   const char *name;
 };
 
 #define PML_TILE(EQ, IN, PR, OX, OY, ST, NT, NAME)                                     \
   Variant {                                                                             \
     &fdm_fused<EQ, IN, PR, OX, OY, ST, NT>, &fdm_fused<EQ, IN, PR, OX, OY, 1, NT>,      \
-        kTileKind, OX, OY, ST, NT, NAME                                                 \
+        &fdm_fused<EQ, IN, PR, OX, OY, ST, NT>, kTileKind, OX, OY, ST, NT, 1, NAME      \
   }
-#define PML_STREAM(EQ, IN, PR, RW, PD, FULL, NAME)                                      \
+#define PML_STREAM(EQ, IN, PR, RW, PD, FULL, CPL, NAME)                                 \
   Variant {                                                                             \
-    &fdm_stream<EQ, IN, PR, RW, PD, FULL>, &fdm_stream<EQ, IN, PR, RW, PD, FULL>, kStreamKind, \
-        64, RW, 1, 256, NAME                                                            \
+    &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, false>,                                  \
+        &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, false>,                              \
+        &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, true>, kStreamKind, 64 * CPL, RW, 1, \
+        256, CPL, NAME                                                                  \
   }
 
 // Variants per (equation, integrator, precision); the first is the default.
@@ -866,15 +926,18 @@ const std::vector<Variant> &variants_for() {
     constexpr int S = EQ::kInvariant ? 1 : IN::kStages;
     constexpr bool wide = sizeof(typename PR::C) == 8;
     if constexpr (S == 1) {
-      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, false, "stream-rw8-pd1"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 4, 1, false, "stream-rw4-pd1"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 16, 1, false, "stream-rw16-pd1"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 10, 2, false, "stream-rw10-pd2"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 20, 2, false, "stream-rw20-pd2"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 32, 1, false, "stream-rw32-pd1"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, "stream-rw8-pd1-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 16, 1, true, "stream-rw16-pd1-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 10, 2, true, "stream-rw10-pd2-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, 1, "stream-rw8-pd1-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, false, 1, "stream-rw8-pd1"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 4, 1, true, 1, "stream-rw4-pd1-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 16, 1, true, 1, "stream-rw16-pd1-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 10, 2, true, 1, "stream-rw10-pd2-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 6, 3, true, 1, "stream-rw6-pd3-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 12, 3, true, 1, "stream-rw12-pd3-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, 2, "stream-c2-rw8-pd1-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 4, 1, true, 2, "stream-c2-rw4-pd1-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 10, 2, true, 2, "stream-c2-rw10-pd2-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 6, 3, true, 2, "stream-c2-rw6-pd3-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 12, 3, true, 2, "stream-c2-rw12-pd3-full"));
       if constexpr (wide) {
         out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
       } else {

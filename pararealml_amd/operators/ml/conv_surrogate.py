@@ -11,8 +11,10 @@ the state by the operator's ``d_t`` -- with a different model:
 * six 3x3 "same" convolutions (zero padding, NHWC BF16 activations, FP32
   accumulation): C -> 64 -> 64 -> 64 -> 64 -> 64 -> C, tanh-form GELU after
   the first five;
-* the input is scaled by its per-sample maximum magnitude s and the output is
-  the residual step ``s (x + net(x))``;
+* the input is scaled by its per-sample maximum magnitude s, x = u / s, and
+  the output is the increment step ``u + s net(x)``: the identity part stays
+  in the state's FP32/FP64 precision and only the learned increment passes
+  through BF16 (training minimises ``(net(x) - (target - u) / s)^2``);
 * optionally the network works on the 2x-coarsened vertex grid (injection in,
   bilinear prolongation out).
 
@@ -61,8 +63,8 @@ class ConvSurrogate:
                 w[:] = 0.0
             params.append((w, np.zeros(co)))
         self._alloc()
+        self._ws_cache = {}
         self.set_params(params)
-        self._ws_key = None
 
     # ------------------------------------------------------------ parameters
     def _alloc(self):
@@ -144,8 +146,8 @@ class ConvSurrogate:
     # ------------------------------------------------------------ workspaces
     def _workspace(self, batch: int, h: int, w: int, training: bool):
         key = (batch, h, w, training)
-        if self._ws_key == key:
-            return self._ws
+        if key in self._ws_cache:
+            return self._ws_cache[key]
         t = self.torch
         px = batch * h * w
         bf = dict(dtype=t.bfloat16, device="cuda")
@@ -162,7 +164,9 @@ class ConvSurrogate:
             ws["dy"] = t.empty(px * c, **bf)
             ws["dz"] = [t.empty(px * HIDDEN, **bf) for _ in range(2)]
             ws["loss"] = t.zeros(1, dtype=t.float64, device="cuda")
-        self._ws, self._ws_key = ws, key
+        if len(self._ws_cache) >= 4:
+            self._ws_cache.clear()
+        self._ws_cache[key] = ws
         return ws
 
     # --------------------------------------------------------------- forward
@@ -202,7 +206,7 @@ class ConvSurrogate:
                 prec, cur.data_ptr(), ws["x"].data_ptr(), ws["s"].data_ptr(), batch, c, h * w, sp))
             self._forward(ws, batch, h, w, store_z=False)
             nat.check(self.lib.pml_surrogate_residual(
-                prec, ws["x"].data_ptr(), ws["y"].data_ptr(), ws["s"].data_ptr(),
+                prec, cur.data_ptr(), ws["y"].data_ptr(), ws["s"].data_ptr(),
                 res.data_ptr(), batch, c, h * w, sp))
             cur = res
         return res
@@ -224,7 +228,7 @@ class ConvSurrogate:
             g.zero_()
         ws["loss"].zero_()
         nat.check(lib.pml_surrogate_loss(
-            ws["x"].data_ptr(), ws["y"].data_ptr(), ws["s"].data_ptr(), target.data_ptr(),
+            u.data_ptr(), ws["y"].data_ptr(), ws["s"].data_ptr(), target.data_ptr(),
             ws["dy"].data_ptr(), ws["loss"].data_ptr(), batch, c, h * w, sp))
         last = N_LAYERS - 1
         # layer 5 (64 -> C): weight / bias gradients and the input gradient
@@ -282,6 +286,45 @@ class ConvSurrogate:
         for k in range(steps):
             idx = t.randint(0, n, (batch,), generator=gen).to(inputs.device)
             loss = self.loss_and_grads(inputs[idx], targets[idx])
+            cur_lr = lr * 0.5 * (1.0 + math.cos(math.pi * k / max(steps, 1)))
+            self.adam_step(cur_lr)
+            losses.append(loss)
+            if log_every and (k + 1) % log_every == 0:
+                print(f"surrogate step {k + 1}/{steps} loss {loss:.3e}", flush=True)
+        return losses
+
+    def fit_trajectories(self, snaps, steps: int, batch: int = 32, lr: float = 1e-3,
+                         pushforward: float = 0.5, seed: int = 0,
+                         log_every: int = 0) -> List[float]:
+        """Adam on trajectories ``snaps`` (device tensor (T, N, C, H, W),
+        consecutive states one network step apart).  A ``pushforward``
+        fraction of each minibatch starts from the network's own one-step
+        prediction of a true state and is trained towards the state two
+        steps ahead (the prediction is treated as data, no gradient flows
+        through it), which teaches the network to damp its own errors under
+        repeated application; the rest are plain one-step pairs.  Cosine-
+        decayed learning rate; returns the losses."""
+        t = self.torch
+        gen = t.Generator(device="cpu")
+        gen.manual_seed(seed)
+        n_t, n_traj = snaps.shape[0], snaps.shape[1]
+        if n_t < 3 and pushforward > 0.0:
+            raise ValueError("pushforward needs trajectories of at least 3 states")
+        n_push = int(round(batch * pushforward))
+        losses = []
+        for k in range(steps):
+            tr = t.randint(0, n_traj, (batch,), generator=gen)
+            t1 = t.randint(0, n_t - 1, (batch - n_push,), generator=gen)
+            t2 = t.randint(0, n_t - 2, (n_push,), generator=gen)
+            parts_in, parts_tg = [], []
+            if batch - n_push:
+                parts_in.append(snaps[t1, tr[n_push:]])
+                parts_tg.append(snaps[t1 + 1, tr[n_push:]])
+            if n_push:
+                start = snaps[t2, tr[:n_push]]
+                parts_in.append(self.predict(start).to(t.float32))
+                parts_tg.append(snaps[t2 + 2, tr[:n_push]])
+            loss = self.loss_and_grads(t.cat(parts_in), t.cat(parts_tg))
             cur_lr = lr * 0.5 * (1.0 + math.cos(math.pi * k / max(steps, 1)))
             self.adam_step(cur_lr)
             losses.append(loss)

@@ -124,10 +124,13 @@ class LearnedPropagator:
 
 
 def gaussian_mixture_states(cp, n: int, seed: int, components: int = 4,
-                            mean_range=(0.2, 0.8), var_range=(0.005, 0.02)) -> np.ndarray:
+                            mean_range=(0.2, 0.8), var_range=(0.005, 0.02),
+                            noise: float = 0.0) -> np.ndarray:
     """Seeded sums of ``components`` normalised isotropic Gaussian densities
-    (random means and variances) on the problem's vertices, with the static
-    Dirichlet values imposed; shape (n, *vertices_shape, y_dim)."""
+    (random means and variances) on the problem's vertices, plus white noise
+    of standard deviation ``noise`` times each state's maximum magnitude,
+    with the static Dirichlet values imposed; shape (n, *vertices_shape,
+    y_dim)."""
     from pararealml_amd.constraint import apply_constraints_along_last_axis
 
     rng = np.random.RandomState(seed)
@@ -145,6 +148,8 @@ def gaussian_mixture_states(cp, n: int, seed: int, components: int = 4,
                 r2 = sum((grids[a] - mean[a]) ** 2 for a in range(2))
                 acc += np.exp(-r2 / (2.0 * var)) / (2.0 * np.pi * var)
             out[k, ..., f] = acc
+        if noise > 0.0:
+            out[k] += noise * np.abs(out[k]).max() * rng.standard_normal(out[k].shape)
         apply_constraints_along_last_axis(cp.static_y_vertex_constraints, out[k])
     return out
 

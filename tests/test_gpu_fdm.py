@@ -257,6 +257,73 @@ def test_all_tile_configs_bitwise_identical():
         results.append(outs[0])
 
 
+def _mixed_bc_diffusion(n0, n1):
+    """Diffusion with Dirichlet rows (axis 0) and Neumann / Cauchy columns
+    (axis 1) on an n0 x n1 vertex grid."""
+    eq = pml.DiffusionEquation(2, 0.7)
+    mesh = pml.Mesh([(0.0, 0.1 * (n0 - 1)), (0.0, 0.125 * (n1 - 1))], [0.1, 0.125])
+    bcs = [
+        (
+            pml.DirichletBoundaryCondition(lambda x, t: np.full((len(x), 1), 0.5), is_static=True),
+            pml.DirichletBoundaryCondition(lambda x, t: x[:, 1:2] * 0.1, is_static=True),
+        ),
+        (
+            pml.NeumannBoundaryCondition(lambda x, t: np.full((len(x), 1), -0.3), is_static=True),
+            pml.CauchyBoundaryCondition(
+                lambda x, t: np.full((len(x), 1), 1.0), lambda x, t: x[:, :1] * 0.2, is_static=True
+            ),
+        ),
+    ]
+    cp = pml.ConstrainedProblem(eq, mesh, bcs)
+    centre = np.array([0.05 * (n0 - 1), 0.06 * (n1 - 1)])
+    ic = pml.GaussianInitialCondition(cp, [(centre, 0.3 * np.eye(2))], [2.0])
+    return cp, ic
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32", "float32c"])
+@pytest.mark.parametrize("shapes", [((21, 130), (34, 66)), ((21, 130), (17, 33))])
+def test_stream_variants_identical_with_boundaries_and_snapshots(prec, shapes):
+    """Every variant (one and two columns per lane, ring depths), eager and
+    graph launches, on a ragged batch with Dirichlet, Neumann and Cauchy
+    edges, strips that end inside the grid and snapshots: bitwise-identical
+    states and snapshots.  The second batch has an odd row length, where the
+    two-column variants hand over to the default one.  FP64 matches the
+    oracle per grid."""
+    print(f"case: stream variants {prec} {shapes}", flush=True)
+    from pararealml_amd import _device as dev
+    from pararealml_amd import _native as nat
+
+    cases = [_mixed_bc_diffusion(*s) for s in shapes]
+    problems = [dev.device_problem(cp) for cp, _ in cases]
+    y0s = [ic.discrete_y_0(True) for _, ic in cases]
+    lib = nat.device_lib()
+    n_cfg = lib.pml_fdm_configs(nat.PML_EQ["diffusion"], nat.PML_INT_FE, dev.precision_id(prec))
+    names = [lib.pml_fdm_config_name(nat.PML_EQ["diffusion"], nat.PML_INT_FE,
+                                     dev.precision_id(prec), c).decode() for c in range(n_cfg)]
+    assert any("-c2-" in n for n in names)
+    d_t, steps, every = 2e-4, 37, 5
+    outs = []
+    for cfg in range(n_cfg):
+        for graph in (False, True):
+            b = dev.FDMBatch(problems, [0, 1], prec, nat.PML_INT_FE, dev.solver_stream())
+            b.load_host(y0s)
+            snaps = b.new_snapshots(steps // every)
+            b.advance(d_t, steps, snapshots=snaps, snapshot_every=every, tile_config=cfg,
+                      use_graph=graph, constrain_input=True)
+            outs.append((names[cfg], graph, b.state_f64().cpu().numpy(),
+                         b.snapshots_f64(snaps, steps // every).cpu().numpy()))
+    for name, graph, st, sn in outs[1:]:
+        np.testing.assert_array_equal(st, outs[0][2], err_msg=f"{name} graph={graph}")
+        np.testing.assert_array_equal(sn, outs[0][3], err_msg=f"{name} graph={graph} snapshots")
+    if prec == "float64":
+        flat = outs[0][2]
+        for k, (cp, ic) in enumerate(cases):
+            _, y_ref = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, steps * d_t), d_t, "fe",
+                                     keep="last")
+            got = problems[k].from_device_layout(b.grid_view(flat, k), cp.mesh.vertices_shape)
+            assert rel_err(got, y_ref[0]) <= 1e-12
+
+
 def test_ragged_batch_matches_per_grid():
     """One launch over grids of different sizes equals per-grid solves."""
     print("case: ragged batch", flush=True)

@@ -117,8 +117,9 @@ def test_network_forward_and_gradients_match_oracle(c):
     u32 = u.astype(np.float32).astype(np.float64)
     s = np.abs(u32).reshape(2, -1).max(axis=1)
     x = np.moveaxis(u32 / s[:, None, None, None], 1, -1)
-    t = np.moveaxis(target.astype(np.float32).astype(np.float64) / s[:, None, None, None], 1, -1)
-    loss_ref, grads = so.loss_and_grads(params, x, t)
+    t32 = target.astype(np.float32).astype(np.float64)
+    inc = np.moveaxis((t32 - u32) / s[:, None, None, None], 1, -1)
+    loss_ref, grads = so.loss_and_grads(params, x, inc)
     assert abs(loss - loss_ref) <= 1e-3 * loss_ref
     for layer, (gw_ref, gb_ref) in enumerate(grads):
         gw = net.gw[layer].cpu().numpy().reshape(gw_ref.shape)
