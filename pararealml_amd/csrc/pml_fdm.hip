@@ -201,7 +201,7 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
                                : nullptr;
       ++snap_idx;
     }
-    if (a.snap_hi && V.kind == kStreamKind) fn = V.fn_snap;
+    if (a.snap_hi) fn = fn == V.fn_single ? V.fn_single_snap : V.fn_snap;
     hipLaunchKernelGGL(fn, grid_dim, block_dim, 0, st, a);
     PML_LAUNCH_CHECK();
     std::swap(cur_hi, nxt_hi);

@@ -27,6 +27,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
 #include <vector>
 
 #include "pml_common.h"
@@ -361,6 +362,23 @@ __device__ __forceinline__ double wave_shl1<double>(double x, double fill) {
   return __builtin_bit_cast(double, r);
 }
 
+// Value of lane `l` (wave-uniform index) of x, as a scalar.
+template <typename T>
+__device__ __forceinline__ T lane_value(T x, int l) {
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x), l));
+  } else {
+    static_assert(sizeof(T) == 8, "4- or 8-byte values");
+    struct Two {
+      int lo, hi;
+    };
+    Two v = __builtin_bit_cast(Two, x);
+    v.lo = __builtin_amdgcn_readlane(v.lo, l);
+    v.hi = __builtin_amdgcn_readlane(v.hi, l);
+    return __builtin_bit_cast(T, v);
+  }
+}
+
 // Raw buffer access: a resource over `bytes` bytes from p (gfx9 word 3:
 // 32-bit data format, no stride); loads outside the range return 0 and
 // stores outside it are dropped.
@@ -657,6 +675,297 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// fdm_stream2: NSTEP (1 or 2) single-stage steps per launch.  Lane l of a
+// wavefront owns column c = j0 - 1 + l; the first step is computed on all
+// 64 columns j0-1 .. j0+62 and the second on lanes 1 .. 62 (the output
+// strip j0 .. j0+61; strips are 62 columns apart), so the second step's
+// neighbours are all inside the wavefront.  Walking down the rows, the
+// first step of row r (input rows r-1 .. r+1, from a ring of prefetched
+// rows) and the second step of row r-1 (first-step rows r-2 .. r, from a
+// second ring) are computed together, so the intermediate state never
+// leaves the registers.  All boundary rules are selects: the axis-1 edge
+// values of a row are scalar loads, the axis-0 edge values of the lane's
+// column are loaded into registers once; rows and columns outside the grid
+// only ever feed selects that replace them.
+
+// Stencil quantities of one field with every edge rule as a select
+// (identical arithmetic to field_quant): e_* are the row's axis-1 Neumann
+// values, r_* the column's axis-0 Neumann values (NaN = none).
+template <typename C, bool WL, bool WD>
+__device__ __forceinline__ void field_quant_sel(Quant<C> &q, C yc, C yw, C ye, C yn, C ys,
+                                                bool lo1, bool hi1, bool lo0, bool hi0,
+                                                bool ax0, C e_lo, C e_hi, C r_lo, C r_hi,
+                                                const StencilK<C> &K) {
+  q.v = yc;
+  q.lap = 0;
+  q.d0 = 0;
+  q.d1 = 0;
+  if (!WL && !WD) return;
+  C lap1 = 0, d1 = 0, lap0 = 0, d0 = 0;
+  if (WL) {
+    const C gw = lo1 ? (e_lo == e_lo ? ye + K.m2dx1 * e_lo : C(0)) : yw;
+    const C ge = hi1 ? (e_hi == e_hi ? yw + K.p2dx1 * e_hi : C(0)) : ye;
+    lap1 = ((ge - C(2) * yc) + gw) * K.inv_dx1sq;
+    if (ax0) {
+      const C gn = lo0 ? (r_lo == r_lo ? ys + K.m2dx0 * r_lo : C(0)) : yn;
+      const C gs = hi0 ? (r_hi == r_hi ? yn + K.p2dx0 * r_hi : C(0)) : ys;
+      lap0 = ((gs - C(2) * yc) + gn) * K.inv_dx0sq;
+    }
+  }
+  if (WD) {
+    const C zw = lo1 ? C(0) : yw, ze = hi1 ? C(0) : ye;
+    d1 = (ze - zw) * K.inv_2dx1;
+    d1 = (lo1 && e_lo == e_lo) ? e_lo : d1;
+    d1 = (hi1 && e_hi == e_hi) ? e_hi : d1;
+    if (ax0) {
+      const C zn = lo0 ? C(0) : yn, zs = hi0 ? C(0) : ys;
+      d0 = (zs - zn) * K.inv_2dx0;
+      d0 = (lo0 && r_lo == r_lo) ? r_lo : d0;
+      d0 = (hi0 && r_hi == r_hi) ? r_hi : d0;
+    }
+  }
+  if (ax0) {
+    q.lap = lap0 + lap1;
+    q.d0 = d0;
+    q.d1 = d1;
+  } else {
+    q.lap = lap1;
+    q.d0 = d1;  // the single physical axis lies along axis 1
+    q.d1 = 0;
+  }
+}
+
+template <class Eq, class Integ, class P, int RW, int PD, int NSTEP, bool SNAP>
+__global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
+  using C = typename P::C;
+  // Dirichlet values: exact FP64 where the state can hold them.
+  using D = typename std::conditional<P::kComp || sizeof(C) == 8, double, C>::type;
+  constexpr int F = Eq::F;
+  constexpr int NW = PD + 3;  // slots of each ring
+  constexpr int OW = 62;      // output columns per strip
+  static_assert(Eq::kInvariant || Integ::kStages == 1, "single-stage steps only");
+  static_assert(NSTEP == 1 || NSTEP == 2, "one or two steps per launch");
+  static_assert(PD >= 1, "the ring must prefetch at least one row");
+  static_assert(RW + 2 <= 64, "one lane per first-step row of the unit");
+  const int lane = threadIdx.x & 63;
+  const int unit = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * 4 +
+                                                  static_cast<int>(threadIdx.x >> 6));
+  if (unit >= a.n_tiles) return;  // whole wavefront: no barriers follow
+  const TileDev U = a.tiles[unit];
+  const GridDev G = a.grids[U.g];
+  const int n0 = G.n0, n1 = G.n1;
+  const bool ax0 = a.ndim == 2;
+  const unsigned plane = static_cast<unsigned>(n0) * n1;
+  const double *bcg = a.bc + G.bc;
+  const int fstride = 4 * (n0 + n1);
+  const StencilK<C> K(G);
+  C pc[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) pc[i] = static_cast<C>(a.p[i]);
+  const C dt = static_cast<C>(a.dt);
+  const int i_beg = U.i0;
+  const int i_end = min(U.i0 + RW, n0);
+  const int col = U.j0 - 1 + lane;
+  const bool lo1 = col == 0, hi1 = col == n1 - 1;
+  const int cc = min(max(col, 0), n1 - 1);
+  // Wave-uniform unit kinds: does any computed column / row lie on (or
+  // beyond) a grid edge?
+  const bool edge_cols = U.j0 - 1 <= 0 || U.j0 + 62 >= n1 - 1;
+  const bool edge_rows = !ax0 || i_beg - 1 <= 0 || i_end >= n0 - 1;
+
+  const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
+  const C *in_lo = P::kComp ? static_cast<const C *>(a.in_lo) + G.off : nullptr;
+  C *out_hi = static_cast<C *>(a.out_hi) + G.off;
+  C *out_lo = P::kComp ? static_cast<C *>(a.out_lo) + G.off : nullptr;
+  C *sn_hi = SNAP ? static_cast<C *>(a.snap_hi) + G.off : nullptr;
+  C *sn_lo = (SNAP && P::kComp) ? static_cast<C *>(a.snap_lo) + G.off : nullptr;
+  const unsigned esz = static_cast<unsigned>(sizeof(C));
+  const unsigned jb = static_cast<unsigned>(col) * esz;  // col = -1 reads 0
+  const int hcol = lane == 0 ? U.j0 - 2 : (lane == 63 ? U.j0 + 63 : col);
+  const unsigned hb = static_cast<unsigned>(hcol) * esz;
+  // Only lanes 1 .. 62 store (lanes 0 and 63 carry the neighbour columns).
+  const unsigned sb = (lane >= 1 && lane <= OW) ? jb : 0x80000000u;
+  const unsigned row_bytes = static_cast<unsigned>(n1) * esz;
+
+  auto load = [&](int r, C *v, C *vl, C *h) {
+    const bool ok = r >= 0 && r < n0 && r <= i_end + 1;
+    const size_t ro = static_cast<size_t>(min(max(r, 0), n0 - 1)) * n1;
+    const unsigned nb = ok ? row_bytes : 0u;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const auto rs = row_rsrc(in_hi + static_cast<size_t>(f) * plane + ro, nb);
+      v[f] = buf_load<C, 1>(rs, jb).v[0];
+      if (P::kComp)
+        vl[f] = buf_load<C, 1>(row_rsrc(in_lo + static_cast<size_t>(f) * plane + ro, nb), jb).v[0];
+      h[f] = buf_load<C, 1>(rs, hb).v[0];
+    }
+  };
+  auto store_row = [&](int r, const C *vh, const C *vl) {
+    const size_t ro = static_cast<size_t>(r) * n1;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const size_t fo = static_cast<size_t>(f) * plane + ro;
+      buf_store<C, 1>(row_rsrc(out_hi + fo, row_bytes), sb, Vec<C, 1>{{vh[f]}});
+      if (P::kComp) buf_store<C, 1>(row_rsrc(out_lo + fo, row_bytes), sb, Vec<C, 1>{{vl[f]}});
+      if (SNAP) {
+        buf_store<C, 1>(row_rsrc(sn_hi + fo, row_bytes), sb, Vec<C, 1>{{vh[f]}});
+        if (P::kComp) buf_store<C, 1>(row_rsrc(sn_lo + fo, row_bytes), sb, Vec<C, 1>{{vl[f]}});
+      }
+    }
+  };
+
+  // The row walk for one unit kind: kCols / kRows select the edge rules
+  // compiled in (interior units run plain stencils).
+  auto run = [&](auto cols_tag, auto rows_tag) {
+    constexpr bool kCols = decltype(cols_tag)::value;
+    constexpr bool kRows = decltype(rows_tag)::value;
+    // Axis-0 edge values of the lane's column (rows 0 and n0-1).
+    C rn_lo[F], rn_hi[F];
+    D rd_lo[F], rd_hi[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      rn_lo[f] = rn_hi[f] = static_cast<C>(NAN);
+      rd_lo[f] = rd_hi[f] = static_cast<D>(NAN);
+      if (kRows && ax0) {
+        const double *fb = bcg + f * fstride;
+        const double *nb = fb + 2 * (n0 + n1);
+        rd_lo[f] = static_cast<D>(fb[cc]);
+        rd_hi[f] = static_cast<D>(fb[n1 + cc]);
+        rn_lo[f] = static_cast<C>(nb[cc]);
+        rn_hi[f] = static_cast<C>(nb[n1 + cc]);
+      }
+    }
+    // Axis-1 edge values of the unit's first-step rows, lane k holding row
+    // i_beg - 1 + k (loaded once; each step reads its row's values from
+    // that lane, so the row walk issues no scalar memory traffic).
+    C pe_lo[F], pe_hi[F];
+    D pd_lo[F], pd_hi[F];
+    if (kCols) {
+      const int rk = min(max(i_beg - 1 + lane, 0), n0 - 1);
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        const double *fb = bcg + f * fstride;
+        const double *nb = fb + 2 * (n0 + n1);
+        pd_lo[f] = static_cast<D>(fb[2 * n1 + rk]);
+        pd_hi[f] = static_cast<D>(fb[2 * n1 + n0 + rk]);
+        pe_lo[f] = static_cast<C>(nb[2 * n1 + rk]);
+        pe_hi[f] = static_cast<C>(nb[2 * n1 + n0 + rk]);
+      }
+    }
+    // One step of row r = i_beg - 1 + k from rows (yu, yc, yd) of the
+    // previous state (hi parts; yc_lo the lo parts of row r), west / east
+    // from the lanes.
+    auto step_row = [&](int k, const C *yu, const C *yc, const C *yc_lo, const C *yd,
+                        const C *fill, C *nh, C *nl) {
+      const int r = i_beg - 1 + k;
+      const bool lo0 = kRows && ax0 && r == 0, hi0 = kRows && ax0 && r == n0 - 1;
+      C e_lo[F], e_hi[F];
+      D d_lo[F], d_hi[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        e_lo[f] = e_hi[f] = static_cast<C>(NAN);
+        d_lo[f] = d_hi[f] = static_cast<D>(NAN);
+        if (kCols) {
+          d_lo[f] = lane_value(pd_lo[f], k);
+          d_hi[f] = lane_value(pd_hi[f], k);
+          e_lo[f] = lane_value(pe_lo[f], k);
+          e_hi[f] = lane_value(pe_hi[f], k);
+        }
+      }
+      const bool l1 = kCols && lo1, h1 = kCols && hi1;
+      Quant<C> q[F];
+#define PML_FQS(F_)                                                                          \
+  if constexpr (F_ < F) {                                                                    \
+    const C yw = wave_shr1<C>(yc[F_], fill[F_]), ye = wave_shl1<C>(yc[F_], fill[F_]);        \
+    field_quant_sel<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD>(q[F_], yc[F_], yw, ye, yu[F_],    \
+                                                           yd[F_], l1, h1, lo0, hi0, ax0,    \
+                                                           e_lo[F_], e_hi[F_], rn_lo[F_],    \
+                                                           rn_hi[F_], K);                    \
+  }
+      PML_FQS(0)
+      PML_FQS(1)
+      PML_FQS(2)
+#undef PML_FQS
+      C rr[F], ra[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) rr[f] = ra[f] = 0;
+      Eq::template rhs<C>(pc, q, rr);
+      if (Eq::kAlg != 0u) Eq::template alg<C>(pc, q, ra);
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        C hv = yc[f], lv = P::kComp ? yc_lo[f] : C(0);
+        if ((Eq::kAlg >> f) & 1u) {
+          hv = ra[f];
+          lv = 0;
+        } else {
+          const C kk = dt * rr[f];
+          C d = kk;
+          if (Eq::kInvariant && Integ::kStages == 4)
+            d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
+          state_add<P>(hv, lv, d);
+        }
+        if (kCols || kRows) {
+          // Dirichlet: the axis-0 edge value first, then the axis-1 one
+          // (selects, no divergent branch).
+          D dv = lo0 ? rd_lo[f] : (hi0 ? rd_hi[f] : static_cast<D>(NAN));
+          dv = (dv == dv) ? dv : (l1 ? d_lo[f] : (h1 ? d_hi[f] : static_cast<D>(NAN)));
+          C sh, sl;
+          state_set<P>(sh, sl, static_cast<double>(dv));
+          const bool fix = dv == dv;
+          hv = fix ? sh : hv;
+          lv = fix ? sl : lv;
+        }
+        nh[f] = hv;
+        nl[f] = lv;
+      }
+    };
+
+    // Input ring: row i_beg - 2 + m in slot m mod NW; first-step ring: row
+    // i_beg - 1 + t in slot t mod NW.
+    C w[NW][F], wl[NW][F], wh[NW][F];
+    C ah[NW][F], al[NW][F];
+#pragma unroll
+    for (int m = 0; m <= PD + 1; ++m) load(i_beg - 2 + m, w[m % NW], wl[m % NW], wh[m % NW]);
+    // Iteration t: first step of row r = i_beg - 1 + t, second step of row
+    // r - 1 (NSTEP = 2).  Fully unrolled: every slot index is a constant.
+#pragma unroll
+    for (int t = 0; t <= RW + 1; ++t) {
+      const int r = i_beg - 1 + t;
+      if (NSTEP == 2 ? r - 1 >= i_end : r >= i_end) break;
+      {
+        const int m = t + 2 + PD;  // prefetch into the slot of input row m - NW
+        load(i_beg - 2 + m, w[m % NW], wl[m % NW], wh[m % NW]);
+      }
+      const int su = t % NW, sc = (t + 1) % NW, sd = (t + 2) % NW;
+      if (NSTEP == 1) {
+        if (t == 0) continue;  // row i_beg - 1 is not written
+        C nh[F], nl[F];
+        step_row(t, w[su], w[sc], wl[sc], w[sd], wh[sc], nh, nl);
+        store_row(r, nh, nl);
+      } else {
+        step_row(t, w[su], w[sc], wl[sc], w[sd], wh[sc], ah[t % NW], al[t % NW]);
+        if (t >= 2) {
+          C nh[F], nl[F];
+          const int au = (t - 2) % NW, ac = (t - 1) % NW, ad = t % NW;
+          // The fill (lanes 0 / 63) is never stored.
+          step_row(t - 1, ah[au], ah[ac], al[ac], ah[ad], ah[ac], nh, nl);
+          store_row(r - 1, nh, nl);
+        }
+      }
+    }
+  };
+  using T = std::true_type;
+  using Fa = std::false_type;
+  if (edge_cols) {
+    if (edge_rows) run(T{}, T{});
+    else run(T{}, Fa{});
+  } else {
+    if (edge_rows) run(Fa{}, T{});
+    else run(Fa{}, Fa{});
+  }
+}
+
 // Imposes the Dirichlet values on a state in place: one workgroup per
 // (grid, field) walks the grid's edge vertices.
 template <class P>
@@ -895,27 +1204,36 @@ using KernelFn = void (*)(FdmArgs);
 enum VariantKind { kTileKind = 0, kStreamKind = 1 };
 
 struct Variant {
-  KernelFn fn;
-  KernelFn fn_single;  // one step per launch, same unit shape
-  KernelFn fn_snap;    // fn for launches that write a snapshot
-  int kind;            // kTileKind or kStreamKind
-  int ox, oy;          // unit shape: output tile (tile kind) or 64 CPL x RW strip
+  KernelFn fn;              // `steps` steps per launch
+  KernelFn fn_single;       // one step per launch, same unit shape
+  KernelFn fn_snap;         // fn for launches that write a snapshot
+  KernelFn fn_single_snap;  // fn_single for launches that write a snapshot
+  int kind;                 // kTileKind or kStreamKind
+  int ox, oy;               // unit shape: output tile / strip width x rows
   int steps, nt;
-  int cpl;             // columns per lane (stream kind)
+  int cpl;                  // columns per lane (stream kind)
   const char *name;
 };
 
-#define PML_TILE(EQ, IN, PR, OX, OY, ST, NT, NAME)                                     \
-  Variant {                                                                             \
-    &fdm_fused<EQ, IN, PR, OX, OY, ST, NT>, &fdm_fused<EQ, IN, PR, OX, OY, 1, NT>,      \
-        &fdm_fused<EQ, IN, PR, OX, OY, ST, NT>, kTileKind, OX, OY, ST, NT, 1, NAME      \
+#define PML_TILE(EQ, IN, PR, OX, OY, ST, NT, NAME)                                       \
+  Variant {                                                                               \
+    &fdm_fused<EQ, IN, PR, OX, OY, ST, NT>, &fdm_fused<EQ, IN, PR, OX, OY, 1, NT>,        \
+        &fdm_fused<EQ, IN, PR, OX, OY, ST, NT>, &fdm_fused<EQ, IN, PR, OX, OY, 1, NT>,    \
+        kTileKind, OX, OY, ST, NT, 1, NAME                                                \
   }
-#define PML_STREAM(EQ, IN, PR, RW, PD, FULL, CPL, NAME)                                 \
-  Variant {                                                                             \
-    &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, false>,                                  \
-        &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, false>,                              \
-        &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, true>, kStreamKind, 64 * CPL, RW, 1, \
-        256, CPL, NAME                                                                  \
+#define PML_STREAM(EQ, IN, PR, RW, PD, FULL, CPL, NAME)                                   \
+  Variant {                                                                               \
+    &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, false>,                                    \
+        &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, false>,                                \
+        &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, true>,                                 \
+        &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, true>, kStreamKind, 64 * CPL, RW, 1,   \
+        256, CPL, NAME                                                                    \
+  }
+#define PML_STREAM2(EQ, IN, PR, RW, PD, NAME)                                             \
+  Variant {                                                                               \
+    &fdm_stream2<EQ, IN, PR, RW, PD, 2, false>, &fdm_stream2<EQ, IN, PR, RW, PD, 1, false>, \
+        &fdm_stream2<EQ, IN, PR, RW, PD, 2, true>,                                        \
+        &fdm_stream2<EQ, IN, PR, RW, PD, 1, true>, kStreamKind, 62, RW, 2, 256, 1, NAME   \
   }
 
 // Variants per (equation, integrator, precision); the first is the default.
@@ -927,17 +1245,13 @@ const std::vector<Variant> &variants_for() {
     constexpr bool wide = sizeof(typename PR::C) == 8;
     if constexpr (S == 1) {
       out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, 1, "stream-rw8-pd1-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, false, 1, "stream-rw8-pd1"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 4, 1, true, 1, "stream-rw4-pd1-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 16, 1, true, 1, "stream-rw16-pd1-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 10, 2, true, 1, "stream-rw10-pd2-full"));
       out.push_back(PML_STREAM(EQ, IN, PR, 6, 3, true, 1, "stream-rw6-pd3-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 12, 3, true, 1, "stream-rw12-pd3-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, 2, "stream-c2-rw8-pd1-full"));
       out.push_back(PML_STREAM(EQ, IN, PR, 4, 1, true, 2, "stream-c2-rw4-pd1-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 10, 2, true, 2, "stream-c2-rw10-pd2-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 6, 3, true, 2, "stream-c2-rw6-pd3-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 12, 3, true, 2, "stream-c2-rw12-pd3-full"));
+      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, 2, "stream-c2-rw8-pd1-full"));
+      out.push_back(PML_STREAM2(EQ, IN, PR, 8, 1, "stream2-rw8-pd1-s2"));
+      out.push_back(PML_STREAM2(EQ, IN, PR, 8, 2, "stream2-rw8-pd2-s2"));
+      out.push_back(PML_STREAM2(EQ, IN, PR, 16, 1, "stream2-rw16-pd1-s2"));
+      out.push_back(PML_STREAM2(EQ, IN, PR, 4, 1, "stream2-rw4-pd1-s2"));
       if constexpr (wide) {
         out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
       } else {

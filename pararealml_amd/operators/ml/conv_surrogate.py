@@ -294,36 +294,41 @@ class ConvSurrogate:
         return losses
 
     def fit_trajectories(self, snaps, steps: int, batch: int = 32, lr: float = 1e-3,
-                         pushforward: float = 0.5, seed: int = 0,
+                         pushforward: float = 0.5, max_push: int = 1, seed: int = 0,
                          log_every: int = 0) -> List[float]:
         """Adam on trajectories ``snaps`` (device tensor (T, N, C, H, W),
         consecutive states one network step apart).  A ``pushforward``
-        fraction of each minibatch starts from the network's own one-step
-        prediction of a true state and is trained towards the state two
-        steps ahead (the prediction is treated as data, no gradient flows
-        through it), which teaches the network to damp its own errors under
-        repeated application; the rest are plain one-step pairs.  Cosine-
-        decayed learning rate; returns the losses."""
+        fraction of each minibatch starts from the network's own m-step
+        prediction of a true state (m drawn from 1..max_push per minibatch)
+        and is trained towards the true state one step further (the
+        predictions are treated as data, no gradient flows through them),
+        which teaches the network to damp its own errors under repeated
+        application; the rest are plain one-step pairs.  Cosine-decayed
+        learning rate; returns the losses."""
         t = self.torch
         gen = t.Generator(device="cpu")
         gen.manual_seed(seed)
         n_t, n_traj = snaps.shape[0], snaps.shape[1]
+        max_push = max(1, min(int(max_push), n_t - 2))
         if n_t < 3 and pushforward > 0.0:
             raise ValueError("pushforward needs trajectories of at least 3 states")
         n_push = int(round(batch * pushforward))
         losses = []
         for k in range(steps):
             tr = t.randint(0, n_traj, (batch,), generator=gen)
+            m = int(t.randint(1, max_push + 1, (1,), generator=gen))
             t1 = t.randint(0, n_t - 1, (batch - n_push,), generator=gen)
-            t2 = t.randint(0, n_t - 2, (n_push,), generator=gen)
+            t2 = t.randint(0, n_t - 1 - m, (n_push,), generator=gen)
             parts_in, parts_tg = [], []
             if batch - n_push:
                 parts_in.append(snaps[t1, tr[n_push:]])
                 parts_tg.append(snaps[t1 + 1, tr[n_push:]])
             if n_push:
-                start = snaps[t2, tr[:n_push]]
-                parts_in.append(self.predict(start).to(t.float32))
-                parts_tg.append(snaps[t2 + 2, tr[:n_push]])
+                cur = snaps[t2, tr[:n_push]]
+                for _ in range(m):
+                    cur = self.predict(cur).to(t.float32)
+                parts_in.append(cur)
+                parts_tg.append(snaps[t2 + m + 1, tr[:n_push]])
             loss = self.loss_and_grads(t.cat(parts_in), t.cat(parts_tg))
             cur_lr = lr * 0.5 * (1.0 + math.cos(math.pi * k / max(steps, 1)))
             self.adam_step(cur_lr)

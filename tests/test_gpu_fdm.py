@@ -283,10 +283,11 @@ def _mixed_bc_diffusion(n0, n1):
 @pytest.mark.parametrize("prec", ["float64", "float32", "float32c"])
 @pytest.mark.parametrize("shapes", [((21, 130), (34, 66)), ((21, 130), (17, 33))])
 def test_stream_variants_identical_with_boundaries_and_snapshots(prec, shapes):
-    """Every variant (one and two columns per lane, ring depths), eager and
-    graph launches, on a ragged batch with Dirichlet, Neumann and Cauchy
-    edges, strips that end inside the grid and snapshots: bitwise-identical
-    states and snapshots.  The second batch has an odd row length, where the
+    """Every variant (one and two columns per lane, ring depths, two steps
+    per launch), eager and graph launches, on a ragged batch with Dirichlet,
+    Neumann and Cauchy edges, strips that end inside the grid and snapshots
+    (an odd step count, so fused variants also run a single-step launch):
+    bitwise-identical states and snapshots.  The second batch has an odd row length, where the
     two-column variants hand over to the default one.  FP64 matches the
     oracle per grid."""
     print(f"case: stream variants {prec} {shapes}", flush=True)
@@ -301,7 +302,7 @@ def test_stream_variants_identical_with_boundaries_and_snapshots(prec, shapes):
     names = [lib.pml_fdm_config_name(nat.PML_EQ["diffusion"], nat.PML_INT_FE,
                                      dev.precision_id(prec), c).decode() for c in range(n_cfg)]
     assert any("-c2-" in n for n in names)
-    d_t, steps, every = 2e-4, 37, 5
+    d_t, steps, every = 2e-4, 37, 4
     outs = []
     for cfg in range(n_cfg):
         for graph in (False, True):
