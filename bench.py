@@ -208,11 +208,14 @@ def parareal_workload(args):
     def run_parareal(g, name):
         history = []
         prob_box = {}
+        t_start = time.perf_counter()
 
         def monitor(i, u):
             e = err_of(u[1:])
             history.append((i + 1, e))
-            return e <= target
+            # A run still above the target after the serial fine solver's
+            # time has lost; stop it there.
+            return e <= target or time.perf_counter() - t_start > t_serial
 
         f = FDMOperator(RK4(), diff, dt_f, precision=args.pr_fine_precision)
         p = PararealOperator(f, g, 0.0, n_slices=S, output="endpoints", monitor=monitor)
