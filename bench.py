@@ -190,6 +190,17 @@ def parareal_workload(args):
         torch.cuda.synchronize()
         return time.perf_counter() - t0, sol.discrete_y()
 
+    # Kernel variants tuned (untimed) for each batch shape the runs use:
+    # single fine grids (serial), the batch of all slices (Parareal fine),
+    # single coarse grids (the coarse sweep).
+    from pararealml_amd.operators.ml.learned_coarse import coarse_problem
+
+    log("parareal: tuning kernel variants")
+    for prec in ("float64", args.pr_fine_precision):
+        FDMOperator(RK4(), diff, dt_f, precision=prec).tune(cp, 1)
+    FDMOperator(RK4(), diff, dt_f, precision=args.pr_fine_precision).tune(cp, S)
+    FDMOperator(ForwardEulerMethod(), diff, dt_c).tune(coarse_problem(cp), 1)
+
     log("parareal: FP64 serial fine reference")
     t_ref, ref = timed_serial("float64")
     ref_flat = ref.reshape(S, -1)

@@ -966,6 +966,242 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// fdm_stream_rk: one multi-stage step (explicit midpoint, RK4) per launch
+// for systems whose right-hand side is re-evaluated at every stage.  A row
+// pipeline: while walking down, stage s works on the row s-1 above stage 1,
+// so every stage input Y_s stays in a three-row register ring and each row
+// of the state crosses HBM once per step.  Stage s is valid on one column
+// fewer per side than stage s-1: lane l owns column j0 - (S-1) + l and the
+// step's result is written by lanes S-1 .. 64-S (strips of 66 - 2S
+// columns).  The per-stage arithmetic, stage weights and Dirichlet rules are
+// those of fdm_fused (bitwise-identical results); boundary rules are
+// selects as in fdm_stream2, units are specialised on touching an edge.
+template <class Eq, class Integ, class P, int RW, int PD, bool SNAP>
+__global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
+  using C = typename P::C;
+  using D = typename std::conditional<P::kComp || sizeof(C) == 8, double, C>::type;
+  constexpr int F = Eq::F;
+  constexpr int S = Integ::kStages;
+  static_assert(!Eq::kInvariant && (S == 2 || S == 4), "multi-stage steps");
+  static_assert(Eq::kAlg == 0u, "no algebraic fields");
+  constexpr int NW = PD + S + 1;        // input ring slots
+  constexpr int NK = S;                 // k-accumulator ring slots (RK4)
+  constexpr int OW = 66 - 2 * S;        // output columns per strip
+  constexpr int NT = RW + 2 * (S - 1);  // pipeline iterations
+  static_assert(PD >= 1, "the ring must prefetch at least one row");
+  static_assert(NT <= 64, "one lane per stage-1 row of the unit");
+  const int lane = threadIdx.x & 63;
+  const int unit = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * 4 +
+                                                  static_cast<int>(threadIdx.x >> 6));
+  if (unit >= a.n_tiles) return;  // whole wavefront: no barriers follow
+  const TileDev U = a.tiles[unit];
+  const GridDev G = a.grids[U.g];
+  const int n0 = G.n0, n1 = G.n1;
+  const bool ax0 = a.ndim == 2;
+  const unsigned plane = static_cast<unsigned>(n0) * n1;
+  const double *bcg = a.bc + G.bc;
+  const int fstride = 4 * (n0 + n1);
+  const StencilK<C> K(G);
+  C pc[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) pc[i] = static_cast<C>(a.p[i]);
+  const C dt = static_cast<C>(a.dt);
+  const C hdt = static_cast<C>(a.dt * 0.5);
+  const int i_beg = U.i0;
+  const int i_end = min(U.i0 + RW, n0);
+  const int c0 = U.j0 - (S - 1);  // column of lane 0
+  const int col = c0 + lane;
+  const bool lo1 = col == 0, hi1 = col == n1 - 1;
+  const int cc = min(max(col, 0), n1 - 1);
+  const int r_first = i_beg - (S - 1);  // first stage-1 row
+  const bool edge = !ax0 || c0 <= 0 || c0 + 63 >= n1 - 1 || r_first <= 0 ||
+                    i_end - 1 + (S - 1) >= n0 - 1;  // wave-uniform
+
+  const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
+  const C *in_lo = P::kComp ? static_cast<const C *>(a.in_lo) + G.off : nullptr;
+  C *out_hi = static_cast<C *>(a.out_hi) + G.off;
+  C *out_lo = P::kComp ? static_cast<C *>(a.out_lo) + G.off : nullptr;
+  C *sn_hi = SNAP ? static_cast<C *>(a.snap_hi) + G.off : nullptr;
+  C *sn_lo = (SNAP && P::kComp) ? static_cast<C *>(a.snap_lo) + G.off : nullptr;
+  const unsigned esz = static_cast<unsigned>(sizeof(C));
+  const unsigned jb = static_cast<unsigned>(col) * esz;  // columns < 0 read 0
+  const int hcol = lane == 0 ? c0 - 1 : (lane == 63 ? c0 + 64 : col);
+  const unsigned hb = static_cast<unsigned>(hcol) * esz;
+  const unsigned sb = (lane >= S - 1 && lane <= 64 - S) ? jb : 0x80000000u;
+  const unsigned row_bytes = static_cast<unsigned>(n1) * esz;
+
+  auto load = [&](int r, C *v, C *vl, C *h) {
+    const bool ok = r >= 0 && r < n0 && r <= i_end - 1 + S;
+    const size_t ro = static_cast<size_t>(min(max(r, 0), n0 - 1)) * n1;
+    const unsigned nb = ok ? row_bytes : 0u;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const auto rs = row_rsrc(in_hi + static_cast<size_t>(f) * plane + ro, nb);
+      v[f] = buf_load<C, 1>(rs, jb).v[0];
+      if (P::kComp)
+        vl[f] = buf_load<C, 1>(row_rsrc(in_lo + static_cast<size_t>(f) * plane + ro, nb), jb).v[0];
+      h[f] = buf_load<C, 1>(rs, hb).v[0];
+    }
+  };
+  auto store_row = [&](int r, const C *vh, const C *vl) {
+    const size_t ro = static_cast<size_t>(r) * n1;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const size_t fo = static_cast<size_t>(f) * plane + ro;
+      buf_store<C, 1>(row_rsrc(out_hi + fo, row_bytes), sb, Vec<C, 1>{{vh[f]}});
+      if (P::kComp) buf_store<C, 1>(row_rsrc(out_lo + fo, row_bytes), sb, Vec<C, 1>{{vl[f]}});
+      if (SNAP) {
+        buf_store<C, 1>(row_rsrc(sn_hi + fo, row_bytes), sb, Vec<C, 1>{{vh[f]}});
+        if (P::kComp) buf_store<C, 1>(row_rsrc(sn_lo + fo, row_bytes), sb, Vec<C, 1>{{vl[f]}});
+      }
+    }
+  };
+
+  auto run = [&](auto edge_tag) {
+    constexpr bool kE = decltype(edge_tag)::value;
+    // Edge values: the lane's column on rows 0 / n0-1, and the axis-1 edge
+    // values of stage-1 row r_first + k in lane k.
+    C rn_lo[F], rn_hi[F], pe_lo[F], pe_hi[F];
+    D rd_lo[F], rd_hi[F], pd_lo[F], pd_hi[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      rn_lo[f] = rn_hi[f] = pe_lo[f] = pe_hi[f] = static_cast<C>(NAN);
+      rd_lo[f] = rd_hi[f] = pd_lo[f] = pd_hi[f] = static_cast<D>(NAN);
+      if (kE) {
+        const double *fb = bcg + f * fstride;
+        const double *nb = fb + 2 * (n0 + n1);
+        if (ax0) {
+          rd_lo[f] = static_cast<D>(fb[cc]);
+          rd_hi[f] = static_cast<D>(fb[n1 + cc]);
+          rn_lo[f] = static_cast<C>(nb[cc]);
+          rn_hi[f] = static_cast<C>(nb[n1 + cc]);
+        }
+        const int rk = min(max(r_first + lane, 0), n0 - 1);
+        pd_lo[f] = static_cast<D>(fb[2 * n1 + rk]);
+        pd_hi[f] = static_cast<D>(fb[2 * n1 + n0 + rk]);
+        pe_lo[f] = static_cast<C>(nb[2 * n1 + rk]);
+        pe_hi[f] = static_cast<C>(nb[2 * n1 + n0 + rk]);
+      }
+    }
+    // dt-weighted right-hand side of row r = r_first + k from stage-input
+    // rows (yu, yc, yd); also returns the row's Dirichlet values.
+    auto stage_k = [&](int k, const C *yu, const C *yc, const C *yd, const C *fill, C w, C *kk,
+                       D *dv) {
+      const int r = r_first + k;
+      const bool lo0 = kE && ax0 && r == 0, hi0 = kE && ax0 && r == n0 - 1;
+      const bool l1 = kE && lo1, h1 = kE && hi1;
+      C e_lo[F], e_hi[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        e_lo[f] = e_hi[f] = static_cast<C>(NAN);
+        dv[f] = static_cast<D>(NAN);
+        if (kE) {
+          e_lo[f] = lane_value(pe_lo[f], k);
+          e_hi[f] = lane_value(pe_hi[f], k);
+          D v = lo0 ? rd_lo[f] : (hi0 ? rd_hi[f] : static_cast<D>(NAN));
+          v = (v == v) ? v
+                       : (l1 ? lane_value(pd_lo[f], k)
+                             : (h1 ? lane_value(pd_hi[f], k) : static_cast<D>(NAN)));
+          dv[f] = v;
+        }
+      }
+      Quant<C> q[F];
+#define PML_FQK(F_)                                                                          \
+  if constexpr (F_ < F) {                                                                    \
+    const C yw = wave_shr1<C>(yc[F_], fill[F_]), ye = wave_shl1<C>(yc[F_], fill[F_]);        \
+    field_quant_sel<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD>(q[F_], yc[F_], yw, ye, yu[F_],    \
+                                                           yd[F_], l1, h1, lo0, hi0, ax0,    \
+                                                           e_lo[F_], e_hi[F_], rn_lo[F_],    \
+                                                           rn_hi[F_], K);                    \
+  }
+      PML_FQK(0)
+      PML_FQK(1)
+      PML_FQK(2)
+#undef PML_FQK
+      C rr[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) rr[f] = 0;
+      Eq::template rhs<C>(pc, q, rr);
+#pragma unroll
+      for (int f = 0; f < F; ++f) kk[f] = w * rr[f];
+    };
+
+    // Input ring: row i_beg - S + m in slot m mod NW.  Stage rings: Y_s
+    // (s >= 2) of the row produced at iteration t in slot t mod 3.  k ring:
+    // the accumulator of the row whose stage 1 ran at iteration t, slot
+    // t mod NK.
+    C w[NW][F], wl[NW][F], wh[NW][F];
+    C y[S > 1 ? S - 1 : 1][3][F];
+    C kacc[NK][F];
+#pragma unroll
+    for (int m = 0; m <= PD + 1; ++m) load(i_beg - S + m, w[m % NW], wl[m % NW], wh[m % NW]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (i_beg - 2 * (S - 1) + t >= i_end) break;  // stage S is past the strip
+      {
+        const int m = t + 2 + PD;
+        load(i_beg - S + m, w[m % NW], wl[m % NW], wh[m % NW]);
+      }
+#pragma unroll
+      for (int s = 1; s <= S; ++s) {
+        if (t < 2 * (s - 1)) continue;  // stage s has no needed row yet
+        const int k = t - (s - 1);      // stage-1 iteration of this row
+        const int my = (t + 2 - s) % NW;  // the row's input slot
+        C kk[F];
+        D dv[F];
+        if (s == 1) {
+          stage_k(k, w[t % NW], w[(t + 1) % NW], w[(t + 2) % NW], wh[(t + 1) % NW],
+                  S == 2 ? hdt : dt, kk, dv);
+        } else {
+          // Y_s rows r-1, r, r+1 were produced at iterations t-2, t-1, t.
+          const C *yu = y[s - 2][(t + 1) % 3], *yc = y[s - 2][(t + 2) % 3],
+                  *yd = y[s - 2][t % 3];
+          stage_k(k, yu, yc, yd, yc, dt, kk, dv);
+        }
+        C *acc = kacc[k % NK];
+#pragma unroll
+        for (int f = 0; f < F; ++f) {
+          const C yh = w[my][f], yl = P::kComp ? wl[my][f] : C(0);
+          const bool fix = dv[f] == dv[f];
+          if (s < S) {
+            C nv;
+            if (S == 2) {
+              nv = state_plus<P>(yh, yl, kk[f]);
+            } else if (s == 1) {
+              acc[f] = kk[f];
+              nv = state_plus<P>(yh, yl, kk[f] / C(2));
+            } else if (s == 2) {
+              acc[f] = acc[f] + C(2) * kk[f];
+              nv = state_plus<P>(yh, yl, kk[f] / C(2));
+            } else {
+              acc[f] = acc[f] + C(2) * kk[f];
+              nv = state_plus<P>(yh, yl, kk[f]);
+            }
+            y[s - 1][t % 3][f] = fix ? static_cast<C>(dv[f]) : nv;
+          } else {
+            C hv = yh, lv = yl;
+            if (S == 2) {
+              state_add<P>(hv, lv, kk[f]);
+            } else {
+              acc[f] = acc[f] + kk[f];
+              state_add<P>(hv, lv, acc[f] / C(6));
+            }
+            C sh, sl;
+            state_set<P>(sh, sl, static_cast<double>(dv[f]));
+            kk[f] = fix ? sh : hv;  // the step's result (hi, lo)
+            dv[f] = static_cast<D>(0);
+            acc[f] = fix ? sl : lv;
+          }
+        }
+        if (s == S) store_row(r_first + k, kk, acc);
+      }
+    }
+  };
+  if (edge) run(std::true_type{});
+  else run(std::false_type{});
+}
+
 // Imposes the Dirichlet values on a state in place: one workgroup per
 // (grid, field) walks the grid's edge vertices.
 template <class P>
@@ -1229,6 +1465,12 @@ struct Variant {
         &fdm_stream<EQ, IN, PR, RW, PD, FULL, CPL, true>, kStreamKind, 64 * CPL, RW, 1,   \
         256, CPL, NAME                                                                    \
   }
+#define PML_STREAMRK(EQ, IN, PR, RW, PD, NAME)                                            \
+  Variant {                                                                               \
+    &fdm_stream_rk<EQ, IN, PR, RW, PD, false>, &fdm_stream_rk<EQ, IN, PR, RW, PD, false>, \
+        &fdm_stream_rk<EQ, IN, PR, RW, PD, true>, &fdm_stream_rk<EQ, IN, PR, RW, PD, true>, \
+        kStreamKind, 66 - 2 * IN::kStages, RW, 1, 256, 1, NAME                            \
+  }
 #define PML_STREAM2(EQ, IN, PR, RW, PD, NAME)                                             \
   Variant {                                                                               \
     &fdm_stream2<EQ, IN, PR, RW, PD, 2, false>, &fdm_stream2<EQ, IN, PR, RW, PD, 1, false>, \
@@ -1261,11 +1503,17 @@ const std::vector<Variant> &variants_for() {
           out.push_back(PML_TILE(EQ, IN, PR, 64, 32, 4, 512, "tile-ox64-oy32-s4"));
         }
       }
-    } else if constexpr (wide) {
-      out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
     } else {
-      out.push_back(PML_TILE(EQ, IN, PR, 64 - 2 * S, 16, 1, 256, "tile-ox64m-oy16-s1"));
-      out.push_back(PML_TILE(EQ, IN, PR, 64 - 2 * S, 32, 1, 256, "tile-ox64m-oy32-s1"));
+      out.push_back(PML_STREAMRK(EQ, IN, PR, 8, 1, "streamrk-rw8-pd1"));
+      out.push_back(PML_STREAMRK(EQ, IN, PR, 16, 1, "streamrk-rw16-pd1"));
+      out.push_back(PML_STREAMRK(EQ, IN, PR, 8, 2, "streamrk-rw8-pd2"));
+      out.push_back(PML_STREAMRK(EQ, IN, PR, 4, 1, "streamrk-rw4-pd1"));
+      if constexpr (wide) {
+        out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
+      } else {
+        out.push_back(PML_TILE(EQ, IN, PR, 64 - 2 * S, 16, 1, 256, "tile-ox64m-oy16-s1"));
+        out.push_back(PML_TILE(EQ, IN, PR, 64 - 2 * S, 32, 1, 256, "tile-ox64m-oy32-s1"));
+      }
     }
     return out;
   }();

@@ -18,7 +18,7 @@ integrators and ``ThreePointCentralDifferenceMethod``.  Anything else raises
 Extensions over the reference (keyword-only, defaults reproduce it):
 ``precision`` ("float64", "float32" or "float32c" = compensated FP32),
 ``snapshot_every`` (keep every k-th step only), ``tile_config`` (kernel
-variant; None = the tuned or default one) and ``use_graph``.
+variant; None = the tuned or default one, see ``tune``) and ``use_graph``.
 """
 
 from typing import Optional
@@ -82,12 +82,37 @@ class FDMOperator(Operator):
         dev.integrator_id(self._integrator)
         return dev.device_problem(cp)
 
-    def tile_config_for(self, problem) -> int:
+    def tile_config_for(self, problem, n_grids: int = 1) -> int:
+        """The kernel variant for a batch of ``n_grids`` grids of
+        ``problem``: the one given to the constructor, else the tuned one for
+        the batch's size class (``tune``), else the default."""
         if self._tile_config is not None:
             return int(self._tile_config)
         from pararealml_amd.autotune import lookup
 
-        return lookup(problem, dev.integrator_id(self._integrator), self._precision)
+        return lookup(problem, dev.integrator_id(self._integrator), self._precision,
+                      cells=problem.plane * int(n_grids))
+
+    def tune(self, cp, n_grids: int = 1, n_steps: int = 64, seed: int = 0) -> int:
+        """Times every kernel variant on a scratch batch of ``n_grids``
+        copies of ``cp``'s grid (random state, this operator's time step)
+        and records the fastest for the batch's size class; returns it."""
+        import torch
+
+        from pararealml_amd.autotune import tune
+
+        problem = self.device_problem(cp)
+        rng = np.random.RandomState(seed)
+        y = 1e-3 * rng.standard_normal(tuple(cp.mesh.vertices_shape) + (problem.n_fields,))
+        stream = dev.solver_stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            batch = dev.FDMBatch([problem], [0] * int(n_grids), self._precision,
+                                 dev.integrator_id(self._integrator), stream)
+            batch.load_host([y] * int(n_grids))
+            best, _ = tune(batch, self.d_t, n_steps=n_steps)
+        torch.cuda.current_stream().wait_stream(stream)
+        return best
 
     def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False):
         """Device propagator over Parareal time slices (see

@@ -180,7 +180,7 @@ def generate_pairs(fdm_op, cp, y0s: np.ndarray, t_span: float, pair_dt: float,
         tmp = torch.empty((n * wp.n_fields, wp.n0, wp.n1), dtype=torch.float64, device="cuda")
         for k in range(n_pairs + 1):
             if k:
-                batch.advance(fdm_op.d_t, steps, tile_config=fdm_op.tile_config_for(problem),
+                batch.advance(fdm_op.d_t, steps, tile_config=fdm_op.tile_config_for(problem, n),
                               use_graph=True)
             batch.state_f64(f64)
             if coarsen:

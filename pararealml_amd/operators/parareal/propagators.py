@@ -46,7 +46,6 @@ class FDMSlicePropagator:
             stream,
         )
         self.elems = self.problem.elems
-        self.tile_config = op.tile_config_for(self.problem)
         self.snaps = None
         if record_trajectory:
             self.snaps = self.batch.new_snapshots(slice_steps)
@@ -75,7 +74,7 @@ class FDMSlicePropagator:
             self.steps,
             snapshots=self.snaps,
             snapshot_every=1 if self.snaps is not None else 0,
-            tile_config=self.tile_config,
+            tile_config=self.op.tile_config_for(self.problem, count),
             use_graph=True,
             constrain_input=True,
             first_grid=first,

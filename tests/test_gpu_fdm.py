@@ -280,9 +280,10 @@ def _mixed_bc_diffusion(n0, n1):
     return cp, ic
 
 
+@pytest.mark.parametrize("integ", ["fe", "midpoint", "rk4"])
 @pytest.mark.parametrize("prec", ["float64", "float32", "float32c"])
 @pytest.mark.parametrize("shapes", [((21, 130), (34, 66)), ((21, 130), (17, 33))])
-def test_stream_variants_identical_with_boundaries_and_snapshots(prec, shapes):
+def test_stream_variants_identical_with_boundaries_and_snapshots(integ, prec, shapes):
     """Every variant (one and two columns per lane, ring depths, two steps
     per launch), eager and graph launches, on a ragged batch with Dirichlet,
     Neumann and Cauchy edges, strips that end inside the grid and snapshots
@@ -290,7 +291,7 @@ def test_stream_variants_identical_with_boundaries_and_snapshots(prec, shapes):
     bitwise-identical states and snapshots.  The second batch has an odd row length, where the
     two-column variants hand over to the default one.  FP64 matches the
     oracle per grid."""
-    print(f"case: stream variants {prec} {shapes}", flush=True)
+    print(f"case: stream variants {integ} {prec} {shapes}", flush=True)
     from pararealml_amd import _device as dev
     from pararealml_amd import _native as nat
 
@@ -298,15 +299,16 @@ def test_stream_variants_identical_with_boundaries_and_snapshots(prec, shapes):
     problems = [dev.device_problem(cp) for cp, _ in cases]
     y0s = [ic.discrete_y_0(True) for _, ic in cases]
     lib = nat.device_lib()
-    n_cfg = lib.pml_fdm_configs(nat.PML_EQ["diffusion"], nat.PML_INT_FE, dev.precision_id(prec))
-    names = [lib.pml_fdm_config_name(nat.PML_EQ["diffusion"], nat.PML_INT_FE,
+    int_id = {"fe": nat.PML_INT_FE, "midpoint": nat.PML_INT_MIDPOINT, "rk4": nat.PML_INT_RK4}[integ]
+    n_cfg = lib.pml_fdm_configs(nat.PML_EQ["diffusion"], int_id, dev.precision_id(prec))
+    names = [lib.pml_fdm_config_name(nat.PML_EQ["diffusion"], int_id,
                                      dev.precision_id(prec), c).decode() for c in range(n_cfg)]
-    assert any("-c2-" in n for n in names)
+    assert any(("-c2-" if integ == "fe" else "streamrk-") in n for n in names)
     d_t, steps, every = 2e-4, 37, 4
     outs = []
     for cfg in range(n_cfg):
         for graph in (False, True):
-            b = dev.FDMBatch(problems, [0, 1], prec, nat.PML_INT_FE, dev.solver_stream())
+            b = dev.FDMBatch(problems, [0, 1], prec, int_id, dev.solver_stream())
             b.load_host(y0s)
             snaps = b.new_snapshots(steps // every)
             b.advance(d_t, steps, snapshots=snaps, snapshot_every=every, tile_config=cfg,
@@ -319,7 +321,7 @@ def test_stream_variants_identical_with_boundaries_and_snapshots(prec, shapes):
     if prec == "float64":
         flat = outs[0][2]
         for k, (cp, ic) in enumerate(cases):
-            _, y_ref = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, steps * d_t), d_t, "fe",
+            _, y_ref = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, steps * d_t), d_t, integ,
                                      keep="last")
             got = problems[k].from_device_layout(b.grid_view(flat, k), cp.mesh.vertices_shape)
             assert rel_err(got, y_ref[0]) <= 1e-12
