@@ -48,7 +48,10 @@ def parse_args():
     ap.add_argument("--pr-slices", type=int, default=128)
     ap.add_argument("--pr-fine-steps", type=int, default=40000, help="fine RK4 steps per slice")
     ap.add_argument("--pr-coarse-steps", type=int, default=800, help="coarse FE steps per slice")
-    ap.add_argument("--pr-substeps", type=int, default=8, help="network applications per slice")
+    ap.add_argument("--pr-base-steps", type=int, default=320,
+                    help="FE steps per slice of the learned operator's base solve")
+    ap.add_argument("--pr-base-levels", type=int, default=1, choices=[1, 2],
+                    help="base solve on the 2x (1) or 4x (2) coarsened grid")
     ap.add_argument("--pr-traj", type=int, default=256, help="training trajectories")
     ap.add_argument("--pr-train-steps", type=int, default=3000)
     ap.add_argument("--pr-target", type=float, default=1e-6)
@@ -145,10 +148,11 @@ def parareal_workload(args):
     means in [0.2, 0.8]^2 and variances in [0.005, 0.02] from
     RandomState(434342745); t in [0, 1] split into S slices; fine RK4 with
     the configured steps per slice; FDM coarse = forward Euler on the
-    2x-coarsened grid; learned coarse = the conv surrogate on the
-    2x-coarsened grid, K applications per slice, trained on trajectories of
-    the device FDM solver from other seeded Gaussian mixtures, with the FDM
-    coarse operator as fallback."""
+    2x-coarsened grid; learned coarse = forward Euler with a large (still
+    stable) step on the 2x-coarsened grid (or the 4x one) corrected by one
+    conv-surrogate application per slice on the 2x grid, trained on trajectories of the device FDM solver from other seeded
+    Gaussian mixtures, with the FDM coarse operator as fallback; the same
+    cheap pipeline without the network is run as the ablation."""
     import torch
 
     import pararealml_amd as pml
@@ -160,7 +164,8 @@ def parareal_workload(args):
     )
     from pararealml_amd.operators.ml import (
         ConvSurrogate,
-        LearnedCoarseOperator,
+        CorrectedCoarseOperator,
+        corrected_pairs,
         gaussian_mixture_states,
         generate_pairs,
     )
@@ -249,20 +254,26 @@ def parareal_workload(args):
     g_fdm = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), diff, dt_c))
     fdm_res = run_parareal(g_fdm, "FDM coarse")
 
-    # learned coarse: data from the device FDM solver, then training
-    K = args.pr_substeps
-    sub = 1.0 / (S * K)
-    data_steps = int(np.ceil(sub / (0.9 * 2.78 * (1.0 / n) ** 2 / (8 * 0.5))))
-    data_op = FDMOperator(RK4(), diff, sub / data_steps, precision="float32")
+    # Learned coarse: a cheap FE solve on the 4x-coarsened grid corrected by
+    # one surrogate application per slice on the 2x grid.  Training data:
+    # the device FDM solver's trajectories from other seeded Gaussian
+    # mixtures, one slice apart, restricted to the 2x grid.
+    delta = 1.0 / S
+    lv = args.pr_base_levels
+    base = FDMOperator(ForwardEulerMethod(), diff, delta / args.pr_base_steps)
+    base_cp = coarse_problem(cp) if lv == 1 else coarse_problem(coarse_problem(cp))
+    base.tune(base_cp, 1)
+    data_steps = int(np.ceil(delta / (0.9 * 2.78 * (1.0 / n) ** 2 / (8 * 0.5))))
+    data_op = FDMOperator(RK4(), diff, delta / data_steps, precision="float32")
     log(f"parareal: generating {args.pr_traj} training trajectories")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     y0s = gaussian_mixture_states(cp, args.pr_traj, seed=SEEDS[2])
-    snaps = generate_pairs(data_op, cp, y0s, 1.0, sub, coarsen=True)
+    snaps = generate_pairs(data_op, cp, y0s, 1.0, delta, coarsen=True)
+    inputs, targets = corrected_pairs(base, cp, snaps, delta, base_levels=lv)
+    del snaps
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
-    inputs = snaps[:-1].reshape(-1, *snaps.shape[2:])
-    targets = snaps[1:].reshape(-1, *snaps.shape[2:])
     net = ConvSurrogate(1, seed=0)
     log(f"parareal: training on {inputs.shape[0]} pairs")
     torch.cuda.synchronize()
@@ -271,10 +282,16 @@ def parareal_workload(args):
                      log_every=max(1, args.pr_train_steps // 10))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
-    del snaps, inputs, targets
+    n_pairs = int(inputs.shape[0])
+    del inputs, targets
     torch.cuda.empty_cache()
-    g_learned = LearnedCoarseOperator(net, sub, coarsen=True, fallback=g_fdm)
+    g_learned = CorrectedCoarseOperator(base, net, fallback=g_fdm, base_levels=lv)
     learned_res = run_parareal(g_learned, "learned coarse")
+    # The same pipeline without the learned correction (an untrained
+    # surrogate is exactly the identity): what the network gains.
+    g_plain = CorrectedCoarseOperator(base, ConvSurrogate(1, seed=0), fallback=g_fdm,
+                                      base_levels=lv)
+    plain_res = run_parareal(g_plain, "cheap coarse without the network")
 
     value = learned_res["time_to_target_s"]
     line = {
@@ -294,8 +311,9 @@ def parareal_workload(args):
             "workload": (
                 f"Parareal 2-D diffusion d=0.5, {n + 1}^2 vertices, Dirichlet 0, t in [0,1], "
                 f"{S} slices, fine RK4 {args.pr_fine_steps} steps/slice, FDM coarse FE "
-                f"{args.pr_coarse_steps} steps/slice on the 2x grid, learned coarse {K} "
-                f"network steps/slice on the 2x grid"
+                f"{args.pr_coarse_steps} steps/slice on the 2x grid, learned coarse = FE "
+                f"{args.pr_base_steps} steps/slice on the {2 ** lv}x grid + one conv-surrogate "
+                f"correction on the 2x grid"
             ),
             "target_rel_error": target,
         },
@@ -306,9 +324,10 @@ def parareal_workload(args):
             "fdm_coarse": fdm_res,
             "learned_coarse": learned_res,
             "learned_fallbacks": g_learned.fallbacks,
+            "cheap_coarse_without_network": plain_res,
             "training": {
                 "trajectories": args.pr_traj,
-                "pairs": int((S * K) * args.pr_traj),
+                "pairs": n_pairs,
                 "data_generation_s": t_data,
                 "train_steps": args.pr_train_steps,
                 "train_s": t_train,

@@ -3,13 +3,17 @@ device FDM solver's trajectories."""
 
 from pararealml_amd.operators.ml.conv_surrogate import ConvSurrogate
 from pararealml_amd.operators.ml.learned_coarse import (
+    CorrectedCoarseOperator,
     LearnedCoarseOperator,
+    corrected_pairs,
     gaussian_mixture_states,
     generate_pairs,
 )
 
 __all__ = [
     "ConvSurrogate",
+    "CorrectedCoarseOperator",
+    "corrected_pairs",
     "LearnedCoarseOperator",
     "gaussian_mixture_states",
     "generate_pairs",

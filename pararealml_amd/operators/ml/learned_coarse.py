@@ -1,6 +1,12 @@
 """The convolutional surrogate as a Parareal coarse propagator, its training
 data generation from the device FDM solver, and the FDM fallback.
 
+Two forms: ``LearnedCoarseOperator`` replaces the coarse solve by repeated
+surrogate steps; ``CorrectedCoarseOperator`` runs a cheap FDM solve on a
+further coarsened grid and applies the surrogate once per slice as a
+learned correction (stable under the Parareal sweep's repeated
+application).
+
 ``LearnedCoarseOperator(surrogate, d_t, coarsen=True, fallback=None)`` steps
 the state by ``d_t`` per network application (the reference's learned
 operators step autoregressively by their ``d_t``,
@@ -121,6 +127,169 @@ class LearnedPropagator:
                     self.op.fallbacks += 1
                     self.fallback.run(u_f64[k : k + 1], first + k, first + k + 1,
                                       out_f64[k : k + 1])
+
+
+class CorrectedCoarseOperator:
+    """Parareal coarse operator: a cheap FDM solve corrected by one surrogate
+    application per slice on the 2x-coarsened grid.
+
+    Per slice: restrict the state to the 2x grid (and, with
+    ``base_levels = 2``, again to the 4x grid), advance it over the slice with
+    ``base`` (an ``FDMOperator`` with a step that is stable on its grid but
+    large, so the solve is cheap and inaccurate), prolong the result back to
+    the 2x grid, apply the
+    surrogate once (``u + s net(u / s)``, trained by ``corrected_pairs`` to
+    map the base prediction to the fine solution restricted to the 2x grid)
+    and prolong to the fine grid; the Dirichlet values are imposed on every
+    grid.  The base solve is contractive, so the network only has to learn a
+    small, local correction and repeated application stays bounded.  With
+    ``base_levels = 1`` the base sees the whole 2x-grid state, so the
+    operator's response to perturbations (what sets Parareal's convergence)
+    keeps their 2x-grid content; a 4x base discards it.  A
+    non-finite slice, or one whose magnitude exceeds ``guard`` times its
+    input's, is recomputed by ``fallback`` and counted in ``fallbacks``."""
+
+    def __init__(self, base, surrogate: ConvSurrogate, fallback=None, guard: float = 10.0,
+                 base_levels: int = 2):
+        if base_levels not in (1, 2):
+            raise ValueError("the base solve runs on the 2x (1) or 4x (2) coarsened grid")
+        self.base = base
+        self.surrogate = surrogate
+        self.fallback = fallback
+        self.guard = guard
+        self.base_levels = base_levels
+        self.fallbacks = 0
+
+    @property
+    def d_t(self) -> float:
+        return self.base.d_t
+
+    @property
+    def vertex_oriented(self) -> Optional[bool]:
+        return True
+
+    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False):
+        fb = None
+        if self.fallback is not None:
+            fb = self.fallback.slice_propagator(
+                cp, n_slices, int(round(slice_steps * self.d_t / self.fallback.d_t)), stream
+            )
+        return CorrectedPropagator(self, cp, slice_steps, stream, fb)
+
+
+class CorrectedPropagator:
+    def __init__(self, op: CorrectedCoarseOperator, cp, steps: int, stream, fallback_prop):
+        import torch
+
+        from pararealml_amd.operators.parareal.propagators import FDMSlicePropagator
+
+        self.op = op
+        self.stream = stream
+        self.fallback = fallback_prop
+        self.torch = torch
+        self.fine = dev.device_problem(cp)
+        work_cp = coarse_problem(cp)
+        self.work = dev.device_problem(work_cp)
+        self.base_cp = coarse_problem(work_cp) if op.base_levels == 2 else work_cp
+        self.basep = dev.device_problem(self.base_cp)
+        self.inner = FDMSlicePropagator(op.base, self.base_cp, 1, steps, stream)
+        self.bc_fine = torch.from_numpy(self.fine.bc).to("cuda")
+        self.bc_work = torch.from_numpy(self.work.bc).to("cuda")
+        f64 = dict(dtype=torch.float64, device="cuda")
+        self.buf = torch.empty((1, self.work.n_fields, self.work.n0, self.work.n1), **f64)
+        self.b_in = torch.empty((1, self.basep.elems), **f64)
+        self.b_out = torch.empty((1, self.basep.elems), **f64)
+        self.lib = nat.device_lib()
+
+    def _dirichlet(self, t, p, bc):
+        nat.check(self.lib.pml_apply_dirichlet(
+            t.data_ptr(), bc.data_ptr(), p.n_fields, p.n0, p.n1, p.ndim, 1,
+            ctypes.c_void_p(self.stream.cuda_stream)))
+
+    def run(self, u_f64, first: int, last: int, out_f64):
+        torch = self.torch
+        sp = ctypes.c_void_p(self.stream.cuda_stream)
+        w, b = self.work, self.basep
+        for k in range(last - first):
+            src = u_f64[k]
+            nat.check(self.lib.pml_restrict2x(nat.PML_F64, src.data_ptr(), self.buf.data_ptr(),
+                                              w.n_fields, w.n0, w.n1, sp))
+            self._dirichlet(self.buf, w, self.bc_work)
+            if self.op.base_levels == 2:
+                nat.check(self.lib.pml_restrict2x(nat.PML_F64, self.buf.data_ptr(),
+                                                  self.b_in.data_ptr(), b.n_fields, b.n0, b.n1,
+                                                  sp))
+                self.inner.run(self.b_in, 0, 1, self.b_out)
+                nat.check(self.lib.pml_prolong2x(nat.PML_F64, self.b_out.data_ptr(),
+                                                 self.buf.data_ptr(), b.n_fields, b.n0, b.n1, sp))
+            else:
+                self.inner.run(self.buf.view(1, -1), 0, 1, self.buf.view(1, -1))
+            self._dirichlet(self.buf, w, self.bc_work)
+            self.op.surrogate.predict(self.buf, out=self.buf)
+            self._dirichlet(self.buf, w, self.bc_work)
+            dst = out_f64[k]
+            nat.check(self.lib.pml_prolong2x(nat.PML_F64, self.buf.data_ptr(), dst.data_ptr(),
+                                             w.n_fields, w.n0, w.n1, sp))
+            self._dirichlet(dst, self.fine, self.bc_fine)
+            if self.fallback is not None:
+                bad = (~torch.isfinite(dst)).any() | (
+                    dst.abs().max() > self.op.guard * src.abs().max().clamp_min(1e-300)
+                )
+                if bool(bad.item()):
+                    self.op.fallbacks += 1
+                    self.fallback.run(u_f64[k : k + 1], first + k, first + k + 1,
+                                      out_f64[k : k + 1])
+
+
+def corrected_pairs(base_op, cp, snaps, slice_dt: float, stream=None, base_levels: int = 2):
+    """Training pairs for ``CorrectedCoarseOperator`` from trajectories on
+    the 2x-coarsened grid (``snaps``: FP32 device tensor (T, N, C, H, W),
+    states one slice apart, as ``generate_pairs`` returns them): the input
+    is the base operator's prediction over one slice from state t (on the 2x
+    grid, or restricted to the 4x grid and prolonged back for
+    ``base_levels = 2``; Dirichlet values imposed), the target is state
+    t + 1.  Returns (inputs, targets), each ((T-1) N, C, H,
+    W) FP32."""
+    import torch
+
+    stream = stream if stream is not None else dev.solver_stream()
+    n_t, n, c, h, w = snaps.shape
+    work_cp = coarse_problem(cp)
+    wp = dev.device_problem(work_cp)
+    base_cp = coarse_problem(work_cp) if base_levels == 2 else work_cp
+    bp = base_op.device_problem(base_cp)
+    steps = int(round(slice_dt / base_op.d_t))
+    lib = nat.device_lib()
+    sp = ctypes.c_void_p(stream.cuda_stream)
+    inputs = torch.empty((n_t - 1, n, c, h, w), dtype=torch.float32, device="cuda")
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        batch = dev.FDMBatch([bp], [0] * n, base_op.precision,
+                             dev.integrator_id(base_op.integrator), stream)
+        bc_work = torch.from_numpy(wp.bc).to("cuda")
+        x4 = torch.empty(n * bp.elems, dtype=torch.float64, device="cuda")
+        y2 = torch.empty((n * c, h, w), dtype=torch.float64, device="cuda")
+        for t in range(n_t - 1):
+            x2 = snaps[t].to(torch.float64).contiguous()
+            if base_levels == 2:
+                nat.check(lib.pml_restrict2x(nat.PML_F64, x2.data_ptr(), x4.data_ptr(), n * c,
+                                             bp.n0, bp.n1, sp))
+                batch.load_f64(x4)
+            else:
+                batch.load_f64(x2.view(-1))
+            batch.advance(base_op.d_t, steps, tile_config=base_op.tile_config_for(bp, n),
+                          use_graph=True, constrain_input=True)
+            y4 = batch.state_f64()
+            if base_levels == 2:
+                nat.check(lib.pml_prolong2x(nat.PML_F64, y4.data_ptr(), y2.data_ptr(), n * c,
+                                            bp.n0, bp.n1, sp))
+            else:
+                y2.view(-1).copy_(y4)
+            nat.check(lib.pml_apply_dirichlet(y2.data_ptr(), bc_work.data_ptr(), c, wp.n0,
+                                              wp.n1, wp.ndim, n, sp))
+            inputs[t].copy_(y2.view(n, c, h, w))
+    torch.cuda.current_stream().wait_stream(stream)
+    return inputs.reshape(-1, c, h, w), snaps[1:].reshape(-1, c, h, w)
 
 
 def gaussian_mixture_states(cp, n: int, seed: int, components: int = 4,

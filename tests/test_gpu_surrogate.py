@@ -239,3 +239,89 @@ def test_learned_coarse_fallback_on_non_finite_output():
     assert p_learned.last_iterations == p_fdm.last_iterations
     np.testing.assert_allclose(y_l, y_f, rtol=0, atol=1e-12)
     torch.cuda.synchronize()
+
+
+def _diffusion_problem(n=32):
+    import pararealml_amd as pml
+    from tests.helpers import zero_bc
+
+    eq = pml.DiffusionEquation(2, 0.5)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1 / n, 1 / n])
+    bc = zero_bc(1)
+    return pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+
+
+def test_corrected_coarse_with_identity_network_is_the_coarse_grid_fdm():
+    """With an untrained surrogate (exactly the identity) and the base on the
+    2x grid, the corrected operator is the coarse-grid FDM operator with the
+    same base: identical Parareal iterates."""
+    import pararealml_amd as pml
+    from pararealml_amd.operators.fdm import (
+        RK4,
+        FDMOperator,
+        ForwardEulerMethod,
+        ThreePointCentralDifferenceMethod,
+    )
+    from pararealml_amd.operators.ml import ConvSurrogate, CorrectedCoarseOperator
+    from pararealml_amd.operators.parareal import CoarseGridFDMOperator, PararealOperator
+
+    print("case: corrected coarse identity", flush=True)
+    diff = ThreePointCentralDifferenceMethod()
+    cp = _diffusion_problem()
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.5, 0.45]), 0.01 * np.eye(2))])
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.02), ic)
+    f = FDMOperator(RK4(), diff, 1e-4)
+    base = FDMOperator(ForwardEulerMethod(), diff, 5e-4)
+    g = CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=1)
+    p_c = PararealOperator(f, g, 1e-7, n_slices=4, output="endpoints")
+    y_c = p_c.solve(ivp).discrete_y()
+    p_f = PararealOperator(f, CoarseGridFDMOperator(base), 1e-7, n_slices=4, output="endpoints")
+    y_f = p_f.solve(ivp).discrete_y()
+    assert p_c.last_iterations == p_f.last_iterations
+    np.testing.assert_array_equal(y_c, y_f)
+    with pytest.raises(ValueError):
+        CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=3)
+
+
+def test_trained_correction_reduces_the_coarse_error_on_held_out_states():
+    """What the network gains: trained on (cheap base prediction, fine
+    solution) pairs, the corrected prediction over one slice is closer to
+    the fine solution than the base prediction alone (at least 2x), on
+    initial states not seen in training."""
+    import torch
+
+    from pararealml_amd.operators.fdm import (
+        RK4,
+        FDMOperator,
+        ForwardEulerMethod,
+        ThreePointCentralDifferenceMethod,
+    )
+    from pararealml_amd.operators.ml import (
+        ConvSurrogate,
+        corrected_pairs,
+        gaussian_mixture_states,
+        generate_pairs,
+    )
+
+    print("case: corrected coarse gain", flush=True)
+    n, slices = 64, 16
+    diff = ThreePointCentralDifferenceMethod()
+    cp = _diffusion_problem(n)
+    delta = 1.0 / slices
+    base = FDMOperator(ForwardEulerMethod(), diff, delta / 40)
+    # RK4 at 90% of its stability limit for the 2-D diffusion stencil
+    data_steps = int(np.ceil(delta / (0.9 * 2.78 * (1.0 / n) ** 2 / (8 * 0.5))))
+    data_op = FDMOperator(RK4(), diff, delta / data_steps, precision="float32")
+    train = generate_pairs(data_op, cp, gaussian_mixture_states(cp, 64, seed=1), 1.0, delta)
+    inp, tgt = corrected_pairs(base, cp, train, delta)
+    net = ConvSurrogate(1, seed=0)
+    losses = net.fit(inp, tgt, steps=1500, batch=32, lr=1e-3)
+    assert np.mean(losses[-20:]) < 0.05 * np.mean(losses[:5])
+    held = generate_pairs(data_op, cp, gaussian_mixture_states(cp, 4, seed=2), 0.5, delta)
+    h_in, h_tg = corrected_pairs(base, cp, held, delta)
+    x, t = h_in.double(), h_tg.double()
+    pred = net.predict(x.clone())
+    err_base = float(torch.linalg.norm(x - t) / torch.linalg.norm(t))
+    err_net = float(torch.linalg.norm(pred - t) / torch.linalg.norm(t))
+    print(f"held-out relative error: base {err_base:.3e}, corrected {err_net:.3e}", flush=True)
+    assert err_net < 0.5 * err_base
