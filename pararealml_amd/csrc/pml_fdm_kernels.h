@@ -534,7 +534,7 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
           const C kk = dt * rr[f];
           C d = kk;
           if (Eq::kInvariant && Integ::kStages == 4)
-            d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
+            d = (((kk + C(2) * kk) + C(2) * kk) + kk) * C(1.0 / 6.0);
           state_add<P>(hv, lv, d);
         }
         if (dv && dv[c][f] == dv[c][f]) state_set<P>(hv, lv, dv[c][f]);
@@ -902,7 +902,7 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
           const C kk = dt * rr[f];
           C d = kk;
           if (Eq::kInvariant && Integ::kStages == 4)
-            d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
+            d = (((kk + C(2) * kk) + C(2) * kk) + kk) * C(1.0 / 6.0);
           state_add<P>(hv, lv, d);
         }
         if (kCols || kRows) {
@@ -1185,7 +1185,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
               state_add<P>(hv, lv, kk[f]);
             } else {
               acc[f] = acc[f] + kk[f];
-              state_add<P>(hv, lv, acc[f] / C(6));
+              state_add<P>(hv, lv, acc[f] * C(1.0 / 6.0));
             }
             C sh, sl;
             state_set<P>(sh, sl, static_cast<double>(dv[f]));
@@ -1195,6 +1195,225 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
           }
         }
         if (s == S) store_row(r_first + k, kk, acc);
+      }
+    }
+  };
+  if (edge) run(std::true_type{});
+  else run(std::false_type{});
+}
+
+// ---------------------------------------------------------------------------
+// fdm_stream_n: NSTEP single-stage steps per launch as a row pipeline (the
+// generalisation of fdm_stream2 to deeper temporal blocking): step s works
+// on the row s-1 above step 1 and reads step s-1's rows from a three-row
+// register ring (hi and lo parts), so the state crosses HBM once per NSTEP
+// steps.  Lane l owns column j0 - (NSTEP-1) + l; step s is valid on lanes
+// s-1 .. 64-s and the result is written by lanes NSTEP-1 .. 64-NSTEP
+// (strips of 66 - 2 NSTEP columns).  Per-step arithmetic as fdm_stream2.
+template <class Eq, class Integ, class P, int RW, int PD, int NSTEP, bool SNAP>
+__global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
+  using C = typename P::C;
+  using D = typename std::conditional<P::kComp || sizeof(C) == 8, double, C>::type;
+  constexpr int F = Eq::F;
+  constexpr int NW = PD + 3;             // input ring slots
+  constexpr int NT = RW + 2 * (NSTEP - 1);  // pipeline iterations
+  static_assert(Eq::kInvariant || Integ::kStages == 1, "single-stage steps only");
+  static_assert(NSTEP >= 1 && NSTEP <= 4, "one to four steps per launch");
+  static_assert(PD >= 1, "the ring must prefetch at least one row");
+  static_assert(NT <= 64, "one lane per step-1 row of the unit");
+  const int lane = threadIdx.x & 63;
+  const int unit = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * 4 +
+                                                  static_cast<int>(threadIdx.x >> 6));
+  if (unit >= a.n_tiles) return;  // whole wavefront: no barriers follow
+  const TileDev U = a.tiles[unit];
+  const GridDev G = a.grids[U.g];
+  const int n0 = G.n0, n1 = G.n1;
+  const bool ax0 = a.ndim == 2;
+  const unsigned plane = static_cast<unsigned>(n0) * n1;
+  const double *bcg = a.bc + G.bc;
+  const int fstride = 4 * (n0 + n1);
+  const StencilK<C> K(G);
+  C pc[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) pc[i] = static_cast<C>(a.p[i]);
+  const C dt = static_cast<C>(a.dt);
+  const int i_beg = U.i0;
+  const int i_end = min(U.i0 + RW, n0);
+  const int c0 = U.j0 - (NSTEP - 1);  // column of lane 0
+  const int col = c0 + lane;
+  const bool lo1 = col == 0, hi1 = col == n1 - 1;
+  const int cc = min(max(col, 0), n1 - 1);
+  const int r_first = i_beg - (NSTEP - 1);  // first step-1 row
+  const bool edge = !ax0 || c0 <= 0 || c0 + 63 >= n1 - 1 || r_first <= 0 ||
+                    i_end - 1 + (NSTEP - 1) >= n0 - 1;  // wave-uniform
+
+  const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
+  const C *in_lo = P::kComp ? static_cast<const C *>(a.in_lo) + G.off : nullptr;
+  C *out_hi = static_cast<C *>(a.out_hi) + G.off;
+  C *out_lo = P::kComp ? static_cast<C *>(a.out_lo) + G.off : nullptr;
+  C *sn_hi = SNAP ? static_cast<C *>(a.snap_hi) + G.off : nullptr;
+  C *sn_lo = (SNAP && P::kComp) ? static_cast<C *>(a.snap_lo) + G.off : nullptr;
+  const unsigned esz = static_cast<unsigned>(sizeof(C));
+  const unsigned jb = static_cast<unsigned>(col) * esz;  // columns < 0 read 0
+  const int hcol = lane == 0 ? c0 - 1 : (lane == 63 ? c0 + 64 : col);
+  const unsigned hb = static_cast<unsigned>(hcol) * esz;
+  const unsigned sb = (lane >= NSTEP - 1 && lane <= 64 - NSTEP) ? jb : 0x80000000u;
+  const unsigned row_bytes = static_cast<unsigned>(n1) * esz;
+
+  auto load = [&](int r, C *v, C *vl, C *h) {
+    const bool ok = r >= 0 && r < n0 && r <= i_end + NSTEP - 1;
+    const size_t ro = static_cast<size_t>(min(max(r, 0), n0 - 1)) * n1;
+    const unsigned nb = ok ? row_bytes : 0u;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const auto rs = row_rsrc(in_hi + static_cast<size_t>(f) * plane + ro, nb);
+      v[f] = buf_load<C, 1>(rs, jb).v[0];
+      if (P::kComp)
+        vl[f] = buf_load<C, 1>(row_rsrc(in_lo + static_cast<size_t>(f) * plane + ro, nb), jb).v[0];
+      h[f] = buf_load<C, 1>(rs, hb).v[0];
+    }
+  };
+  auto store_row = [&](int r, const C *vh, const C *vl) {
+    const size_t ro = static_cast<size_t>(r) * n1;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const size_t fo = static_cast<size_t>(f) * plane + ro;
+      buf_store<C, 1>(row_rsrc(out_hi + fo, row_bytes), sb, Vec<C, 1>{{vh[f]}});
+      if (P::kComp) buf_store<C, 1>(row_rsrc(out_lo + fo, row_bytes), sb, Vec<C, 1>{{vl[f]}});
+      if (SNAP) {
+        buf_store<C, 1>(row_rsrc(sn_hi + fo, row_bytes), sb, Vec<C, 1>{{vh[f]}});
+        if (P::kComp) buf_store<C, 1>(row_rsrc(sn_lo + fo, row_bytes), sb, Vec<C, 1>{{vl[f]}});
+      }
+    }
+  };
+
+  auto run = [&](auto edge_tag) {
+    constexpr bool kE = decltype(edge_tag)::value;
+    C rn_lo[F], rn_hi[F], pe_lo[F], pe_hi[F];
+    D rd_lo[F], rd_hi[F], pd_lo[F], pd_hi[F];
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      rn_lo[f] = rn_hi[f] = pe_lo[f] = pe_hi[f] = static_cast<C>(NAN);
+      rd_lo[f] = rd_hi[f] = pd_lo[f] = pd_hi[f] = static_cast<D>(NAN);
+      if (kE) {
+        const double *fb = bcg + f * fstride;
+        const double *nb = fb + 2 * (n0 + n1);
+        if (ax0) {
+          rd_lo[f] = static_cast<D>(fb[cc]);
+          rd_hi[f] = static_cast<D>(fb[n1 + cc]);
+          rn_lo[f] = static_cast<C>(nb[cc]);
+          rn_hi[f] = static_cast<C>(nb[n1 + cc]);
+        }
+        const int rk = min(max(r_first + lane, 0), n0 - 1);
+        pd_lo[f] = static_cast<D>(fb[2 * n1 + rk]);
+        pd_hi[f] = static_cast<D>(fb[2 * n1 + n0 + rk]);
+        pe_lo[f] = static_cast<C>(nb[2 * n1 + rk]);
+        pe_hi[f] = static_cast<C>(nb[2 * n1 + n0 + rk]);
+      }
+    }
+    // One step of row r = r_first + k from rows (yu, yc, yd) of the
+    // previous state (hi parts; yc_lo the lo parts of row r).
+    auto step_row = [&](int k, const C *yu, const C *yc, const C *yc_lo, const C *yd,
+                        const C *fill, C *nh, C *nl) {
+      const int r = r_first + k;
+      const bool lo0 = kE && ax0 && r == 0, hi0 = kE && ax0 && r == n0 - 1;
+      const bool l1 = kE && lo1, h1 = kE && hi1;
+      C e_lo[F], e_hi[F];
+      D d_lo[F], d_hi[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        e_lo[f] = e_hi[f] = static_cast<C>(NAN);
+        d_lo[f] = d_hi[f] = static_cast<D>(NAN);
+        if (kE) {
+          d_lo[f] = lane_value(pd_lo[f], k);
+          d_hi[f] = lane_value(pd_hi[f], k);
+          e_lo[f] = lane_value(pe_lo[f], k);
+          e_hi[f] = lane_value(pe_hi[f], k);
+        }
+      }
+      Quant<C> q[F];
+#define PML_FQN(F_)                                                                          \
+  if constexpr (F_ < F) {                                                                    \
+    const C yw = wave_shr1<C>(yc[F_], fill[F_]), ye = wave_shl1<C>(yc[F_], fill[F_]);        \
+    field_quant_sel<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD>(q[F_], yc[F_], yw, ye, yu[F_],    \
+                                                           yd[F_], l1, h1, lo0, hi0, ax0,    \
+                                                           e_lo[F_], e_hi[F_], rn_lo[F_],    \
+                                                           rn_hi[F_], K);                    \
+  }
+      PML_FQN(0)
+      PML_FQN(1)
+      PML_FQN(2)
+#undef PML_FQN
+      C rr[F], ra[F];
+#pragma unroll
+      for (int f = 0; f < F; ++f) rr[f] = ra[f] = 0;
+      Eq::template rhs<C>(pc, q, rr);
+      if (Eq::kAlg != 0u) Eq::template alg<C>(pc, q, ra);
+#pragma unroll
+      for (int f = 0; f < F; ++f) {
+        C hv = yc[f], lv = P::kComp ? yc_lo[f] : C(0);
+        if ((Eq::kAlg >> f) & 1u) {
+          hv = ra[f];
+          lv = 0;
+        } else {
+          const C kk = dt * rr[f];
+          C d = kk;
+          if (Eq::kInvariant && Integ::kStages == 4)
+            d = (((kk + C(2) * kk) + C(2) * kk) + kk) * C(1.0 / 6.0);
+          state_add<P>(hv, lv, d);
+        }
+        if (kE) {
+          D dv = lo0 ? rd_lo[f] : (hi0 ? rd_hi[f] : static_cast<D>(NAN));
+          dv = (dv == dv) ? dv : (l1 ? d_lo[f] : (h1 ? d_hi[f] : static_cast<D>(NAN)));
+          C sh, sl;
+          state_set<P>(sh, sl, static_cast<double>(dv));
+          const bool fix = dv == dv;
+          hv = fix ? sh : hv;
+          lv = fix ? sl : lv;
+        }
+        nh[f] = hv;
+        nl[f] = lv;
+      }
+    };
+
+    // Input ring: row i_beg - NSTEP + m in slot m mod NW.  Level rings:
+    // the output of step s (s < NSTEP) for the row produced at iteration t
+    // in slot t mod 3.
+    C w[NW][F], wl[NW][F], wh[NW][F];
+    C yh[NSTEP > 1 ? NSTEP - 1 : 1][3][F], yl[NSTEP > 1 ? NSTEP - 1 : 1][3][F];
+#pragma unroll
+    for (int m = 0; m <= PD + 1; ++m)
+      load(i_beg - NSTEP + m, w[m % NW], wl[m % NW], wh[m % NW]);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      if (i_beg - 2 * (NSTEP - 1) + t >= i_end) break;  // the last step is past the strip
+      {
+        const int m = t + 2 + PD;
+        load(i_beg - NSTEP + m, w[m % NW], wl[m % NW], wh[m % NW]);
+      }
+#pragma unroll
+      for (int s = 1; s <= NSTEP; ++s) {
+        if (t < 2 * (s - 1)) continue;  // step s has no needed row yet
+        const int k = t - (s - 1);
+        C nh[F], nl[F];
+        if (s == 1) {
+          step_row(k, w[t % NW], w[(t + 1) % NW], wl[(t + 1) % NW], w[(t + 2) % NW],
+                   wh[(t + 1) % NW], nh, nl);
+        } else {
+          // Step s-1 rows r-1, r, r+1 were produced at iterations t-2, t-1, t.
+          const int su = (t + 1) % 3, sc = (t + 2) % 3, sd = t % 3;
+          step_row(k, yh[s - 2][su], yh[s - 2][sc], yl[s - 2][sc], yh[s - 2][sd],
+                   yh[s - 2][sc], nh, nl);
+        }
+        if (s < NSTEP) {
+#pragma unroll
+          for (int f = 0; f < F; ++f) {
+            yh[s - 1][t % 3][f] = nh[f];
+            yl[s - 1][t % 3][f] = nl[f];
+          }
+        } else {
+          store_row(r_first + k, nh, nl);
+        }
       }
     }
   };
@@ -1366,7 +1585,7 @@ __global__ __launch_bounds__(NT) void fdm_fused(FdmArgs a) {
             } else if (Eq::kInvariant) {
               const C kk = kn[k][f];
               C d = kk;
-              if (Integ::kStages == 4) d = (((kk + C(2) * kk) + C(2) * kk) + kk) / C(6);
+              if (Integ::kStages == 4) d = (((kk + C(2) * kk) + C(2) * kk) + kk) * C(1.0 / 6.0);
               state_add<P>(yh[k][f], yl[k][f], d);
               nv = yh[k][f];
             } else if (Integ::kStages == 1) {
@@ -1392,7 +1611,7 @@ __global__ __launch_bounds__(NT) void fdm_fused(FdmArgs a) {
                 nv = state_plus<P>(yh[k][f], yl[k][f], kk);
               } else {
                 kacc[k][f] = kacc[k][f] + kk;
-                state_add<P>(yh[k][f], yl[k][f], kacc[k][f] / C(6));
+                state_add<P>(yh[k][f], yl[k][f], kacc[k][f] * C(1.0 / 6.0));
                 nv = yh[k][f];
               }
             }
@@ -1471,6 +1690,14 @@ struct Variant {
         &fdm_stream_rk<EQ, IN, PR, RW, PD, true>, &fdm_stream_rk<EQ, IN, PR, RW, PD, true>, \
         kStreamKind, 66 - 2 * IN::kStages, RW, 1, 256, 1, NAME                            \
   }
+#define PML_STREAMN(EQ, IN, PR, RW, PD, NS, NAME)                                         \
+  Variant {                                                                               \
+    &fdm_stream_n<EQ, IN, PR, RW, PD, NS, false>,                                         \
+        &fdm_stream_n<EQ, IN, PR, RW, PD, 1, false>,                                      \
+        &fdm_stream_n<EQ, IN, PR, RW, PD, NS, true>,                                      \
+        &fdm_stream_n<EQ, IN, PR, RW, PD, 1, true>, kStreamKind, 66 - 2 * NS, RW, NS, 256, 1, \
+        NAME                                                                              \
+  }
 #define PML_STREAM2(EQ, IN, PR, RW, PD, NAME)                                             \
   Variant {                                                                               \
     &fdm_stream2<EQ, IN, PR, RW, PD, 2, false>, &fdm_stream2<EQ, IN, PR, RW, PD, 1, false>, \
@@ -1494,6 +1721,11 @@ const std::vector<Variant> &variants_for() {
       out.push_back(PML_STREAM2(EQ, IN, PR, 8, 2, "stream2-rw8-pd2-s2"));
       out.push_back(PML_STREAM2(EQ, IN, PR, 16, 1, "stream2-rw16-pd1-s2"));
       out.push_back(PML_STREAM2(EQ, IN, PR, 4, 1, "stream2-rw4-pd1-s2"));
+      out.push_back(PML_STREAMN(EQ, IN, PR, 8, 2, 2, "streamn-rw8-pd2-s2"));
+      out.push_back(PML_STREAMN(EQ, IN, PR, 8, 2, 3, "streamn-rw8-pd2-s3"));
+      out.push_back(PML_STREAMN(EQ, IN, PR, 12, 2, 3, "streamn-rw12-pd2-s3"));
+      out.push_back(PML_STREAMN(EQ, IN, PR, 8, 1, 3, "streamn-rw8-pd1-s3"));
+      out.push_back(PML_STREAMN(EQ, IN, PR, 8, 2, 4, "streamn-rw8-pd2-s4"));
       if constexpr (wide) {
         out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
       } else {
