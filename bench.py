@@ -56,6 +56,8 @@ def parse_args():
     ap.add_argument("--pr-train-steps", type=int, default=3000)
     ap.add_argument("--pr-target", type=float, default=1e-6)
     ap.add_argument("--pr-fine-precision", default="float32c")
+    ap.add_argument("--pr-no-learned", action="store_true",
+                    help="skip the learned coarse operator (FDM coarse and serial only)")
     ap.add_argument("--tile-config", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
@@ -162,13 +164,7 @@ def parareal_workload(args):
         ForwardEulerMethod,
         ThreePointCentralDifferenceMethod,
     )
-    from pararealml_amd.operators.ml import (
-        ConvSurrogate,
-        CorrectedCoarseOperator,
-        corrected_pairs,
-        gaussian_mixture_states,
-        generate_pairs,
-    )
+    from pararealml_amd.operators.ml import gaussian_mixture_states
     from pararealml_amd.operators.parareal import CoarseGridFDMOperator, PararealOperator
 
     torch.cuda.set_device(0)
@@ -254,8 +250,80 @@ def parareal_workload(args):
     g_fdm = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), diff, dt_c))
     fdm_res = run_parareal(g_fdm, "FDM coarse")
 
-    # Learned coarse: a cheap FE solve on the 4x-coarsened grid corrected by
-    # one surrogate application per slice on the 2x grid.  Training data:
+    learned_res = plain_res = None
+    n_pairs = 0
+    t_data = t_train = 0.0
+    losses = [float("nan")]
+    g_learned = None
+    if not args.pr_no_learned:
+        learned_res, plain_res, n_pairs, t_data, t_train, losses, g_learned = _learned_coarse_runs(
+            args, cp, diff, n, S, g_fdm, run_parareal, log
+        )
+
+    value = learned_res["time_to_target_s"] if learned_res else None
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "s (learned-coarse Parareal time to target error)",
+        "n_gpus": 1,
+        "steps": S * args.pr_fine_steps,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": args.pr_fine_precision,
+        "data": "synthetic (seeded Gaussian-mixture initial conditions)",
+        "config": {
+            "workload": (
+                f"Parareal 2-D diffusion d=0.5, {n + 1}^2 vertices, Dirichlet 0, t in [0,1], "
+                f"{S} slices, fine RK4 {args.pr_fine_steps} steps/slice, FDM coarse FE "
+                f"{args.pr_coarse_steps} steps/slice on the 2x grid, learned coarse = FE "
+                f"{args.pr_base_steps} steps/slice on the {2 ** args.pr_base_levels}x grid + one conv-surrogate "
+                f"correction on the 2x grid"
+            ),
+            "target_rel_error": target,
+        },
+        "parareal": {
+            "serial_fine_fp64_s": t_ref,
+            "serial_fine_s": t_serial,
+            "serial_fine_error": err_serial,
+            "fdm_coarse": fdm_res,
+            "learned_coarse": learned_res,
+            "learned_fallbacks": g_learned.fallbacks if g_learned else None,
+            "cheap_coarse_without_network": plain_res,
+            "training": {
+                "trajectories": args.pr_traj,
+                "pairs": n_pairs,
+                "data_generation_s": t_data,
+                "train_steps": args.pr_train_steps,
+                "train_s": t_train,
+                "loss_first": float(np.mean(losses[:10])),
+                "loss_last": float(np.mean(losses[-10:])),
+            },
+        },
+        "host": host_info(),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def _learned_coarse_runs(args, cp, diff, n, S, g_fdm, run_parareal, log):
+    """The learned coarse operator's data generation, training and
+    Parareal runs (and the ablation without the network)."""
+    import torch
+
+    from pararealml_amd.operators.fdm import RK4, FDMOperator, ForwardEulerMethod
+    from pararealml_amd.operators.ml import (
+        ConvSurrogate,
+        CorrectedCoarseOperator,
+        corrected_pairs,
+        gaussian_mixture_states,
+        generate_pairs,
+    )
+    from pararealml_amd.operators.ml.learned_coarse import coarse_problem
+
+    # Learned coarse: a cheap large-step FE solve corrected by one surrogate
+    # application per slice on the 2x grid.  Training data:
     # the device FDM solver's trajectories from other seeded Gaussian
     # mixtures, one slice apart, restricted to the 2x grid.
     delta = 1.0 / S
@@ -292,52 +360,7 @@ def parareal_workload(args):
     g_plain = CorrectedCoarseOperator(base, ConvSurrogate(1, seed=0), fallback=g_fdm,
                                       base_levels=lv)
     plain_res = run_parareal(g_plain, "cheap coarse without the network")
-
-    value = learned_res["time_to_target_s"]
-    line = {
-        "metric": METRIC,
-        "value": value,
-        "unit": "s (learned-coarse Parareal time to target error)",
-        "n_gpus": 1,
-        "steps": S * args.pr_fine_steps,
-        "warmup": 0,
-        "ms_per_step": None,
-        "higher_is_better": False,
-        "scaling": "strong",
-        "vs_baseline": None,
-        "dtype": args.pr_fine_precision,
-        "data": "synthetic (seeded Gaussian-mixture initial conditions)",
-        "config": {
-            "workload": (
-                f"Parareal 2-D diffusion d=0.5, {n + 1}^2 vertices, Dirichlet 0, t in [0,1], "
-                f"{S} slices, fine RK4 {args.pr_fine_steps} steps/slice, FDM coarse FE "
-                f"{args.pr_coarse_steps} steps/slice on the 2x grid, learned coarse = FE "
-                f"{args.pr_base_steps} steps/slice on the {2 ** lv}x grid + one conv-surrogate "
-                f"correction on the 2x grid"
-            ),
-            "target_rel_error": target,
-        },
-        "parareal": {
-            "serial_fine_fp64_s": t_ref,
-            "serial_fine_s": t_serial,
-            "serial_fine_error": err_serial,
-            "fdm_coarse": fdm_res,
-            "learned_coarse": learned_res,
-            "learned_fallbacks": g_learned.fallbacks,
-            "cheap_coarse_without_network": plain_res,
-            "training": {
-                "trajectories": args.pr_traj,
-                "pairs": n_pairs,
-                "data_generation_s": t_data,
-                "train_steps": args.pr_train_steps,
-                "train_s": t_train,
-                "loss_first": float(np.mean(losses[:10])),
-                "loss_last": float(np.mean(losses[-10:])),
-            },
-        },
-        "host": host_info(),
-    }
-    print(json.dumps(line), flush=True)
+    return learned_res, plain_res, n_pairs, t_data, t_train, losses, g_learned
 
 
 def copy_workload(args):

@@ -1716,16 +1716,10 @@ const std::vector<Variant> &variants_for() {
       out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, 1, "stream-rw8-pd1-full"));
       out.push_back(PML_STREAM(EQ, IN, PR, 6, 3, true, 1, "stream-rw6-pd3-full"));
       out.push_back(PML_STREAM(EQ, IN, PR, 4, 1, true, 2, "stream-c2-rw4-pd1-full"));
-      out.push_back(PML_STREAM(EQ, IN, PR, 8, 1, true, 2, "stream-c2-rw8-pd1-full"));
       out.push_back(PML_STREAM2(EQ, IN, PR, 8, 1, "stream2-rw8-pd1-s2"));
       out.push_back(PML_STREAM2(EQ, IN, PR, 8, 2, "stream2-rw8-pd2-s2"));
-      out.push_back(PML_STREAM2(EQ, IN, PR, 16, 1, "stream2-rw16-pd1-s2"));
-      out.push_back(PML_STREAM2(EQ, IN, PR, 4, 1, "stream2-rw4-pd1-s2"));
-      out.push_back(PML_STREAMN(EQ, IN, PR, 8, 2, 2, "streamn-rw8-pd2-s2"));
       out.push_back(PML_STREAMN(EQ, IN, PR, 8, 2, 3, "streamn-rw8-pd2-s3"));
       out.push_back(PML_STREAMN(EQ, IN, PR, 12, 2, 3, "streamn-rw12-pd2-s3"));
-      out.push_back(PML_STREAMN(EQ, IN, PR, 8, 1, 3, "streamn-rw8-pd1-s3"));
-      out.push_back(PML_STREAMN(EQ, IN, PR, 8, 2, 4, "streamn-rw8-pd2-s4"));
       if constexpr (wide) {
         out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
       } else {
