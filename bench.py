@@ -42,7 +42,10 @@ def parse_args():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--workload", default="ch256", choices=["ch256", "parareal", "copy"])
+    ap.add_argument("--workload", default="ch256",
+                    choices=["ch256", "parareal", "copy", "surrogate"])
+    ap.add_argument("--sur-batch", type=int, default=32)
+    ap.add_argument("--sur-size", type=int, default=129, help="grid vertices per axis")
     # Parareal workload (2-D diffusion, learned vs FDM coarse)
     ap.add_argument("--pr-n", type=int, default=256, help="cells per axis (vertices n + 1)")
     ap.add_argument("--pr-slices", type=int, default=128)
@@ -278,9 +281,10 @@ def parareal_workload(args):
             "workload": (
                 f"Parareal 2-D diffusion d=0.5, {n + 1}^2 vertices, Dirichlet 0, t in [0,1], "
                 f"{S} slices, fine RK4 {args.pr_fine_steps} steps/slice, FDM coarse FE "
-                f"{args.pr_coarse_steps} steps/slice on the 2x grid, learned coarse = FE "
-                f"{args.pr_base_steps} steps/slice on the {2 ** args.pr_base_levels}x grid + one conv-surrogate "
-                f"correction on the 2x grid"
+                f"{args.pr_coarse_steps} steps/slice on the 2x grid"
+                + ("" if args.pr_no_learned else
+                   f", learned coarse = FE {args.pr_base_steps} steps/slice on the "
+                   f"{2 ** args.pr_base_levels}x grid + one conv-surrogate correction on the 2x grid")
             ),
             "target_rel_error": target,
         },
@@ -363,6 +367,57 @@ def _learned_coarse_runs(args, cp, diff, n, S, g_fdm, run_parareal, log):
     return learned_res, plain_res, n_pairs, t_data, t_train, losses, g_learned
 
 
+def surrogate_workload(args):
+    """Training throughput of the conv surrogate (BF16 MFMA kernels) on
+    synthetic smooth fields: batch x 1 x size x size, ``--steps`` Adam
+    steps of forward + backward.  FLOPs per pixel of one forward pass =
+    2 * 9 * sum over layers of C_in * C_out (297,216 for 1 -> 64 x 4 -> 1);
+    a training step counts 3 forward passes (forward, input gradient,
+    weight gradient)."""
+    import torch
+
+    from pararealml_amd.operators.ml import ConvSurrogate
+
+    torch.cuda.set_device(0)
+    b, n = args.sur_batch, args.sur_size
+    rng = np.random.RandomState(SEEDS[3])
+    g = np.linspace(0.0, 1.0, n)
+    x0, x1 = np.meshgrid(g, g, indexing="ij")
+    u = np.zeros((b, 1, n, n))
+    for k in range(b):
+        m = rng.uniform(0.2, 0.8, 2)
+        u[k, 0] = np.exp(-((x0 - m[0]) ** 2 + (x1 - m[1]) ** 2) / (2 * rng.uniform(0.005, 0.02)))
+    ut = torch.tensor(u, dtype=torch.float32, device="cuda")
+    tt = (0.97 * ut).contiguous()
+    net = ConvSurrogate(1, seed=0)
+    for _ in range(max(1, args.warmup)):
+        net.loss_and_grads(ut, tt)
+        net.adam_step(1e-4)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        net.loss_and_grads(ut, tt)
+        net.adam_step(1e-4)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    step_s = e0.elapsed_time(e1) / 1e3 / args.steps
+    fwd_flop = 2 * 9 * (1 * 64 + 4 * 64 * 64 + 64 * 1)
+    flops = 3 * fwd_flop * b * n * n
+    print(json.dumps({
+        "workload": "surrogate training",
+        "config": {"batch": b, "size": n, "layers": "1->64->64->64->64->64->1, 3x3, GELU"},
+        "steps": args.steps, "ms_per_step": step_s * 1e3, "wall_s": wall,
+        "training_TFLOPs": flops / step_s / 1e12,
+        "mfma_peak_TFLOPs_bf16_dense": 2500.0,
+        "frac_of_peak": flops / step_s / 1e12 / 2500.0,
+        "conv64_forward_flop_per_launch": 2 * 9 * 64 * 64 * b * n * n,
+    }), flush=True)
+
+
 def copy_workload(args):
     """A plain device-to-device copy of 1 GiB, repeated: the calibration
     reference for the FETCH_SIZE / WRITE_SIZE counters (each dispatch moves
@@ -404,6 +459,9 @@ def main():
         return
     if args.workload == "copy":
         copy_workload(args)
+        return
+    if args.workload == "surrogate":
+        surrogate_workload(args)
         return
 
     import torch
