@@ -216,7 +216,8 @@ int pml_prolong2x(int precision, const void *coarse, void *fine,
  *                           layer, wk = [9][64 ci][64 co] with the taps
  *                           flipped, zprev = the input's pre-activation;
  *                           writes a_out = dL/dz of the previous layer.
- *   pml_wgrad64         : dw[9][64 ci][64 co] += weight gradient (FP32).
+ *   pml_wgrad64         : dw[9][64 ci][64 co] += weight gradient and, when
+ *                         db is not NULL, db[64] += bias gradient (FP32).
  *   pml_conv_small      : C -> 64 (C in 1..3) or 64 -> C layers, w =
  *                         [9][ci][co]; forward writes z_out/a_out (64
  *                         outputs) or y_out (FP32, C outputs); mode 1 is the
@@ -239,8 +240,8 @@ int pml_prolong2x(int precision, const void *coarse, void *fine,
 int pml_conv64(int mode, const void *in, const void *wk, const float *bias,
                const void *zprev, void *z_out, void *a_out, int B, int H, int W,
                void *stream);
-int pml_wgrad64(const void *x, const void *dz, float *dw, int B, int H, int W,
-                void *stream);
+int pml_wgrad64(const void *x, const void *dz, float *dw, float *db, int B, int H,
+                int W, void *stream);
 int pml_conv_small(int mode, const void *in, const void *w, const float *bias,
                    const void *zprev, void *z_out, void *a_out, float *y_out,
                    int Ci, int Co, int B, int H, int W, void *stream);

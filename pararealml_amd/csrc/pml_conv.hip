@@ -11,14 +11,16 @@
 //    GELU, BF16 pre-activation and activation) and the input gradient
 //    (weights pre-arranged flipped and transposed; epilogue: times GELU'
 //    of the previous layer's pre-activation).
-//  * wgrad64: the 64 -> 64 weight gradient on the matrix cores, M = input
+//  * wgrad64: the 64 -> 64 weight (and bias) gradient on the matrix cores, M = input
 //    channels, N = output channels, K = pixels, one wavefront per tap,
 //    accumulated over a persistent pixel-tile loop and added to an FP32
 //    gradient buffer.
-//  * small layers (C_in -> 64 and 64 -> C_out with C <= 4): VALU kernels.
+//  * small layers (C_in -> 64 and 64 -> C_out with C <= 3): VALU kernels with
+//    vectorised 64-channel rows (a lane group per pixel).
 //  * Adam on FP32 masters, writing the BF16 weight layouts the kernels read.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -27,29 +29,59 @@
 namespace pml {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16;
 
-constexpr float kGeluC = 0.7978845608028654f;  // sqrt(2 / pi)
+// GELU, tanh form: 0.5 z (1 + tanh(u)), u = sqrt(2/pi) (z + 0.044715 z^3).
+// Evaluated as z * sigmoid(2u) (0.5 (1 + tanh(u)) = sigmoid(2u)) with one
+// hardware exp2 and one reciprocal: 2^(z (kGeluA + kGeluB z^2)) = exp(-2u).
+constexpr float kGeluA = -2.3022081983f;   // -2 sqrt(2/pi) log2(e)
+constexpr float kGeluB = -0.1029432396f;   // kGeluA * 0.044715
+constexpr float kGeluC = 1.5957691216f;    // 2 sqrt(2/pi)
+constexpr float kGeluD = 0.2140644488f;    // kGeluC * 3 * 0.044715
 
-__device__ __forceinline__ float gelu_f(float z) {
-  return 0.5f * z * (1.0f + tanhf(kGeluC * (z + 0.044715f * z * z * z)));
+__device__ __forceinline__ float gelu_sig(float z) {
+  const float e = __builtin_amdgcn_exp2f(z * (kGeluA + kGeluB * (z * z)));
+  return __builtin_amdgcn_rcpf(1.0f + e);
 }
+__device__ __forceinline__ float gelu_f(float z) { return z * gelu_sig(z); }
+// d/dz [z s(z)] = s + z s (1 - s) 2 u'(z), 2 u' = kGeluC + kGeluD z^2.
 __device__ __forceinline__ float gelu_grad_f(float z) {
-  const float u = kGeluC * (z + 0.044715f * z * z * z);
-  const float t = tanhf(u);
-  const float du = kGeluC * (1.0f + 3.0f * 0.044715f * z * z);
-  return 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * du;
+  const float s = gelu_sig(z);
+  return s + z * s * (1.0f - s) * (kGeluC + kGeluD * (z * z));
 }
 
 // ---------------------------------------------------------------------------
-// conv64 (MFMA).  Tile: TR = 8 rows x TC = 32 columns of output pixels.
-constexpr int kTR = 8, kTC = 32;
+// conv64 (MFMA).  Output tile: kTR = 16 rows x kTC = 32 columns x 64
+// channels.  The input patch (18 x 34 pixels) sits in LDS as 144-byte rows
+// (64 BF16 channels + 16 bytes of padding: 16 lanes reading consecutive
+// rows hit all 64 banks once, and every patch address is a per-lane base
+// plus an immediate).  All nine taps of the weights sit beside it as
+// unpadded 128-byte rows whose 16-byte chunks are XOR-swizzled by
+// (row / 2) mod 8, which spreads any 16 consecutive rows over the 64 banks.
+// The MFMA's A operand is a weight block (32 output channels x 16 input
+// channels), B a pixel block (16 input channels x 32 pixels of one row).
+// Wavefront w owns output rows 4 (w / 2) .. + 3 and output channels
+// 32 (w % 2) .. + 31: for each tap column and 16-channel slice it reads the
+// six patch rows and the three kernel-row weight blocks those four rows need
+// once and issues 12 MFMAs.  The next tile's patch is loaded into registers
+// while the current one is computed.
+constexpr int kTR = 16, kTC = 32;
 constexpr int kPR = kTR + 2, kPC = kTC + 2;  // input patch incl. halo
-constexpr int kCP = 72;                      // LDS channel pitch (bf16)
-constexpr int kPatchElems = kPR * kPC * kCP;
-constexpr int kWElems = 9 * 64 * kCP;
-constexpr size_t kConvLds = (kPatchElems + kWElems) * sizeof(bf16);
+constexpr int kPatchPitch = 144;             // bytes per patch pixel
+constexpr int kPatchChunks = kPR * kPC * 8;  // 16-byte chunks
+constexpr int kPatchBytes = kPR * kPC * kPatchPitch;
+constexpr int kWBytes = 9 * 64 * 128;
+constexpr size_t kConvLds = kPatchBytes + kWBytes;
+constexpr int kConvThreads = 512;
+constexpr int kPrefetch = (kPatchChunks + kConvThreads - 1) / kConvThreads;
+static_assert(kConvLds <= 160 * 1024, "conv64 LDS budget");
+
+// Byte offset of 16-byte chunk `chunk` (0..7) of weight row `row`.
+__device__ __forceinline__ int wswz(int row, int chunk) {
+  return row * 128 + ((chunk ^ ((row >> 1) & 7)) << 4);
+}
 
 enum ConvMode { kFwd = 0, kBwd = 1 };
 
@@ -62,200 +94,373 @@ struct ConvArgs {
   bf16 *a_out;        // forward: GELU(z) / backward: gradient
   int B, H, W;
   int tiles_h, tiles_w, n_tiles;
+  unsigned bytes;     // bytes of each [B, H, W, 64] tensor (< 2^31)
 };
 
+// Raw buffer resources over the activation tensors: loads beyond `bytes`
+// return 0 and stores beyond it are dropped, so halo pixels outside the
+// image and output pixels outside it need no branches.  kDrop is an offset
+// past every such buffer.
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+constexpr unsigned kDrop = 0x80000000u;
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t conv_rsrc(const void *p, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, static_cast<int>(bytes),
+                                           0x00020000);
+}
+
+struct TilePos {
+  int b, h0, w0;
+};
+__device__ __forceinline__ TilePos tile_pos(const ConvArgs &a, int tile) {
+  const int per_img = a.tiles_h * a.tiles_w;
+  const int b = tile / per_img;
+  const int rem = tile - b * per_img;
+  return {b, (rem / a.tiles_w) * kTR, (rem % a.tiles_w) * kTC};
+}
+
 template <int MODE>
-__global__ __launch_bounds__(512) void k_conv64(ConvArgs a) {
+__global__ __launch_bounds__(kConvThreads) void k_conv64(ConvArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16 *patch = reinterpret_cast<bf16 *>(smem);
-  bf16 *wl = patch + kPatchElems;
+  unsigned char *wl = smem;
+  unsigned char *patch = smem + kWBytes;
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
+  const int rg = wave >> 1, nh = wave & 1;
+  const int i = lane & 31, kh = lane >> 5;
+  const __amdgpu_buffer_rsrc_t r_in = conv_rsrc(a.in, a.bytes);
+  const __amdgpu_buffer_rsrc_t r_out = conv_rsrc(a.a_out, a.bytes);
+  const __amdgpu_buffer_rsrc_t r_z =
+      conv_rsrc(MODE == kFwd ? static_cast<const void *>(a.z_out) : a.zprev,
+                (MODE == kFwd ? a.z_out != nullptr : true) ? a.bytes : 0u);
 
-  // Weights into LDS once: 9 x 64 rows of 64 bf16 (8 chunks of 16 B).
-  for (int c = tid; c < 9 * 64 * 8; c += 512) {
-    const int row = c >> 3, part = c & 7;
-    const uint4 v = reinterpret_cast<const uint4 *>(a.wk)[row * 8 + part];
-    *reinterpret_cast<uint4 *>(wl + row * kCP + part * 8) = v;
+  // Weights into LDS once: 9 x 64 rows of 8 chunks.
+  for (int c = tid; c < 9 * 64 * 8; c += kConvThreads)
+    *reinterpret_cast<uint4 *>(wl + wswz(c >> 3, c & 7)) = reinterpret_cast<const uint4 *>(a.wk)[c];
+  // Accumulator element r = 4q + e of a lane is output channel
+  // 32 nh + 8q + 4kh + e of pixel column i; the forward accumulators start
+  // at the bias.
+  float bias[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    bias[r] = MODE == kFwd ? a.bias[nh * 32 + 8 * (r >> 2) + 4 * kh + (r & 3)] : 0.0f;
+    // Settle the load here: a load still pending at the tile loop's header
+    // would make every tile wait for all its own prefetches before its
+    // first MFMA.
+    asm volatile("" : "+v"(bias[r]));
   }
 
-  for (int tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
-    const int per_img = a.tiles_h * a.tiles_w;
-    const int b = tile / per_img;
-    const int rem = tile - b * per_img;
-    const int h0 = (rem / a.tiles_w) * kTR;
-    const int w0 = (rem % a.tiles_w) * kTC;
-    __syncthreads();  // previous tile's reads of the patch are done
-    for (int c = tid; c < kPR * kPC * 8; c += 512) {
-      const int px = c >> 3, part = c & 7;
-      const int pr = px / kPC, pc = px - (px / kPC) * kPC;
-      const int h = h0 + pr - 1, w = w0 + pc - 1;
-      uint4 v = make_uint4(0, 0, 0, 0);
-      if (h >= 0 && h < a.H && w >= 0 && w < a.W)
-        v = reinterpret_cast<const uint4 *>(
-            a.in + ((static_cast<int64_t>(b) * a.H + h) * a.W + w) * 64)[part];
-      *reinterpret_cast<uint4 *>(patch + (pr * kPC + pc) * kCP + part * 8) = v;
+  // Patch chunks this thread stages: pixel (pr, pc) of the patch, chunk
+  // `part`, at a fixed byte offset from the tile origin.
+  int rel[kPrefetch], prc[kPrefetch];
+#pragma unroll
+  for (int s = 0; s < kPrefetch; ++s) {
+    const int c = tid + s * kConvThreads;
+    const int px = c >> 3, part = c & 7;
+    const int pr = px / kPC, pc = px - pr * kPC;
+    rel[s] = ((pr - 1) * a.W + (pc - 1)) * 128 + part * 16;
+    prc[s] = c < kPatchChunks ? (pr << 8) | pc : -1;  // -1: beyond the patch
+  }
+  u32x4 pre[kPrefetch];
+  auto fetch = [&](int tile) {
+    const TilePos tp = tile_pos(a, tile);
+    const int origin = ((tp.b * a.H + tp.h0) * a.W + tp.w0) * 128;
+#pragma unroll
+    for (int s = 0; s < kPrefetch; ++s) {
+      const unsigned h = static_cast<unsigned>(tp.h0 + (prc[s] >> 8) - 1);
+      const unsigned w = static_cast<unsigned>(tp.w0 + (prc[s] & 0xff) - 1);
+      const bool ok = prc[s] >= 0 && h < static_cast<unsigned>(a.H) && w < static_cast<unsigned>(a.W);
+      pre[s] = __builtin_amdgcn_raw_buffer_load_b128(
+          r_in, ok ? static_cast<unsigned>(origin + rel[s]) : kDrop, 0, 0);
+    }
+  };
+
+  // Per-lane LDS bases: patch pixel (4 rg, i) chunk kh, and weight row
+  // 32 nh + i of tap 0 for each 16-channel slice kc.
+  const int pbase = ((4 * rg) * kPC + i) * kPatchPitch + kh * 16;
+  int wbase[4];
+#pragma unroll
+  for (int kc = 0; kc < 4; ++kc) wbase[kc] = wswz(nh * 32 + i, 2 * kc + kh);
+
+  int tile = blockIdx.x;
+  if (tile < a.n_tiles) fetch(tile);
+  for (; tile < a.n_tiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile's patch reads are done
+#pragma unroll
+    for (int s = 0; s < kPrefetch; ++s) {
+      const int c = tid + s * kConvThreads;
+      if (c < kPatchChunks)
+        *reinterpret_cast<u32x4 *>(patch + (c >> 3) * kPatchPitch + (c & 7) * 16) = pre[s];
     }
     __syncthreads();
+    const TilePos tp = tile_pos(a, tile);
+    if (tile + static_cast<int>(gridDim.x) < a.n_tiles) fetch(tile + gridDim.x);
 
-    f32x16 acc0, acc1;
+    // The two wavefronts sharing a SIMD (w and w + 4) run the MFMA phase
+    // at different priorities, so the first finishes its matrix work and
+    // runs its VALU epilogue while the second still issues MFMAs.
+    if (wave < 4) __builtin_amdgcn_s_setprio(2);
+    else __builtin_amdgcn_s_setprio(1);
+    f32x16 acc[4];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc0[r] = acc1[r] = 0.0f;
-    const int i = lane & 31;        // pixel column within the tile row (A row)
-    const int kh8 = (lane >> 5) * 8;  // k offset of this lane's 8 values
-#pragma unroll 1
-    for (int t = 0; t < 9; ++t) {
-      const int th = t / 3, tw = t - (t / 3) * 3;
-      const bf16 *arow = patch + ((wave + th) * kPC + (i + tw)) * kCP + kh8;
-      const bf16 *b0 = wl + (t * 64 + i) * kCP + kh8;
-      const bf16 *b1 = wl + (t * 64 + 32 + i) * kCP + kh8;
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-      for (int kc = 0; kc < 4; ++kc) {
-        const bf16x8 av = *reinterpret_cast<const bf16x8 *>(arow + kc * 16);
-        const bf16x8 bv0 = *reinterpret_cast<const bf16x8 *>(b0 + kc * 16);
-        const bf16x8 bv1 = *reinterpret_cast<const bf16x8 *>(b1 + kc * 16);
-        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv0, acc0, 0, 0, 0);
-        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv1, acc1, 0, 0, 0);
-      }
+      for (int r = 0; r < 16; ++r) acc[j][r] = bias[r];
+    // 12 steps (tap column tw, 16-channel slice kc); the fragments of step
+    // st + 1 are read while the MFMAs of step st run.
+    bf16x8 pv[2][6], wv[2][3];
+    auto load_step = [&](int st, bf16x8 (&p6)[6], bf16x8 (&w3)[3]) {
+      const int tw = st >> 2, kc = st & 3;
+#pragma unroll
+      for (int r = 0; r < 6; ++r)
+        p6[r] = *reinterpret_cast<const bf16x8 *>(patch + pbase + (r * kPC + tw) * kPatchPitch +
+                                                  kc * 32);
+#pragma unroll
+      for (int th = 0; th < 3; ++th)
+        w3[th] = *reinterpret_cast<const bf16x8 *>(wl + wbase[kc] + (th * 3 + tw) * 64 * 128);
+    };
+    load_step(0, pv[0], wv[0]);
+#pragma unroll
+    for (int st = 0; st < 12; ++st) {
+      const int cur = st & 1;
+      if (st + 1 < 12) load_step(st + 1, pv[cur ^ 1], wv[cur ^ 1]);
+      // Keep the reads ahead of this step's MFMAs (the scheduler would
+      // otherwise sink each read next to its first use).
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int th = 0; th < 3; ++th)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wv[cur][th], pv[cur][j + th], acc[j],
+                                                           0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
     }
+    __builtin_amdgcn_s_setprio(0);
 
-    // Epilogue.  Accumulator element r of lane l holds output channel
-    // n = l % 32 (+ 32 for acc1) of the tile-row pixel 8 (r / 4) + 4 (l / 32)
-    // + r % 4.
-    const int h = h0 + wave;
-    if (h < a.H) {
+    // Epilogue: 4 consecutive channels per 8-byte access; pixels outside
+    // the image use the drop offset.  The backward pass issues all its
+    // pre-activation loads before using any.
+    const int w = tp.w0 + i;
+    unsigned base[4];
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int px = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int w = w0 + px;
-        if (w >= a.W) continue;
-        const int64_t base = ((static_cast<int64_t>(b) * a.H + h) * a.W + w) * 64;
+    for (int j = 0; j < 4; ++j) {
+      const int h = tp.h0 + 4 * rg + j;
+      base[j] = h < a.H && w < a.W
+                    ? ((static_cast<unsigned>(tp.b * a.H + h) * a.W + w) * 64 + nh * 32 + 4 * kh) * 2
+                    : kDrop;
+    }
+    u32x2 zp[4][4];
+    if (MODE == kBwd) {
 #pragma unroll
-        for (int nb = 0; nb < 2; ++nb) {
-          const int n = nb * 32 + (lane & 31);
-          const float v = nb == 0 ? acc0[r] : acc1[r];
-          if (MODE == kFwd) {
-            const float z = v + a.bias[n];
-            const bf16 zb = static_cast<bf16>(z);
-            if (a.z_out) a.z_out[base + n] = zb;
-            a.a_out[base + n] = static_cast<bf16>(gelu_f(z));
-          } else {
-            const float zp = static_cast<float>(a.zprev[base + n]);
-            a.a_out[base + n] = static_cast<bf16>(v * gelu_grad_f(zp));
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          zp[j][q] = __builtin_amdgcn_raw_buffer_load_b64(r_z, base[j] + 16 * q, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        bf16x4 o4;
+        if (MODE == kFwd) {
+          bf16x4 z4;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const float z = acc[j][4 * q + e];
+            z4[e] = static_cast<bf16>(z);
+            o4[e] = static_cast<bf16>(gelu_f(z));
           }
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, z4), r_z,
+                                                base[j] + 16 * q, 0, 0);
+        } else {
+          const bf16x4 zq = __builtin_bit_cast(bf16x4, zp[j][q]);
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            o4[e] = static_cast<bf16>(acc[j][4 * q + e] * gelu_grad_f(static_cast<float>(zq[e])));
         }
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, o4), r_out,
+                                              base[j] + 16 * q, 0, 0);
       }
     }
   }
 }
 
 // ---------------------------------------------------------------------------
-// wgrad64 (MFMA).  Pixel tiles of 8 x 32; nine wavefronts, one per tap.
-constexpr int kGP = 40;  // LDS pitch (elements) of a transposed patch row
-constexpr int kGPatch = 64 * kPR * kGP;     // [ci][prow][pcol]
-constexpr int kGdz = 64 * kTR * kTC;        // [co][tile pixel]
-constexpr size_t kWgradLds = (kGPatch + kGdz) * sizeof(bf16);
+// wgrad64 (MFMA): dw[t][ci][co] += sum_p x[p + off_t][ci] dz[p][co] and,
+// optionally, db[co] += sum_p dz[p][co].  M = input channels, N = output
+// channels, K = pixels.  Pixel tiles of kGR = 8 rows x kGC = 32 columns;
+// the MFMA's K = 16 covers two pixel columns of 8 rows.  The input patch
+// (10 x 34 pixels) and the dz tile are staged channel-major in LDS: a
+// 16-byte chunk holds one channel at 8 consecutive rows of one column, and
+// the patch is kept as three copies shifted by the kernel row (copy g holds
+// patch rows g .. g + 7), so every MFMA operand is one aligned 16-byte read.
+// Channel rows are padded to an odd number of chunks, which makes 16
+// consecutive channels fall into distinct banks.  The transposition is done
+// in registers while staging: a thread loads the 16-byte chunks (8 channels)
+// of 4 consecutive rows and writes 8 channel runs of 4 rows.
+// Twelve wavefronts: wave w owns the kernel row g = w / 4, input channels
+// 32 (w % 2) .. + 31, output channels 32 ((w / 2) % 2) .. + 31 and the
+// three taps of that row; the two wavefronts with g = 0 and w % 2 = 0 also
+// sum their dz operands (VALU) for the bias gradient.
+constexpr int kGR = 8, kGC = 32;
+constexpr int kGXC = kGC + 2;                 // patch columns
+constexpr int kGXPitch = 35 * 16;             // bytes per channel of a shifted patch copy
+constexpr int kGDPitch = 33 * 16;             // bytes per channel of the dz tile
+constexpr int kGXBytes = 64 * kGXPitch;
+constexpr int kGDBytes = 64 * kGDPitch;
+constexpr size_t kWgradLds = 3 * kGXBytes + kGDBytes;
+constexpr int kWgradThreads = 768;
+constexpr int kGXTasks = 3 * 2 * 8 * kGXC;    // (copy, row group, chunk, column)
+constexpr int kGXWaves = (kGXTasks + 63) / 64;
+constexpr int kGDTasks = 2 * 8 * kGC;         // (row group, chunk, column)
+constexpr int kGDWaves = kGDTasks / 64;
+constexpr int kGSlots = (kGXWaves + kGDWaves + 11) / 12;
+static_assert(kWgradLds <= 160 * 1024, "wgrad64 LDS budget");
+static_assert(kGDTasks % 64 == 0, "dz staging fills whole wavefronts");
 
 struct WgradArgs {
   const bf16 *x;   // [B, H, W, 64] layer input
   const bf16 *dz;  // [B, H, W, 64] gradient of the pre-activation
   float *dw;       // [9][64 ci][64 co], accumulated
+  float *db;       // [64], accumulated (may be null)
   int B, H, W;
   int tiles_h, tiles_w, n_tiles;
+  unsigned bytes;  // bytes of each [B, H, W, 64] tensor (< 2^31)
 };
 
-__global__ __launch_bounds__(576) void k_wgrad64(WgradArgs a) {
+__global__ __launch_bounds__(kWgradThreads) void k_wgrad64(WgradArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
-  bf16 *pt = reinterpret_cast<bf16 *>(smem);
-  bf16 *dzt = pt + kGPatch;
   const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;  // wave = tap
-  const int th = wave / 3, tw = wave - (wave / 3) * 3;
-  f32x16 acc[2][2];
-#pragma unroll
-  for (int m = 0; m < 2; ++m)
-#pragma unroll
-    for (int n = 0; n < 2; ++n)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) acc[m][n][r] = 0.0f;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int i = lane & 31, kh = lane >> 5;
+  const __amdgpu_buffer_rsrc_t r_x = conv_rsrc(a.x, a.bytes);
+  const __amdgpu_buffer_rsrc_t r_dz = conv_rsrc(a.dz, a.bytes);
 
-  for (int tile = blockIdx.x; tile < a.n_tiles; tile += gridDim.x) {
-    const int per_img = a.tiles_h * a.tiles_w;
-    const int b = tile / per_img;
-    const int rem = tile - b * per_img;
-    const int h0 = (rem / a.tiles_w) * kTR;
-    const int w0 = (rem % a.tiles_w) * kTC;
-    __syncthreads();
-    // Transposed staging: patch[ci][pr][pc] and dz[co][tile pixel].
-    for (int c = tid; c < kPR * kPC * 8; c += 576) {
-      const int px = c >> 3, part = c & 7;
-      const int pr = px / kPC, pc = px - (px / kPC) * kPC;
-      const int h = h0 + pr - 1, w = w0 + pc - 1;
-      bf16x8 v;
+  // Staging slots: wave-task s * 12 + wave; the first kGXWaves move patch
+  // chunks, the next kGDWaves dz chunks.  Per slot: row/column offset of
+  // the task's first row from the tile origin, and its LDS byte address.
+  int sh[kGSlots], sw[kGSlots], sl[kGSlots];
 #pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = static_cast<bf16>(0.0f);
-      if (h >= 0 && h < a.H && w >= 0 && w < a.W)
-        v = reinterpret_cast<const bf16x8 *>(
-            a.x + ((static_cast<int64_t>(b) * a.H + h) * a.W + w) * 64)[part];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) pt[((part * 8 + e) * kPR + pr) * kGP + pc] = v[e];
-    }
-    for (int c = tid; c < kTR * kTC * 8; c += 576) {
-      const int px = c >> 3, part = c & 7;
-      const int pr = px / kTC, pc = px - (px / kTC) * kTC;
-      const int h = h0 + pr, w = w0 + pc;
-      bf16x8 v;
-#pragma unroll
-      for (int e = 0; e < 8; ++e) v[e] = static_cast<bf16>(0.0f);
-      if (h < a.H && w < a.W)
-        v = reinterpret_cast<const bf16x8 *>(
-            a.dz + ((static_cast<int64_t>(b) * a.H + h) * a.W + w) * 64)[part];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) dzt[(part * 8 + e) * (kTR * kTC) + px] = v[e];
-    }
-    __syncthreads();
-    // K = the tile's 256 pixels in steps of 16 (half a tile row).
-    const int i = lane & 31, k8 = (lane >> 5) * 8;
-#pragma unroll 1
-    for (int ks = 0; ks < kTR * kTC / 16; ++ks) {
-      const int prow = ks >> 1, pcol0 = (ks & 1) * 16 + k8;
-      // A: rows = input channels, 8 consecutive pixels of the shifted patch.
-      bf16x8 av[2];
-#pragma unroll
-      for (int m = 0; m < 2; ++m) {
-        const bf16 *src = pt + ((m * 32 + i) * kPR + prow + th) * kGP + pcol0 + tw;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) av[m][e] = src[e];
+  for (int s = 0; s < kGSlots; ++s) {
+    const int wt = s * 12 + wave;
+    sh[s] = -1000000;  // no task: never inside the image
+    sw[s] = 0;
+    sl[s] = 0;
+    if (wt < kGXWaves) {
+      const int t = wt * 64 + lane;
+      if (t < kGXTasks) {
+        const int c = t % kGXC, r1 = t / kGXC;
+        const int part = r1 & 7, kk = (r1 >> 3) & 1, g = r1 >> 4;
+        sh[s] = g + 4 * kk - 1;
+        sw[s] = c - 1;
+        sl[s] = g * kGXBytes + part * 8 * kGXPitch + c * 16 + kk * 8;
+        sw[s] = (sw[s] << 4) | part;
       }
-      // B: output channels, 8 consecutive tile pixels.
-      bf16x8 bv[2];
-#pragma unroll
-      for (int n = 0; n < 2; ++n)
-        bv[n] = *reinterpret_cast<const bf16x8 *>(dzt + (n * 32 + i) * (kTR * kTC) +
-                                                  prow * kTC + pcol0);
-#pragma unroll
-      for (int m = 0; m < 2; ++m)
-#pragma unroll
-        for (int n = 0; n < 2; ++n)
-          acc[m][n] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[m], bv[n], acc[m][n], 0, 0, 0);
+    } else if (wt < kGXWaves + kGDWaves) {
+      const int t = (wt - kGXWaves) * 64 + lane;
+      const int c = t % kGC, r1 = t / kGC;
+      const int part = r1 & 7, kk = r1 >> 3;
+      sh[s] = 4 * kk;
+      sw[s] = (c << 4) | part;
+      sl[s] = 3 * kGXBytes + part * 8 * kGDPitch + c * 16 + kk * 8;
     }
   }
-  // dw[tap][ci][co] += acc: element r of lane l is (ci row 8 (r/4) + 4 (l/32)
-  // + r%4, co column l%32) within each 32x32 block.
+  u32x4 pre[kGSlots][4];
+  auto fetch = [&](int tile) {
+    const int per_img = a.tiles_h * a.tiles_w;
+    const int b = tile / per_img, rem = tile - b * per_img;
+    const int h0 = (rem / a.tiles_w) * kGR, w0 = (rem % a.tiles_w) * kGC;
 #pragma unroll
-  for (int m = 0; m < 2; ++m)
+    for (int s = 0; s < kGSlots; ++s) {
+      const bool from_x = s * 12 + wave < kGXWaves;  // wave-uniform
+      const int w = w0 + (sw[s] >> 4), part = sw[s] & 15;
 #pragma unroll
-    for (int n = 0; n < 2; ++n)
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int ci = m * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int co = n * 32 + (lane & 31);
-        atomicAdd(&a.dw[(wave * 64 + ci) * 64 + co], acc[m][n][r]);
+      for (int k = 0; k < 4; ++k) {
+        const int h = h0 + sh[s] + k;
+        const bool ok = static_cast<unsigned>(h) < static_cast<unsigned>(a.H) &&
+                        static_cast<unsigned>(w) < static_cast<unsigned>(a.W);
+        const unsigned off =
+            ok ? (static_cast<unsigned>((b * a.H + h) * a.W + w) * 64 + part * 8) * 2 : kDrop;
+        pre[s][k] = __builtin_amdgcn_raw_buffer_load_b128(from_x ? r_x : r_dz, off, 0, 0);
       }
+    }
+  };
+
+  const int g = wave >> 2, m = wave & 1, n = (wave >> 1) & 1;
+  const bool bias_wave = g == 0 && m == 0 && a.db != nullptr;
+  const int abase = g * kGXBytes + (m * 32 + i) * kGXPitch + kh * 16;
+  const int bbase = 3 * kGXBytes + (n * 32 + i) * kGDPitch + kh * 16;
+  f32x16 acc[3];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[0][r] = acc[1][r] = acc[2][r] = 0.0f;
+  float bsum = 0.0f;  // dz sum of output channel 32 n + i (bias wavefronts)
+
+  int tile = blockIdx.x;
+  if (tile < a.n_tiles) fetch(tile);
+  for (; tile < a.n_tiles; tile += gridDim.x) {
+    __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+    for (int s = 0; s < kGSlots; ++s) {
+      if (sh[s] > -1000000) {
+        const int pitch = s * 12 + wave < kGXWaves ? kGXPitch : kGDPitch;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+          const int d = e >> 1;
+          u32x2 v;
+          if (e & 1) {
+            v[0] = (pre[s][0][d] >> 16) | (pre[s][1][d] & 0xffff0000u);
+            v[1] = (pre[s][2][d] >> 16) | (pre[s][3][d] & 0xffff0000u);
+          } else {
+            v[0] = (pre[s][0][d] & 0xffffu) | (pre[s][1][d] << 16);
+            v[1] = (pre[s][2][d] & 0xffffu) | (pre[s][3][d] << 16);
+          }
+          *reinterpret_cast<u32x2 *>(smem + sl[s] + e * pitch) = v;
+        }
+      }
+    }
+    __syncthreads();
+    if (tile + static_cast<int>(gridDim.x) < a.n_tiles) fetch(tile + gridDim.x);
+
+    // 16 K-steps of two pixel columns; the three wavefronts of each SIMD
+    // hide one another's LDS latency.
+#pragma unroll
+    for (int st = 0; st < kGC / 2; ++st) {
+      bf16x8 av[3];
+#pragma unroll
+      for (int tw = 0; tw < 3; ++tw)
+        av[tw] = *reinterpret_cast<const bf16x8 *>(smem + abase + (2 * st + tw) * 16);
+      const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(smem + bbase + 2 * st * 16);
+#pragma unroll
+      for (int tw = 0; tw < 3; ++tw)
+        acc[tw] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av[tw], bv, acc[tw], 0, 0, 0);
+      if (bias_wave) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) bsum += static_cast<float>(bv[e]);
+      }
+    }
+  }
+  // dw[g * 3 + tw][ci][co] += acc[tw]: element r of lane l is
+  // (ci = 32 m + 8 (r / 4) + 4 (l / 32) + r % 4, co = 32 n + l % 32).
+#pragma unroll
+  for (int tw = 0; tw < 3; ++tw)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int ci = m * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
+      atomicAdd(&a.dw[((g * 3 + tw) * 64 + ci) * 64 + n * 32 + i], acc[tw][r]);
+    }
+  // Lanes i and i + 32 summed the two column halves of channel 32 n + i.
+  if (bias_wave) {
+    bsum += __shfl_xor(bsum, 32);
+    if (kh == 0) atomicAdd(&a.db[n * 32 + i], bsum);
+  }
 }
 
 // ---------------------------------------------------------------------------
-// Small layers (VALU).
+// Small layers (VALU).  NHWC rows of 64 BF16 channels are 128 bytes; every
+// kernel gives a pixel a group of lanes that each own a contiguous run of
+// channels, so the 64-channel traffic moves as 16-byte (8 channels) or
+// 8-byte (4 channels) vector accesses and a wavefront covers 8 or 4
+// neighbouring pixels.  Weights live in LDS as FP32.
 struct SmallArgs {
   const bf16 *in;      // [B, H, W, Ci]
   const bf16 *w;       // [9][Ci][Co]
@@ -267,167 +472,309 @@ struct SmallArgs {
   int B, H, W, Ci, Co;
 };
 
-// Forward, one thread per output pixel, all Co outputs in registers.
-template <int CI, int CO>
-__global__ __launch_bounds__(256) void k_small_fwd(SmallArgs a) {
-  const int64_t P = static_cast<int64_t>(a.B) * a.H * a.W;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < P;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const int w = p % a.W;
-    const int h = (p / a.W) % a.H;
-    const int64_t b = p / (static_cast<int64_t>(a.W) * a.H);
-    float acc[CO];
+// Flat index of the pixel at offset (dh, dw) from p and whether it lies in
+// the image; an outside neighbour is replaced by p itself, so the caller
+// can load unconditionally and select zero.
+__device__ __forceinline__ int tap_pixel(int p, int h, int w, int dh, int dw, int H, int W,
+                                         bool &ok) {
+  const int hh = h + dh, ww = w + dw;
+  ok = hh >= 0 && hh < H && ww >= 0 && ww < W;
+  return ok ? p + dh * W + dw : p;
+}
+
+template <int N>
+__device__ __forceinline__ void lds_fill(float *dst, const bf16 *src) {
+  for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = static_cast<float>(src[i]);
+}
+
+// Forward of a CI -> 64 layer with GELU: 8 lanes per pixel, 8 output
+// channels per lane.
+template <int CI>
+__global__ __launch_bounds__(256) void k_in_fwd(SmallArgs a) {
+  __shared__ float wl[9 * CI * 64];
+  __shared__ float bl[64];
+  lds_fill<9 * CI * 64>(wl, a.w);
+  if (threadIdx.x < 64) bl[threadIdx.x] = a.bias[threadIdx.x];
+  __syncthreads();
+  const int j = threadIdx.x & 7;
+  const int P = a.B * a.H * a.W;
+  for (int p = (blockIdx.x * 256 + threadIdx.x) >> 3; p < P; p += gridDim.x * 32) {
+    // With more than one field the weights would not fit in registers:
+    // re-read them from LDS each pixel instead of hoisting them.
+    if (CI > 1) asm volatile("" ::: "memory");
+    const int w = p % a.W, h = (p / a.W) % a.H;
+    float acc[8];
 #pragma unroll
-    for (int o = 0; o < CO; ++o) acc[o] = a.bias[o];
+    for (int k = 0; k < 8; ++k) acc[k] = bl[8 * j + k];
+#pragma unroll
     for (int t = 0; t < 9; ++t) {
-      const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
-      if (hh < 0 || hh >= a.H || ww < 0 || ww >= a.W) continue;
-      const bf16 *src = a.in + ((b * a.H + hh) * a.W + ww) * CI;
-      const bf16 *wt = a.w + t * CI * CO;
-#pragma unroll 4
-      for (int c = 0; c < CI; ++c) {
-        const float xv = static_cast<float>(src[c]);
+      bool ok;
+      const int q = tap_pixel(p, h, w, t / 3 - 1, t % 3 - 1, a.H, a.W, ok);
 #pragma unroll
-        for (int o = 0; o < CO; ++o) acc[o] += xv * static_cast<float>(wt[c * CO + o]);
+      for (int c = 0; c < CI; ++c) {
+        const float xv = ok ? static_cast<float>(a.in[static_cast<int64_t>(q) * CI + c]) : 0.0f;
+        const float *wt = wl + (t * CI + c) * 64 + 8 * j;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] += xv * wt[k];
       }
     }
-    if (CO == 64) {
+    bf16x8 zv, av;
 #pragma unroll
-      for (int o = 0; o < CO; ++o) {
-        const bf16 zb = static_cast<bf16>(acc[o]);
-        if (a.z_out) a.z_out[p * CO + o] = zb;
-        a.a_out[p * CO + o] = static_cast<bf16>(gelu_f(acc[o]));
+    for (int k = 0; k < 8; ++k) {
+      zv[k] = static_cast<bf16>(acc[k]);
+      av[k] = static_cast<bf16>(gelu_f(acc[k]));
+    }
+    const int64_t o = static_cast<int64_t>(p) * 64 + 8 * j;
+    if (a.z_out) *reinterpret_cast<bf16x8 *>(a.z_out + o) = zv;
+    *reinterpret_cast<bf16x8 *>(a.a_out + o) = av;
+  }
+}
+
+// Forward of a 64 -> CO layer (no activation, FP32 output): 8 lanes per
+// pixel, each summing its 8 input channels over the 9 taps; the partial
+// sums meet through lane shuffles.
+template <int CO>
+__global__ __launch_bounds__(256) void k_out_fwd(SmallArgs a) {
+  __shared__ float wl[9 * 64 * CO];
+  lds_fill<9 * 64 * CO>(wl, a.w);
+  __syncthreads();
+  const int j = threadIdx.x & 7;
+  const int P = a.B * a.H * a.W;
+  for (int p = (blockIdx.x * 256 + threadIdx.x) >> 3; p < P; p += gridDim.x * 32) {
+    // With more than one field the weights would not fit in registers:
+    // re-read them from LDS each pixel instead of hoisting them.
+    if (CO > 1) asm volatile("" ::: "memory");
+    const int w = p % a.W, h = (p / a.W) % a.H;
+    float acc[CO];
+#pragma unroll
+    for (int o = 0; o < CO; ++o) acc[o] = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+      bool ok;
+      const int q = tap_pixel(p, h, w, t / 3 - 1, t % 3 - 1, a.H, a.W, ok);
+      const bf16x8 v = *reinterpret_cast<const bf16x8 *>(a.in + static_cast<int64_t>(q) * 64 + 8 * j);
+      const float *wt = wl + (t * 64 + 8 * j) * CO;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const float xv = ok ? static_cast<float>(v[k]) : 0.0f;
+#pragma unroll
+        for (int o = 0; o < CO; ++o) acc[o] += xv * wt[k * CO + o];
       }
-    } else {
+    }
 #pragma unroll
-      for (int o = 0; o < CO; ++o) a.y_out[p * CO + o] = acc[o];
+    for (int o = 0; o < CO; ++o) {
+      acc[o] += __shfl_xor(acc[o], 1);
+      acc[o] += __shfl_xor(acc[o], 2);
+      acc[o] += __shfl_xor(acc[o], 4);
+    }
+    if (j == 0) {
+#pragma unroll
+      for (int o = 0; o < CO; ++o) a.y_out[static_cast<int64_t>(p) * CO + o] = acc[o] + a.bias[o];
     }
   }
 }
 
 // Input gradient of a 64 -> CO layer, times GELU' of the input's
-// pre-activation; one thread per pixel, 64 outputs.
+// pre-activation: 8 lanes per pixel, 8 channels per lane.
 template <int CO>
-__global__ __launch_bounds__(256) void k_small_bwd(SmallArgs a) {
-  const int64_t P = static_cast<int64_t>(a.B) * a.H * a.W;
-  for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < P;
-       p += (int64_t)gridDim.x * blockDim.x) {
-    const int w = p % a.W;
-    const int h = (p / a.W) % a.H;
-    const int64_t b = p / (static_cast<int64_t>(a.W) * a.H);
-    float acc[64];
+__global__ __launch_bounds__(256) void k_out_bwd(SmallArgs a) {
+  __shared__ float wl[9 * 64 * CO];
+  lds_fill<9 * 64 * CO>(wl, a.w);
+  __syncthreads();
+  const int j = threadIdx.x & 7;
+  const int P = a.B * a.H * a.W;
+  for (int p = (blockIdx.x * 256 + threadIdx.x) >> 3; p < P; p += gridDim.x * 32) {
+    // With more than one field the weights would not fit in registers:
+    // re-read them from LDS each pixel instead of hoisting them.
+    if (CO > 1) asm volatile("" ::: "memory");
+    const int w = p % a.W, h = (p / a.W) % a.H;
+    float acc[8];
 #pragma unroll
-    for (int c = 0; c < 64; ++c) acc[c] = 0.0f;
+    for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+#pragma unroll
     for (int t = 0; t < 9; ++t) {
       // input (h, w) feeds output (h - th + 1, w - tw + 1)
-      const int hh = h - t / 3 + 1, ww = w - t % 3 + 1;
-      if (hh < 0 || hh >= a.H || ww < 0 || ww >= a.W) continue;
-      const bf16 *g = a.in + ((b * a.H + hh) * a.W + ww) * CO;
-      const bf16 *wt = a.w + t * 64 * CO;
+      bool ok;
+      const int q = tap_pixel(p, h, w, 1 - t / 3, 1 - t % 3, a.H, a.W, ok);
+      const float *wt = wl + (t * 64 + 8 * j) * CO;
 #pragma unroll
       for (int o = 0; o < CO; ++o) {
-        const float gv = static_cast<float>(g[o]);
+        const float gv = ok ? static_cast<float>(a.in[static_cast<int64_t>(q) * CO + o]) : 0.0f;
 #pragma unroll
-        for (int c = 0; c < 64; ++c) acc[c] += gv * static_cast<float>(wt[c * CO + o]);
+        for (int k = 0; k < 8; ++k) acc[k] += gv * wt[k * CO + o];
       }
     }
+    const int64_t e = static_cast<int64_t>(p) * 64 + 8 * j;
+    const bf16x8 zp = *reinterpret_cast<const bf16x8 *>(a.zprev + e);
+    bf16x8 dv;
 #pragma unroll
-    for (int c = 0; c < 64; ++c) {
-      const float zp = static_cast<float>(a.zprev[p * 64 + c]);
-      a.a_out[p * 64 + c] = static_cast<bf16>(acc[c] * gelu_grad_f(zp));
-    }
+    for (int k = 0; k < 8; ++k) dv[k] = static_cast<bf16>(acc[k] * gelu_grad_f(static_cast<float>(zp[k])));
+    *reinterpret_cast<bf16x8 *>(a.a_out + e) = dv;
   }
 }
 
-// Weight and bias gradients of a small layer: lane = the 64-wide channel
-// (output channel for CI -> 64, input channel for 64 -> CO); block-level
-// partial sums over a strided pixel range, then FP32 atomics.
-template <int CI, int CO>
-__global__ __launch_bounds__(256) void k_small_wgrad(const bf16 *x, const bf16 *dz, float *dw,
-                                                     float *db, int B, int H, int W) {
-  constexpr bool kIn64 = CI == 64;  // 64 -> CO layer
-  constexpr int kOther = kIn64 ? CO : CI;
+// Sums NV per-lane partials over the four pixel slots of each wavefront
+// (lanes l, l + 16, l + 32, l + 48 own the same channels) and the four
+// wavefronts of the block; returns with red[i][l] (l < 16) holding the
+// block's totals.
+template <int NV>
+__device__ __forceinline__ void block_reduce16(float (&v)[NV], float (*red)[4][16]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float acc[9][kOther];
-  float bacc[kIn64 ? CO : 1];
 #pragma unroll
-  for (int t = 0; t < 9; ++t)
+  for (int i = 0; i < NV; ++i) {
+    v[i] += __shfl_xor(v[i], 16);
+    v[i] += __shfl_xor(v[i], 32);
+  }
+  if (lane < 16) {
 #pragma unroll
-    for (int o = 0; o < kOther; ++o) acc[t][o] = 0.0f;
+    for (int i = 0; i < NV; ++i) red[i][wave][lane] = v[i];
+  }
+  __syncthreads();
+}
+
+// Weight and bias gradients of a CI -> 64 layer: dw[t][ci][co] and db[co]
+// summed over pixels.  16 lanes per pixel, 4 output channels per lane; two
+// pixels per iteration so their loads overlap.
+template <int CI>
+__global__ __launch_bounds__(256) void k_in_wgrad(const bf16 *x, const bf16 *dz, float *dw,
+                                                  float *db, int B, int H, int W) {
+  constexpr int NA = 9 * CI * 4, NV = NA + 4;
+  __shared__ float red[NV][4][16];
+  const int j = threadIdx.x & 15;
+  float v[NV];
 #pragma unroll
-  for (int o = 0; o < (kIn64 ? CO : 1); ++o) bacc[o] = 0.0f;
-  float bias64 = 0.0f;
-  const int64_t P = static_cast<int64_t>(B) * H * W;
-  for (int64_t p = blockIdx.x * 4 + wave; p < P; p += (int64_t)gridDim.x * 4) {
-    const int w = p % W;
-    const int h = (p / W) % H;
-    const int64_t b = p / (static_cast<int64_t>(W) * H);
-    if (!kIn64) {
-      const float g = static_cast<float>(dz[p * 64 + lane]);  // co = lane
-      bias64 += g;
+  for (int i = 0; i < NV; ++i) v[i] = 0.0f;
+  const int P = B * H * W;
+  const int stride = gridDim.x * 16;
+  for (int p0 = (blockIdx.x * 256 + threadIdx.x) >> 4; p0 < P; p0 += 2 * stride) {
+    const int p1r = p0 + stride;
+    const bool has1 = p1r < P;
+    const int pp[2] = {p0, has1 ? p1r : p0};
+    float g[2][4];
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const bf16x4 gv = *reinterpret_cast<const bf16x4 *>(dz + static_cast<int64_t>(pp[s]) * 64 + 4 * j);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) g[s][k] = (s == 0 || has1) ? static_cast<float>(gv[k]) : 0.0f;
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int p = pp[s];
+      const int w = p % W, h = (p / W) % H;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[NA + k] += g[s][k];
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
-        const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
-        if (hh < 0 || hh >= H || ww < 0 || ww >= W) continue;
-        const bf16 *src = x + ((b * H + hh) * W + ww) * CI;
+        bool ok;
+        const int q = tap_pixel(p, h, w, t / 3 - 1, t % 3 - 1, H, W, ok);
 #pragma unroll
-        for (int c = 0; c < CI; ++c) acc[t][c] += static_cast<float>(src[c]) * g;
-      }
-    } else {
-      float g[CO];
+        for (int c = 0; c < CI; ++c) {
+          const float xv = ok ? static_cast<float>(x[static_cast<int64_t>(q) * CI + c]) : 0.0f;
 #pragma unroll
-      for (int o = 0; o < CO; ++o) {
-        g[o] = static_cast<float>(dz[p * CO + o]);
-        bacc[o] += g[o];
-      }
-#pragma unroll
-      for (int t = 0; t < 9; ++t) {
-        const int hh = h + t / 3 - 1, ww = w + t % 3 - 1;
-        if (hh < 0 || hh >= H || ww < 0 || ww >= W) continue;
-        const float xv = static_cast<float>(x[((b * H + hh) * W + ww) * 64 + lane]);  // ci = lane
-#pragma unroll
-        for (int o = 0; o < CO; ++o) acc[t][o] += xv * g[o];
+          for (int k = 0; k < 4; ++k) v[(t * CI + c) * 4 + k] += xv * g[s][k];
+        }
       }
     }
   }
-  // Reduce the four wavefronts through LDS, then one atomic per value.
-  __shared__ float red[4][64];
-#pragma unroll 1
-  for (int t = 0; t < 9; ++t) {
-#pragma unroll 1
-    for (int o = 0; o < kOther; ++o) {
-      red[wave][lane] = acc[t][o];
-      __syncthreads();
-      if (wave == 0) {
-        const float s = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
-        if (!kIn64) atomicAdd(&dw[(t * CI + o) * 64 + lane], s);   // [t][ci=o][co=lane]
-        else atomicAdd(&dw[(t * 64 + lane) * CO + o], s);           // [t][ci=lane][co=o]
+  block_reduce16<NV>(v, red);
+  for (int e = threadIdx.x; e < NV * 16; e += 256) {
+    const int i = e >> 4, l = e & 15;
+    const float s = red[i][0][l] + red[i][1][l] + red[i][2][l] + red[i][3][l];
+    if (i < NA) atomicAdd(&dw[(i >> 2) * 64 + 4 * l + (i & 3)], s);  // [t][ci][co = 4l + k]
+    else atomicAdd(&db[4 * l + (i - NA)], s);
+  }
+}
+
+// Weight and bias gradients of a 64 -> CO layer: dw[t][ci][o] and db[o].
+// 16 lanes per pixel, 4 input channels per lane; two pixels per iteration.
+template <int CO>
+__global__ __launch_bounds__(256) void k_out_wgrad(const bf16 *x, const bf16 *dz, float *dw,
+                                                   float *db, int B, int H, int W) {
+  constexpr int NA = 9 * 4 * CO, NV = NA + CO;
+  __shared__ float red[NV][4][16];
+  const int j = threadIdx.x & 15;
+  float v[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) v[i] = 0.0f;
+  const int P = B * H * W;
+  const int stride = gridDim.x * 16;
+  for (int p0 = (blockIdx.x * 256 + threadIdx.x) >> 4; p0 < P; p0 += 2 * stride) {
+    const int p1r = p0 + stride;
+    const bool has1 = p1r < P;
+    const int pp[2] = {p0, has1 ? p1r : p0};
+    float g[2][CO];
+#pragma unroll
+    for (int s = 0; s < 2; ++s)
+#pragma unroll
+      for (int o = 0; o < CO; ++o)
+        g[s][o] = (s == 0 || has1) ? static_cast<float>(dz[static_cast<int64_t>(pp[s]) * CO + o]) : 0.0f;
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
+      const int p = pp[s];
+      const int w = p % W, h = (p / W) % H;
+      if (j == 0) {
+#pragma unroll
+        for (int o = 0; o < CO; ++o) v[NA + o] += g[s][o];
       }
-      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        bool ok;
+        const int q = tap_pixel(p, h, w, t / 3 - 1, t % 3 - 1, H, W, ok);
+        const bf16x4 xv4 = *reinterpret_cast<const bf16x4 *>(x + static_cast<int64_t>(q) * 64 + 4 * j);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+          const float xv = ok ? static_cast<float>(xv4[k]) : 0.0f;
+#pragma unroll
+          for (int o = 0; o < CO; ++o) v[(t * 4 + k) * CO + o] += xv * g[s][o];
+        }
+      }
     }
   }
-  if (!kIn64) {
-    red[wave][lane] = bias64;
-    __syncthreads();
-    if (wave == 0) atomicAdd(&db[lane], red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]);
-  } else {
-    if (lane == 0) {
-#pragma unroll
-      for (int o = 0; o < CO; ++o) atomicAdd(&db[o], bacc[o]);
+  block_reduce16<NV>(v, red);
+  for (int e = threadIdx.x; e < NV * 16; e += 256) {
+    const int i = e >> 4, l = e & 15;
+    const float s = red[i][0][l] + red[i][1][l] + red[i][2][l] + red[i][3][l];
+    if (i < NA) {
+      const int t = i / (4 * CO), k = (i / CO) & 3, o = i % CO;
+      atomicAdd(&dw[(t * 64 + 4 * l + k) * CO + o], s);  // [t][ci = 4l + k][o]
+    } else if (l == 0) {
+      atomicAdd(&db[i - NA], s);
     }
   }
 }
 
 // Bias gradient of a 64-channel layer: db[c] += sum over pixels of dz[p][c].
+// 8 lanes per pixel, 8 channels per lane, two pixels per iteration.
 __global__ __launch_bounds__(256) void k_bias_grad64(const bf16 *dz, float *db, int64_t P) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  float s = 0.0f;
-  for (int64_t p = blockIdx.x * 4 + wave; p < P; p += (int64_t)gridDim.x * 4)
-    s += static_cast<float>(dz[p * 64 + lane]);
+  const int j = threadIdx.x & 7, lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  float s[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s[k] = 0.0f;
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * 32;
+  for (int64_t p0 = (blockIdx.x * 256 + threadIdx.x) >> 3; p0 < P; p0 += 2 * stride) {
+    const int64_t p1 = p0 + stride;
+    const bf16x8 g0 = *reinterpret_cast<const bf16x8 *>(dz + p0 * 64 + 8 * j);
+    const bf16x8 g1 = *reinterpret_cast<const bf16x8 *>(dz + (p1 < P ? p1 : p0) * 64 + 8 * j);
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      s[k] += static_cast<float>(g0[k]) + (p1 < P ? static_cast<float>(g1[k]) : 0.0f);
+  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) {
+    s[k] += __shfl_xor(s[k], 8);
+    s[k] += __shfl_xor(s[k], 16);
+    s[k] += __shfl_xor(s[k], 32);
+  }
   __shared__ float red[4][64];
-  red[wave][lane] = s;
+  if (lane < 8) {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) red[wave][8 * lane + k] = s[k];
+  }
   __syncthreads();
-  if (wave == 0) atomicAdd(&db[lane], red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane]);
+  if (threadIdx.x < 64) {
+    const int c = threadIdx.x;
+    atomicAdd(&db[c], red[0][c] + red[1][c] + red[2][c] + red[3][c]);
+  }
 }
 
 // Adam on FP32 masters.  Writes the BF16 weights in up to three layouts:
@@ -555,17 +902,6 @@ static int cu_count() {
 
 using namespace pml;
 
-#define PML_SMALL_DISPATCH(KERNEL, ARGS, ...)                                          \
-  do {                                                                                 \
-    if (a.Ci == 1 && a.Co == 64) hipLaunchKernelGGL((KERNEL<1, 64>), __VA_ARGS__, ARGS); \
-    else if (a.Ci == 2 && a.Co == 64) hipLaunchKernelGGL((KERNEL<2, 64>), __VA_ARGS__, ARGS); \
-    else if (a.Ci == 3 && a.Co == 64) hipLaunchKernelGGL((KERNEL<3, 64>), __VA_ARGS__, ARGS); \
-    else if (a.Ci == 64 && a.Co == 1) hipLaunchKernelGGL((KERNEL<64, 1>), __VA_ARGS__, ARGS); \
-    else if (a.Ci == 64 && a.Co == 2) hipLaunchKernelGGL((KERNEL<64, 2>), __VA_ARGS__, ARGS); \
-    else if (a.Ci == 64 && a.Co == 3) hipLaunchKernelGGL((KERNEL<64, 3>), __VA_ARGS__, ARGS); \
-    else return fail(PML_E_INVALID, "small layers are C (1..3) -> 64 or 64 -> C");      \
-  } while (0)
-
 extern "C" int pml_conv64(int mode, const void *in, const void *wk, const float *bias,
                           const void *zprev, void *z_out, void *a_out, int B, int H, int W,
                           void *stream) {
@@ -573,66 +909,91 @@ extern "C" int pml_conv64(int mode, const void *in, const void *wk, const float 
     return fail(PML_E_INVALID, "bad conv64 arguments");
   if (mode == kFwd && !bias) return fail(PML_E_INVALID, "forward needs a bias");
   if (mode == kBwd && !zprev) return fail(PML_E_INVALID, "backward needs the pre-activation");
-  ConvArgs a{};
-  a.in = static_cast<const bf16 *>(in);
-  a.wk = static_cast<const bf16 *>(wk);
-  a.bias = bias;
-  a.zprev = static_cast<const bf16 *>(zprev);
-  a.z_out = static_cast<bf16 *>(z_out);
-  a.a_out = static_cast<bf16 *>(a_out);
-  a.B = B;
-  a.H = H;
-  a.W = W;
-  a.tiles_h = (H + kTR - 1) / kTR;
-  a.tiles_w = (W + kTC - 1) / kTC;
-  const int64_t nt = static_cast<int64_t>(B) * a.tiles_h * a.tiles_w;
-  if (nt >= (int64_t(1) << 31)) return fail(PML_E_INVALID, "too many tiles");
-  a.n_tiles = static_cast<int>(nt);
-  const int grid = static_cast<int>(std::min<int64_t>(nt, cu_count()));
-  hipStream_t st = as_stream(stream);
-  if (mode == kFwd) {
-    PML_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv64<kFwd>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
-    hipLaunchKernelGGL(k_conv64<kFwd>, dim3(grid), dim3(512), kConvLds, st, a);
-  } else if (mode == kBwd) {
-    PML_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_conv64<kBwd>),
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
-    hipLaunchKernelGGL(k_conv64<kBwd>, dim3(grid), dim3(512), kConvLds, st, a);
-  } else {
+  if (mode != kFwd && mode != kBwd)
     return fail(PML_E_INVALID, "mode must be 0 (forward) or 1 (input gradient)");
+  // Each launch covers whole images whose tensors stay below 2^31 bytes,
+  // the range of the kernel's 32-bit buffer offsets.
+  const int64_t img_elems = static_cast<int64_t>(H) * W * 64;
+  const int64_t img_bytes = img_elems * 2;
+  if (img_bytes >= (int64_t(1) << 31)) return fail(PML_E_INVALID, "image too large for conv64");
+  const int per = static_cast<int>(std::max<int64_t>(1, ((int64_t(1) << 31) - 1) / img_bytes));
+  const void *kern = mode == kFwd ? reinterpret_cast<const void *>(&k_conv64<kFwd>)
+                                  : reinterpret_cast<const void *>(&k_conv64<kBwd>);
+  PML_HIP_CHECK(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, kConvLds));
+  hipStream_t st = as_stream(stream);
+  for (int b0 = 0; b0 < B; b0 += per) {
+    const int nb = std::min(per, B - b0);
+    const int64_t off = b0 * img_elems;
+    ConvArgs a{};
+    a.in = static_cast<const bf16 *>(in) + off;
+    a.wk = static_cast<const bf16 *>(wk);
+    a.bias = bias;
+    a.zprev = zprev ? static_cast<const bf16 *>(zprev) + off : nullptr;
+    a.z_out = z_out ? static_cast<bf16 *>(z_out) + off : nullptr;
+    a.a_out = static_cast<bf16 *>(a_out) + off;
+    a.B = nb;
+    a.H = H;
+    a.W = W;
+    a.tiles_h = (H + kTR - 1) / kTR;
+    a.tiles_w = (W + kTC - 1) / kTC;
+    a.n_tiles = nb * a.tiles_h * a.tiles_w;
+    a.bytes = static_cast<unsigned>(nb * img_bytes);
+    const int grid = std::min(a.n_tiles, cu_count());
+    if (mode == kFwd)
+      hipLaunchKernelGGL(k_conv64<kFwd>, dim3(grid), dim3(kConvThreads), kConvLds, st, a);
+    else
+      hipLaunchKernelGGL(k_conv64<kBwd>, dim3(grid), dim3(kConvThreads), kConvLds, st, a);
+    PML_LAUNCH_CHECK();
   }
-  PML_LAUNCH_CHECK();
   return PML_OK;
 }
 
-extern "C" int pml_wgrad64(const void *x, const void *dz, float *dw, int B, int H, int W,
-                           void *stream) {
+extern "C" int pml_wgrad64(const void *x, const void *dz, float *dw, float *db, int B, int H,
+                           int W, void *stream) {
   if (B <= 0 || H <= 0 || W <= 0 || !x || !dz || !dw)
     return fail(PML_E_INVALID, "bad wgrad64 arguments");
-  WgradArgs a{};
-  a.x = static_cast<const bf16 *>(x);
-  a.dz = static_cast<const bf16 *>(dz);
-  a.dw = dw;
-  a.B = B;
-  a.H = H;
-  a.W = W;
-  a.tiles_h = (H + kTR - 1) / kTR;
-  a.tiles_w = (W + kTC - 1) / kTC;
-  const int64_t nt = static_cast<int64_t>(B) * a.tiles_h * a.tiles_w;
-  if (nt >= (int64_t(1) << 31)) return fail(PML_E_INVALID, "too many tiles");
-  a.n_tiles = static_cast<int>(nt);
-  const int grid = static_cast<int>(std::min<int64_t>(nt, cu_count()));
+  // Each launch covers whole images whose tensors stay below 2^31 bytes.
+  const int64_t img_elems = static_cast<int64_t>(H) * W * 64;
+  const int64_t img_bytes = img_elems * 2;
+  if (img_bytes >= (int64_t(1) << 31)) return fail(PML_E_INVALID, "image too large for wgrad64");
+  const int per = static_cast<int>(std::max<int64_t>(1, ((int64_t(1) << 31) - 1) / img_bytes));
   PML_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_wgrad64),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, kWgradLds));
-  hipLaunchKernelGGL(k_wgrad64, dim3(grid), dim3(576), kWgradLds, as_stream(stream), a);
-  PML_LAUNCH_CHECK();
+  for (int b0 = 0; b0 < B; b0 += per) {
+    const int nb = std::min(per, B - b0);
+    WgradArgs a{};
+    a.x = static_cast<const bf16 *>(x) + b0 * img_elems;
+    a.dz = static_cast<const bf16 *>(dz) + b0 * img_elems;
+    a.dw = dw;
+    a.db = db;
+    a.B = nb;
+    a.H = H;
+    a.W = W;
+    a.tiles_h = (H + kGR - 1) / kGR;
+    a.tiles_w = (W + kGC - 1) / kGC;
+    a.n_tiles = nb * a.tiles_h * a.tiles_w;
+    a.bytes = static_cast<unsigned>(nb * img_bytes);
+    hipLaunchKernelGGL(k_wgrad64, dim3(std::min(a.n_tiles, cu_count())), dim3(kWgradThreads),
+                       kWgradLds, as_stream(stream), a);
+    PML_LAUNCH_CHECK();
+  }
   return PML_OK;
+}
+
+// Blocks for a grid-stride loop that covers `units` lane groups of
+// `per_block` per block, capped at `per_cu` blocks per CU.
+static int small_grid(int64_t units, int per_block, int per_cu) {
+  const int64_t need = (units + per_block - 1) / per_block;
+  const int64_t cap = static_cast<int64_t>(per_cu) * cu_count();
+  return static_cast<int>(std::max<int64_t>(1, std::min(need, cap)));
 }
 
 extern "C" int pml_conv_small(int mode, const void *in, const void *w, const float *bias,
                               const void *zprev, void *z_out, void *a_out, float *y_out, int Ci,
                               int Co, int B, int H, int W, void *stream) {
   if (B <= 0 || H <= 0 || W <= 0 || !in || !w) return fail(PML_E_INVALID, "bad conv arguments");
+  const int64_t P = static_cast<int64_t>(B) * H * W;
+  if (P >= (int64_t(1) << 31)) return fail(PML_E_INVALID, "too many pixels");
   SmallArgs a{};
   a.in = static_cast<const bf16 *>(in);
   a.w = static_cast<const bf16 *>(w);
@@ -647,18 +1008,24 @@ extern "C" int pml_conv_small(int mode, const void *in, const void *w, const flo
   a.Ci = Ci;
   a.Co = Co;
   hipStream_t st = as_stream(stream);
-  const int nb = n_blocks(static_cast<int64_t>(B) * H * W, 1 << 16);
+  const dim3 grid(small_grid(P, 32, 8)), block(256);
   if (mode == kFwd) {
     if (!bias) return fail(PML_E_INVALID, "forward needs a bias");
     if (Co == 64 && !a_out) return fail(PML_E_INVALID, "missing activation output");
     if (Co < 64 && !y_out) return fail(PML_E_INVALID, "missing output");
-    PML_SMALL_DISPATCH(k_small_fwd, a, dim3(nb), dim3(256), 0, st);
+    if (Ci == 1 && Co == 64) hipLaunchKernelGGL(k_in_fwd<1>, grid, block, 0, st, a);
+    else if (Ci == 2 && Co == 64) hipLaunchKernelGGL(k_in_fwd<2>, grid, block, 0, st, a);
+    else if (Ci == 3 && Co == 64) hipLaunchKernelGGL(k_in_fwd<3>, grid, block, 0, st, a);
+    else if (Ci == 64 && Co == 1) hipLaunchKernelGGL(k_out_fwd<1>, grid, block, 0, st, a);
+    else if (Ci == 64 && Co == 2) hipLaunchKernelGGL(k_out_fwd<2>, grid, block, 0, st, a);
+    else if (Ci == 64 && Co == 3) hipLaunchKernelGGL(k_out_fwd<3>, grid, block, 0, st, a);
+    else return fail(PML_E_INVALID, "small layers are C (1..3) -> 64 or 64 -> C");
   } else if (mode == kBwd) {
     // input gradient of a 64 -> Co layer (the caller passes Ci = 64)
     if (Ci != 64 || !zprev || !a_out) return fail(PML_E_INVALID, "bad input-gradient arguments");
-    if (Co == 1) hipLaunchKernelGGL(k_small_bwd<1>, dim3(nb), dim3(256), 0, st, a);
-    else if (Co == 2) hipLaunchKernelGGL(k_small_bwd<2>, dim3(nb), dim3(256), 0, st, a);
-    else if (Co == 3) hipLaunchKernelGGL(k_small_bwd<3>, dim3(nb), dim3(256), 0, st, a);
+    if (Co == 1) hipLaunchKernelGGL(k_out_bwd<1>, grid, block, 0, st, a);
+    else if (Co == 2) hipLaunchKernelGGL(k_out_bwd<2>, grid, block, 0, st, a);
+    else if (Co == 3) hipLaunchKernelGGL(k_out_bwd<3>, grid, block, 0, st, a);
     else return fail(PML_E_INVALID, "input gradient covers 64 -> 1..3 layers");
   } else {
     return fail(PML_E_INVALID, "mode must be 0 or 1");
@@ -671,28 +1038,27 @@ extern "C" int pml_conv_small_wgrad(const void *x, const void *dz, float *dw, fl
                                     int Co, int B, int H, int W, void *stream) {
   if (B <= 0 || H <= 0 || W <= 0 || !x || !dz || !dw || !db)
     return fail(PML_E_INVALID, "bad weight-gradient arguments");
+  const int64_t P = static_cast<int64_t>(B) * H * W;
+  if (P >= (int64_t(1) << 30)) return fail(PML_E_INVALID, "too many pixels");
   hipStream_t st = as_stream(stream);
-  const int grid = 4 * cu_count();
+  const dim3 grid(small_grid(P, 32, 4)), block(256);
   const bf16 *xb = static_cast<const bf16 *>(x);
   const bf16 *dzb = static_cast<const bf16 *>(dz);
-#define PML_WG(CI, CO) \
-  hipLaunchKernelGGL((k_small_wgrad<CI, CO>), dim3(grid), dim3(256), 0, st, xb, dzb, dw, db, B, H, W)
-  if (Ci == 1 && Co == 64) PML_WG(1, 64);
-  else if (Ci == 2 && Co == 64) PML_WG(2, 64);
-  else if (Ci == 3 && Co == 64) PML_WG(3, 64);
-  else if (Ci == 64 && Co == 1) PML_WG(64, 1);
-  else if (Ci == 64 && Co == 2) PML_WG(64, 2);
-  else if (Ci == 64 && Co == 3) PML_WG(64, 3);
+  if (Ci == 1 && Co == 64) hipLaunchKernelGGL(k_in_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
+  else if (Ci == 2 && Co == 64) hipLaunchKernelGGL(k_in_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
+  else if (Ci == 3 && Co == 64) hipLaunchKernelGGL(k_in_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
+  else if (Ci == 64 && Co == 1) hipLaunchKernelGGL(k_out_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
+  else if (Ci == 64 && Co == 2) hipLaunchKernelGGL(k_out_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
+  else if (Ci == 64 && Co == 3) hipLaunchKernelGGL(k_out_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
   else return fail(PML_E_INVALID, "small layers are C (1..3) -> 64 or 64 -> C");
-#undef PML_WG
   PML_LAUNCH_CHECK();
   return PML_OK;
 }
 
 extern "C" int pml_bias_grad64(const void *dz, float *db, int64_t pixels, void *stream) {
   if (pixels <= 0 || !dz || !db) return fail(PML_E_INVALID, "bad bias-gradient arguments");
-  hipLaunchKernelGGL(k_bias_grad64, dim3(4 * cu_count()), dim3(256), 0, as_stream(stream),
-                     static_cast<const bf16 *>(dz), db, pixels);
+  hipLaunchKernelGGL(k_bias_grad64, dim3(small_grid(pixels, 64, 4)), dim3(256), 0,
+                     as_stream(stream), static_cast<const bf16 *>(dz), db, pixels);
   PML_LAUNCH_CHECK();
   return PML_OK;
 }

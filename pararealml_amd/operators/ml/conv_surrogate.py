@@ -244,9 +244,7 @@ class ConvSurrogate:
         for layer in range(last - 1, 0, -1):
             nat.check(lib.pml_wgrad64(
                 ws["a"][layer - 1].data_ptr(), dz_cur.data_ptr(), self.gw[layer].data_ptr(),
-                batch, h, w, sp))
-            nat.check(lib.pml_bias_grad64(dz_cur.data_ptr(), self.gb[layer].data_ptr(),
-                                          batch * h * w, sp))
+                self.gb[layer].data_ptr(), batch, h, w, sp))
             nat.check(lib.pml_conv64(
                 1, dz_cur.data_ptr(), self.wb[layer].data_ptr(), None,
                 ws["z"][layer - 1].data_ptr(), None, dz_next.data_ptr(), batch, h, w, sp))

@@ -76,24 +76,34 @@ def test_conv64_input_gradient_matches_oracle():
     assert rel(out.float().cpu().numpy(), ref) <= 4e-3
 
 
-def test_wgrad64_matches_oracle():
+@pytest.mark.parametrize("shape", [(3, 10, 45), (2, 37, 70)])
+@pytest.mark.parametrize("with_bias", [True, False])
+def test_wgrad64_matches_oracle(shape, with_bias):
     import torch
 
     from pararealml_amd import _native as nat
 
-    print("case: wgrad64", flush=True)
-    b, h, w = 3, 10, 45
+    print(f"case: wgrad64 {shape} bias={with_bias}", flush=True)
+    b, h, w = shape
     rng = np.random.RandomState(2)
     x = so.bf16(rng.randn(b, h, w, 64))
     dz = so.bf16(rng.randn(b, h, w, 64))
     ref = so.conv3x3_weight_grad(x, dz).reshape(9, 64, 64)
     lib = nat.device_lib()
     dw = torch.zeros(9 * 64 * 64, dtype=torch.float32, device="cuda")
+    db = torch.zeros(64, dtype=torch.float32, device="cuda")
     xt = torch.tensor(x, dtype=torch.bfloat16, device="cuda")
     dzt = torch.tensor(dz, dtype=torch.bfloat16, device="cuda")
-    nat.check(lib.pml_wgrad64(xt.data_ptr(), dzt.data_ptr(), dw.data_ptr(), b, h, w, None))
+    nat.check(lib.pml_wgrad64(xt.data_ptr(), dzt.data_ptr(), dw.data_ptr(),
+                              db.data_ptr() if with_bias else None, b, h, w, None))
     torch.cuda.synchronize()
     assert rel(dw.cpu().numpy().reshape(9, 64, 64), ref) <= 1e-5
+    if with_bias:
+        # the bias gradient is the dz column sum over all pixels
+        db_ref = dz.reshape(-1, 64).sum(axis=0)
+        assert rel(db.cpu().numpy(), db_ref) <= 1e-5
+    else:
+        assert not db.any()
 
 
 @pytest.mark.parametrize("c", [1, 3])
