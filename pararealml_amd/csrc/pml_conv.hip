@@ -459,27 +459,44 @@ __global__ __launch_bounds__(kWgradThreads) void k_wgrad64(WgradArgs a) {
 // Small layers (VALU).  NHWC rows of 64 BF16 channels are 128 bytes; every
 // kernel gives a pixel a group of lanes that each own a contiguous run of
 // channels, so the 64-channel traffic moves as 16-byte (8 channels) or
-// 8-byte (4 channels) vector accesses and a wavefront covers 8 or 4
-// neighbouring pixels.  Weights live in LDS as FP32.
+// 8-byte (4 channels) accesses and a wavefront covers 8 or 4 neighbouring
+// pixels.  Weights live in LDS as FP32.  A pixel's nine taps are addressed
+// as its own byte offset plus a per-tap constant; taps outside the image
+// (from a 9-bit mask computed once per pixel) read through the drop offset,
+// which returns zero.  With one field a thread handles two pixels per
+// iteration, all their loads issued before the arithmetic.
 struct SmallArgs {
   const bf16 *in;      // [B, H, W, Ci]
   const bf16 *w;       // [9][Ci][Co]
   const float *bias;   // [Co]
   const bf16 *zprev;   // backward: [B, H, W, Ci] pre-activation of the input
-  bf16 *z_out;         // forward, Co == 64: pre-activation
+  bf16 *z_out;         // forward, Co == 64: pre-activation (may be null)
   bf16 *a_out;         // forward, Co == 64: activation / backward: gradient
   float *y_out;        // forward, Co < 64: output (no activation)
   int B, H, W, Ci, Co;
 };
 
-// Flat index of the pixel at offset (dh, dw) from p and whether it lies in
-// the image; an outside neighbour is replaced by p itself, so the caller
-// can load unconditionally and select zero.
-__device__ __forceinline__ int tap_pixel(int p, int h, int w, int dh, int dw, int H, int W,
-                                         bool &ok) {
-  const int hh = h + dh, ww = w + dw;
-  ok = hh >= 0 && hh < H && ww >= 0 && ww < W;
-  return ok ? p + dh * W + dw : p;
+// Bit 3 th + tw is set when the tap at offset (th - 1, tw - 1) of pixel
+// (h, w) lies in the image; MIRROR uses offset (1 - th, 1 - tw).
+template <bool MIRROR>
+__device__ __forceinline__ unsigned tap_mask(int h, int w, int H, int W) {
+  const unsigned lo_r = h > 0 ? 1u : 0u, hi_r = h + 1 < H ? 1u : 0u;
+  const unsigned lo_c = w > 0 ? 1u : 0u, hi_c = w + 1 < W ? 1u : 0u;
+  const unsigned r = MIRROR ? (hi_r | 2u | (lo_r << 2)) : (lo_r | 2u | (hi_r << 2));
+  const unsigned c = MIRROR ? (hi_c | 2u | (lo_c << 2)) : (lo_c | 2u | (hi_c << 2));
+  return ((r & 1u) ? c : 0u) | ((r & 2u) ? c << 3 : 0u) | ((r & 4u) ? c << 6 : 0u);
+}
+// Byte offset of tap t relative to the pixel, `bpp` bytes per pixel.
+template <bool MIRROR>
+__device__ __forceinline__ int tap_delta(int t, int W, int bpp) {
+  const int dh = t / 3 - 1, dw = t % 3 - 1;
+  return (MIRROR ? -(dh * W + dw) : dh * W + dw) * bpp;
+}
+__device__ __forceinline__ unsigned tap_off(unsigned mask, int t, unsigned base, int delta) {
+  return (mask >> t) & 1u ? base + static_cast<unsigned>(delta) : kDrop;
+}
+__device__ __forceinline__ float bf16_bits(unsigned short u) {
+  return __builtin_bit_cast(float, static_cast<unsigned>(u) << 16);
 }
 
 template <int N>
@@ -496,37 +513,54 @@ __global__ __launch_bounds__(256) void k_in_fwd(SmallArgs a) {
   lds_fill<9 * CI * 64>(wl, a.w);
   if (threadIdx.x < 64) bl[threadIdx.x] = a.bias[threadIdx.x];
   __syncthreads();
+  constexpr int NP = CI == 1 ? 2 : 1;  // pixels per iteration
   const int j = threadIdx.x & 7;
   const int P = a.B * a.H * a.W;
-  for (int p = (blockIdx.x * 256 + threadIdx.x) >> 3; p < P; p += gridDim.x * 32) {
-    // With more than one field the weights would not fit in registers:
-    // re-read them from LDS each pixel instead of hoisting them.
-    if (CI > 1) asm volatile("" ::: "memory");
-    const int w = p % a.W, h = (p / a.W) % a.H;
-    float acc[8];
+  const __amdgpu_buffer_rsrc_t r_in = conv_rsrc(a.in, P * CI * 2);
+  const __amdgpu_buffer_rsrc_t r_z = conv_rsrc(a.z_out, a.z_out ? P * 128 : 0);
+  const __amdgpu_buffer_rsrc_t r_a = conv_rsrc(a.a_out, P * 128);
+  const int stride = gridDim.x * 32;
+  for (int p0 = (blockIdx.x * 256 + threadIdx.x) >> 3; p0 < P; p0 += NP * stride) {
+    float xv[NP][9][CI];
+    unsigned obase[NP];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = bl[8 * j + k];
+    for (int s = 0; s < NP; ++s) {
+      const int p = p0 + s * stride;
+      const bool in = p < P;
+      const int w = p % a.W, h = (p / a.W) % a.H;
+      const unsigned mask = in ? tap_mask<false>(h, w, a.H, a.W) : 0u;
+      obase[s] = in ? static_cast<unsigned>(p) * 128 + 16 * j : kDrop;
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-      bool ok;
-      const int q = tap_pixel(p, h, w, t / 3 - 1, t % 3 - 1, a.H, a.W, ok);
+      for (int t = 0; t < 9; ++t) {
+        const unsigned off = tap_off(mask, t, static_cast<unsigned>(p) * CI * 2,
+                                     tap_delta<false>(t, a.W, CI * 2));
 #pragma unroll
-      for (int c = 0; c < CI; ++c) {
-        const float xv = ok ? static_cast<float>(a.in[static_cast<int64_t>(q) * CI + c]) : 0.0f;
-        const float *wt = wl + (t * CI + c) * 64 + 8 * j;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) acc[k] += xv * wt[k];
+        for (int c = 0; c < CI; ++c)
+          xv[s][t][c] = bf16_bits(__builtin_amdgcn_raw_buffer_load_b16(r_in, off + 2 * c, 0, 0));
       }
     }
-    bf16x8 zv, av;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      zv[k] = static_cast<bf16>(acc[k]);
-      av[k] = static_cast<bf16>(gelu_f(acc[k]));
+    for (int s = 0; s < NP; ++s) {
+      float acc[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = bl[8 * j + k];
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int c = 0; c < CI; ++c) {
+          const float *wt = wl + (t * CI + c) * 64 + 8 * j;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] += xv[s][t][c] * wt[k];
+        }
+      bf16x8 zv, av;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        zv[k] = static_cast<bf16>(acc[k]);
+        av[k] = static_cast<bf16>(gelu_f(acc[k]));
+      }
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, zv), r_z, obase[s], 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, av), r_a, obase[s], 0, 0);
     }
-    const int64_t o = static_cast<int64_t>(p) * 64 + 8 * j;
-    if (a.z_out) *reinterpret_cast<bf16x8 *>(a.z_out + o) = zv;
-    *reinterpret_cast<bf16x8 *>(a.a_out + o) = av;
   }
 }
 
@@ -538,38 +572,54 @@ __global__ __launch_bounds__(256) void k_out_fwd(SmallArgs a) {
   __shared__ float wl[9 * 64 * CO];
   lds_fill<9 * 64 * CO>(wl, a.w);
   __syncthreads();
+  constexpr int NP = CO == 1 ? 2 : 1;  // pixels per iteration
   const int j = threadIdx.x & 7;
   const int P = a.B * a.H * a.W;
-  for (int p = (blockIdx.x * 256 + threadIdx.x) >> 3; p < P; p += gridDim.x * 32) {
-    // With more than one field the weights would not fit in registers:
-    // re-read them from LDS each pixel instead of hoisting them.
-    if (CO > 1) asm volatile("" ::: "memory");
-    const int w = p % a.W, h = (p / a.W) % a.H;
-    float acc[CO];
+  const __amdgpu_buffer_rsrc_t r_in = conv_rsrc(a.in, P * 128);
+  const int stride = gridDim.x * 32;
+  for (int p0 = (blockIdx.x * 256 + threadIdx.x) >> 3; p0 < P; p0 += NP * stride) {
+    bf16x8 v[NP][9];
 #pragma unroll
-    for (int o = 0; o < CO; ++o) acc[o] = 0.0f;
+    for (int s = 0; s < NP; ++s) {
+      const int p = p0 + s * stride;
+      const bool in = p < P;
+      const int w = p % a.W, h = (p / a.W) % a.H;
+      const unsigned mask = in ? tap_mask<false>(h, w, a.H, a.W) : 0u;
 #pragma unroll
-    for (int t = 0; t < 9; ++t) {
-      bool ok;
-      const int q = tap_pixel(p, h, w, t / 3 - 1, t % 3 - 1, a.H, a.W, ok);
-      const bf16x8 v = *reinterpret_cast<const bf16x8 *>(a.in + static_cast<int64_t>(q) * 64 + 8 * j);
-      const float *wt = wl + (t * 64 + 8 * j) * CO;
+      for (int t = 0; t < 9; ++t)
+        v[s][t] = __builtin_bit_cast(
+            bf16x8, __builtin_amdgcn_raw_buffer_load_b128(
+                        r_in, tap_off(mask, t, static_cast<unsigned>(p) * 128 + 16 * j,
+                                      tap_delta<false>(t, a.W, 128)),
+                        0, 0));
+    }
 #pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const float xv = ok ? static_cast<float>(v[k]) : 0.0f;
+    for (int s = 0; s < NP; ++s) {
+      const int p = p0 + s * stride;
+      float acc[CO];
 #pragma unroll
-        for (int o = 0; o < CO; ++o) acc[o] += xv * wt[k * CO + o];
+      for (int o = 0; o < CO; ++o) acc[o] = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const float *wt = wl + (t * 64 + 8 * j) * CO;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const float xv = static_cast<float>(v[s][t][k]);
+#pragma unroll
+          for (int o = 0; o < CO; ++o) acc[o] += xv * wt[k * CO + o];
+        }
       }
-    }
 #pragma unroll
-    for (int o = 0; o < CO; ++o) {
-      acc[o] += __shfl_xor(acc[o], 1);
-      acc[o] += __shfl_xor(acc[o], 2);
-      acc[o] += __shfl_xor(acc[o], 4);
-    }
-    if (j == 0) {
+      for (int o = 0; o < CO; ++o) {
+        acc[o] += __shfl_xor(acc[o], 1);
+        acc[o] += __shfl_xor(acc[o], 2);
+        acc[o] += __shfl_xor(acc[o], 4);
+      }
+      if (j == 0 && p < P) {
 #pragma unroll
-      for (int o = 0; o < CO; ++o) a.y_out[static_cast<int64_t>(p) * CO + o] = acc[o] + a.bias[o];
+        for (int o = 0; o < CO; ++o)
+          a.y_out[static_cast<int64_t>(p) * CO + o] = acc[o] + a.bias[o];
+      }
     }
   }
 }
@@ -581,42 +631,61 @@ __global__ __launch_bounds__(256) void k_out_bwd(SmallArgs a) {
   __shared__ float wl[9 * 64 * CO];
   lds_fill<9 * 64 * CO>(wl, a.w);
   __syncthreads();
+  constexpr int NP = CO == 1 ? 2 : 1;  // pixels per iteration
   const int j = threadIdx.x & 7;
   const int P = a.B * a.H * a.W;
-  for (int p = (blockIdx.x * 256 + threadIdx.x) >> 3; p < P; p += gridDim.x * 32) {
-    // With more than one field the weights would not fit in registers:
-    // re-read them from LDS each pixel instead of hoisting them.
-    if (CO > 1) asm volatile("" ::: "memory");
-    const int w = p % a.W, h = (p / a.W) % a.H;
-    float acc[8];
+  const __amdgpu_buffer_rsrc_t r_g = conv_rsrc(a.in, P * CO * 2);
+  const __amdgpu_buffer_rsrc_t r_zp = conv_rsrc(a.zprev, P * 128);
+  const __amdgpu_buffer_rsrc_t r_out = conv_rsrc(a.a_out, P * 128);
+  const int stride = gridDim.x * 32;
+  for (int p0 = (blockIdx.x * 256 + threadIdx.x) >> 3; p0 < P; p0 += NP * stride) {
+    float gv[NP][9][CO];
+    bf16x8 zp[NP];
+    unsigned obase[NP];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
-#pragma unroll
-    for (int t = 0; t < 9; ++t) {
+    for (int s = 0; s < NP; ++s) {
+      const int p = p0 + s * stride;
+      const bool in = p < P;
+      const int w = p % a.W, h = (p / a.W) % a.H;
       // input (h, w) feeds output (h - th + 1, w - tw + 1)
-      bool ok;
-      const int q = tap_pixel(p, h, w, 1 - t / 3, 1 - t % 3, a.H, a.W, ok);
-      const float *wt = wl + (t * 64 + 8 * j) * CO;
+      const unsigned mask = in ? tap_mask<true>(h, w, a.H, a.W) : 0u;
+      obase[s] = in ? static_cast<unsigned>(p) * 128 + 16 * j : kDrop;
+      zp[s] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(r_zp, obase[s], 0, 0));
 #pragma unroll
-      for (int o = 0; o < CO; ++o) {
-        const float gv = ok ? static_cast<float>(a.in[static_cast<int64_t>(q) * CO + o]) : 0.0f;
+      for (int t = 0; t < 9; ++t) {
+        const unsigned off = tap_off(mask, t, static_cast<unsigned>(p) * CO * 2,
+                                     tap_delta<true>(t, a.W, CO * 2));
 #pragma unroll
-        for (int k = 0; k < 8; ++k) acc[k] += gv * wt[k * CO + o];
+        for (int o = 0; o < CO; ++o)
+          gv[s][t][o] = bf16_bits(__builtin_amdgcn_raw_buffer_load_b16(r_g, off + 2 * o, 0, 0));
       }
     }
-    const int64_t e = static_cast<int64_t>(p) * 64 + 8 * j;
-    const bf16x8 zp = *reinterpret_cast<const bf16x8 *>(a.zprev + e);
-    bf16x8 dv;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) dv[k] = static_cast<bf16>(acc[k] * gelu_grad_f(static_cast<float>(zp[k])));
-    *reinterpret_cast<bf16x8 *>(a.a_out + e) = dv;
+    for (int s = 0; s < NP; ++s) {
+      float acc[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.0f;
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const float *wt = wl + (t * 64 + 8 * j) * CO;
+#pragma unroll
+        for (int o = 0; o < CO; ++o)
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] += gv[s][t][o] * wt[k * CO + o];
+      }
+      bf16x8 dv;
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        dv[k] = static_cast<bf16>(acc[k] * gelu_grad_f(static_cast<float>(zp[s][k])));
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4, dv), r_out, obase[s], 0, 0);
+    }
   }
 }
 
 // Sums NV per-lane partials over the four pixel slots of each wavefront
 // (lanes l, l + 16, l + 32, l + 48 own the same channels) and the four
-// wavefronts of the block; returns with red[i][l] (l < 16) holding the
-// block's totals.
+// wavefronts of the block; returns with red[i][w][l] (l < 16) holding the
+// partial sums of wavefront w.
 template <int NV>
 __device__ __forceinline__ void block_reduce16(float (&v)[NV], float (*red)[4][16]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -633,8 +702,7 @@ __device__ __forceinline__ void block_reduce16(float (&v)[NV], float (*red)[4][1
 }
 
 // Weight and bias gradients of a CI -> 64 layer: dw[t][ci][co] and db[co]
-// summed over pixels.  16 lanes per pixel, 4 output channels per lane; two
-// pixels per iteration so their loads overlap.
+// summed over pixels.  16 lanes per pixel, 4 output channels per lane.
 template <int CI>
 __global__ __launch_bounds__(256) void k_in_wgrad(const bf16 *x, const bf16 *dz, float *dw,
                                                   float *db, int B, int H, int W) {
@@ -645,35 +713,41 @@ __global__ __launch_bounds__(256) void k_in_wgrad(const bf16 *x, const bf16 *dz,
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = 0.0f;
   const int P = B * H * W;
+  const __amdgpu_buffer_rsrc_t r_x = conv_rsrc(x, P * CI * 2);
+  const __amdgpu_buffer_rsrc_t r_dz = conv_rsrc(dz, P * 128);
   const int stride = gridDim.x * 16;
   for (int p0 = (blockIdx.x * 256 + threadIdx.x) >> 4; p0 < P; p0 += 2 * stride) {
-    const int p1r = p0 + stride;
-    const bool has1 = p1r < P;
-    const int pp[2] = {p0, has1 ? p1r : p0};
-    float g[2][4];
+    float g[2][4], xv[2][9][CI];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const bf16x4 gv = *reinterpret_cast<const bf16x4 *>(dz + static_cast<int64_t>(pp[s]) * 64 + 4 * j);
+      const int p = p0 + s * stride;
+      const bool in = p < P;
+      const int w = p % W, h = (p / W) % H;
+      const unsigned mask = in ? tap_mask<false>(h, w, H, W) : 0u;
+      const bf16x4 gv = __builtin_bit_cast(
+          bf16x4, __builtin_amdgcn_raw_buffer_load_b64(
+                      r_dz, in ? static_cast<unsigned>(p) * 128 + 8 * j : kDrop, 0, 0));
 #pragma unroll
-      for (int k = 0; k < 4; ++k) g[s][k] = (s == 0 || has1) ? static_cast<float>(gv[k]) : 0.0f;
+      for (int k = 0; k < 4; ++k) g[s][k] = static_cast<float>(gv[k]);
+#pragma unroll
+      for (int t = 0; t < 9; ++t) {
+        const unsigned off = tap_off(mask, t, static_cast<unsigned>(p) * CI * 2,
+                                     tap_delta<false>(t, W, CI * 2));
+#pragma unroll
+        for (int c = 0; c < CI; ++c)
+          xv[s][t][c] = bf16_bits(__builtin_amdgcn_raw_buffer_load_b16(r_x, off + 2 * c, 0, 0));
+      }
     }
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const int p = pp[s];
-      const int w = p % W, h = (p / W) % H;
 #pragma unroll
       for (int k = 0; k < 4; ++k) v[NA + k] += g[s][k];
 #pragma unroll
-      for (int t = 0; t < 9; ++t) {
-        bool ok;
-        const int q = tap_pixel(p, h, w, t / 3 - 1, t % 3 - 1, H, W, ok);
+      for (int t = 0; t < 9; ++t)
 #pragma unroll
-        for (int c = 0; c < CI; ++c) {
-          const float xv = ok ? static_cast<float>(x[static_cast<int64_t>(q) * CI + c]) : 0.0f;
+        for (int c = 0; c < CI; ++c)
 #pragma unroll
-          for (int k = 0; k < 4; ++k) v[(t * CI + c) * 4 + k] += xv * g[s][k];
-        }
-      }
+          for (int k = 0; k < 4; ++k) v[(t * CI + c) * 4 + k] += xv[s][t][c] * g[s][k];
     }
   }
   block_reduce16<NV>(v, red);
@@ -686,7 +760,7 @@ __global__ __launch_bounds__(256) void k_in_wgrad(const bf16 *x, const bf16 *dz,
 }
 
 // Weight and bias gradients of a 64 -> CO layer: dw[t][ci][o] and db[o].
-// 16 lanes per pixel, 4 input channels per lane; two pixels per iteration.
+// 16 lanes per pixel, 4 input channels per lane.
 template <int CO>
 __global__ __launch_bounds__(256) void k_out_wgrad(const bf16 *x, const bf16 *dz, float *dw,
                                                    float *db, int B, int H, int W) {
@@ -697,37 +771,44 @@ __global__ __launch_bounds__(256) void k_out_wgrad(const bf16 *x, const bf16 *dz
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = 0.0f;
   const int P = B * H * W;
+  const __amdgpu_buffer_rsrc_t r_x = conv_rsrc(x, P * 128);
+  const __amdgpu_buffer_rsrc_t r_g = conv_rsrc(dz, P * CO * 2);
   const int stride = gridDim.x * 16;
   for (int p0 = (blockIdx.x * 256 + threadIdx.x) >> 4; p0 < P; p0 += 2 * stride) {
-    const int p1r = p0 + stride;
-    const bool has1 = p1r < P;
-    const int pp[2] = {p0, has1 ? p1r : p0};
     float g[2][CO];
-#pragma unroll
-    for (int s = 0; s < 2; ++s)
-#pragma unroll
-      for (int o = 0; o < CO; ++o)
-        g[s][o] = (s == 0 || has1) ? static_cast<float>(dz[static_cast<int64_t>(pp[s]) * CO + o]) : 0.0f;
+    bf16x4 xv[2][9];
 #pragma unroll
     for (int s = 0; s < 2; ++s) {
-      const int p = pp[s];
+      const int p = p0 + s * stride;
+      const bool in = p < P;
       const int w = p % W, h = (p / W) % H;
+      const unsigned mask = in ? tap_mask<false>(h, w, H, W) : 0u;
+#pragma unroll
+      for (int o = 0; o < CO; ++o)
+        g[s][o] = bf16_bits(__builtin_amdgcn_raw_buffer_load_b16(
+            r_g, in ? static_cast<unsigned>(p) * CO * 2 + 2 * o : kDrop, 0, 0));
+#pragma unroll
+      for (int t = 0; t < 9; ++t)
+        xv[s][t] = __builtin_bit_cast(
+            bf16x4, __builtin_amdgcn_raw_buffer_load_b64(
+                        r_x, tap_off(mask, t, static_cast<unsigned>(p) * 128 + 8 * j,
+                                     tap_delta<false>(t, W, 128)),
+                        0, 0));
+    }
+#pragma unroll
+    for (int s = 0; s < 2; ++s) {
       if (j == 0) {
 #pragma unroll
         for (int o = 0; o < CO; ++o) v[NA + o] += g[s][o];
       }
 #pragma unroll
-      for (int t = 0; t < 9; ++t) {
-        bool ok;
-        const int q = tap_pixel(p, h, w, t / 3 - 1, t % 3 - 1, H, W, ok);
-        const bf16x4 xv4 = *reinterpret_cast<const bf16x4 *>(x + static_cast<int64_t>(q) * 64 + 4 * j);
+      for (int t = 0; t < 9; ++t)
 #pragma unroll
         for (int k = 0; k < 4; ++k) {
-          const float xv = ok ? static_cast<float>(xv4[k]) : 0.0f;
+          const float xk = static_cast<float>(xv[s][t][k]);
 #pragma unroll
-          for (int o = 0; o < CO; ++o) v[(t * 4 + k) * CO + o] += xv * g[s][o];
+          for (int o = 0; o < CO; ++o) v[(t * 4 + k) * CO + o] += xk * g[s][o];
         }
-      }
     }
   }
   block_reduce16<NV>(v, red);
@@ -808,28 +889,48 @@ __global__ __launch_bounds__(256) void k_adam(float *p, float *m, float *v, cons
 
 // Per-sample scale s = max |u| over all fields and the normalised BF16 NHWC
 // input x = u / s.  u is field-major [B][C][H*W] in FP64 or FP32.
+// Per-sample normalisation x = u / s, s = max |u| (at least 1e-30), field-
+// major u -> NHWC BF16 x, in three launches over the whole batch: clear s,
+// per-block maxima merged with an integer atomicMax on the float bits
+// (non-negative floats order like their bit patterns; fmaxf drops NaNs),
+// then the scaling pass.
+__global__ void k_norm_clear(float *s, int B) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < B) s[b] = 0.0f;
+}
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_normalize(const T *u, bf16 *x, float *s_out, int C,
-                                                   int64_t HW) {
-  const int b = blockIdx.x;
-  const T *ub = u + static_cast<int64_t>(b) * C * HW;
+__global__ __launch_bounds__(256) void k_norm_max(const T *u, float *s, int C, int64_t HW) {
+  const int b = blockIdx.y;
+  const int64_t n = C * HW;
+  const T *ub = u + static_cast<int64_t>(b) * n;
   float m = 0.0f;
-  for (int64_t i = threadIdx.x; i < C * HW; i += blockDim.x)
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256)
     m = fmaxf(m, fabsf(static_cast<float>(ub[i])));
-  __shared__ float red[256];
-  red[threadIdx.x] = m;
+  for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+  __shared__ float red[4];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if (threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    atomicMax(reinterpret_cast<unsigned *>(s) + b, __float_as_uint(m));
   }
-  const float sc = fmaxf(red[0], 1e-30f);
-  if (threadIdx.x == 0) s_out[b] = sc;
-  for (int64_t i = threadIdx.x; i < C * HW; i += blockDim.x) {
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_norm_scale(const T *u, bf16 *x, float *s, int C,
+                                                    int64_t HW) {
+  const int b = blockIdx.y;
+  const float sc = fmaxf(s[b], 1e-30f);
+  // Every block stores the same clamped value, so the order of these
+  // stores and the other blocks' reads does not matter.
+  if (blockIdx.x == 0 && threadIdx.x == 0) s[b] = sc;
+  const int64_t n = C * HW;
+  const T *ub = u + static_cast<int64_t>(b) * n;
+  for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
     const int c = static_cast<int>(i / HW);
     const int64_t px = i - c * HW;
-    x[(static_cast<int64_t>(b) * HW + px) * C + c] =
-        static_cast<bf16>(static_cast<float>(ub[i]) / sc);
+    x[(static_cast<int64_t>(b) * HW + px) * C + c] = static_cast<bf16>(static_cast<float>(ub[i]) / sc);
   }
 }
 
@@ -988,49 +1089,67 @@ static int small_grid(int64_t units, int per_block, int per_cu) {
   return static_cast<int>(std::max<int64_t>(1, std::min(need, cap)));
 }
 
+// Images per launch of the small-layer kernels: their 64-channel tensors
+// stay below 2^31 bytes, the range of the 32-bit buffer offsets.
+static int small_chunk(int H, int W) {
+  const int64_t img_bytes = static_cast<int64_t>(H) * W * 128;
+  return static_cast<int>(std::max<int64_t>(1, ((int64_t(1) << 31) - 1) / img_bytes));
+}
+
 extern "C" int pml_conv_small(int mode, const void *in, const void *w, const float *bias,
                               const void *zprev, void *z_out, void *a_out, float *y_out, int Ci,
                               int Co, int B, int H, int W, void *stream) {
   if (B <= 0 || H <= 0 || W <= 0 || !in || !w) return fail(PML_E_INVALID, "bad conv arguments");
-  const int64_t P = static_cast<int64_t>(B) * H * W;
-  if (P >= (int64_t(1) << 31)) return fail(PML_E_INVALID, "too many pixels");
-  SmallArgs a{};
-  a.in = static_cast<const bf16 *>(in);
-  a.w = static_cast<const bf16 *>(w);
-  a.bias = bias;
-  a.zprev = static_cast<const bf16 *>(zprev);
-  a.z_out = static_cast<bf16 *>(z_out);
-  a.a_out = static_cast<bf16 *>(a_out);
-  a.y_out = y_out;
-  a.B = B;
-  a.H = H;
-  a.W = W;
-  a.Ci = Ci;
-  a.Co = Co;
-  hipStream_t st = as_stream(stream);
-  const dim3 grid(small_grid(P, 32, 8)), block(256);
+  if (static_cast<int64_t>(H) * W * 128 >= (int64_t(1) << 31))
+    return fail(PML_E_INVALID, "image too large");
+  const bool in_layer = Co == 64 && Ci >= 1 && Ci <= 3;
+  const bool out_layer = Ci == 64 && Co >= 1 && Co <= 3;
+  if (!in_layer && !out_layer) return fail(PML_E_INVALID, "small layers are C (1..3) -> 64 or 64 -> C");
   if (mode == kFwd) {
     if (!bias) return fail(PML_E_INVALID, "forward needs a bias");
-    if (Co == 64 && !a_out) return fail(PML_E_INVALID, "missing activation output");
-    if (Co < 64 && !y_out) return fail(PML_E_INVALID, "missing output");
-    if (Ci == 1 && Co == 64) hipLaunchKernelGGL(k_in_fwd<1>, grid, block, 0, st, a);
-    else if (Ci == 2 && Co == 64) hipLaunchKernelGGL(k_in_fwd<2>, grid, block, 0, st, a);
-    else if (Ci == 3 && Co == 64) hipLaunchKernelGGL(k_in_fwd<3>, grid, block, 0, st, a);
-    else if (Ci == 64 && Co == 1) hipLaunchKernelGGL(k_out_fwd<1>, grid, block, 0, st, a);
-    else if (Ci == 64 && Co == 2) hipLaunchKernelGGL(k_out_fwd<2>, grid, block, 0, st, a);
-    else if (Ci == 64 && Co == 3) hipLaunchKernelGGL(k_out_fwd<3>, grid, block, 0, st, a);
-    else return fail(PML_E_INVALID, "small layers are C (1..3) -> 64 or 64 -> C");
+    if (in_layer && !a_out) return fail(PML_E_INVALID, "missing activation output");
+    if (out_layer && !y_out) return fail(PML_E_INVALID, "missing output");
   } else if (mode == kBwd) {
     // input gradient of a 64 -> Co layer (the caller passes Ci = 64)
-    if (Ci != 64 || !zprev || !a_out) return fail(PML_E_INVALID, "bad input-gradient arguments");
-    if (Co == 1) hipLaunchKernelGGL(k_out_bwd<1>, grid, block, 0, st, a);
-    else if (Co == 2) hipLaunchKernelGGL(k_out_bwd<2>, grid, block, 0, st, a);
-    else if (Co == 3) hipLaunchKernelGGL(k_out_bwd<3>, grid, block, 0, st, a);
-    else return fail(PML_E_INVALID, "input gradient covers 64 -> 1..3 layers");
+    if (!out_layer || !zprev || !a_out) return fail(PML_E_INVALID, "bad input-gradient arguments");
   } else {
     return fail(PML_E_INVALID, "mode must be 0 or 1");
   }
-  PML_LAUNCH_CHECK();
+  hipStream_t st = as_stream(stream);
+  const int per = small_chunk(H, W);
+  const int64_t hw = static_cast<int64_t>(H) * W;
+  for (int b0 = 0; b0 < B; b0 += per) {
+    const int nb = std::min(per, B - b0);
+    const int64_t px = b0 * hw;  // first pixel of the chunk
+    SmallArgs a{};
+    a.in = static_cast<const bf16 *>(in) + px * Ci;
+    a.w = static_cast<const bf16 *>(w);
+    a.bias = bias;
+    a.zprev = zprev ? static_cast<const bf16 *>(zprev) + px * 64 : nullptr;
+    a.z_out = z_out ? static_cast<bf16 *>(z_out) + px * 64 : nullptr;
+    a.a_out = a_out ? static_cast<bf16 *>(a_out) + px * 64 : nullptr;
+    a.y_out = y_out ? y_out + px * Co : nullptr;
+    a.B = nb;
+    a.H = H;
+    a.W = W;
+    a.Ci = Ci;
+    a.Co = Co;
+    // two pixels per lane group and iteration: 64 pixels per block
+    const dim3 grid(small_grid(nb * hw, 64, 8)), block(256);
+    if (mode == kFwd) {
+      if (Ci == 1) hipLaunchKernelGGL(k_in_fwd<1>, grid, block, 0, st, a);
+      else if (Ci == 2) hipLaunchKernelGGL(k_in_fwd<2>, grid, block, 0, st, a);
+      else if (Ci == 3) hipLaunchKernelGGL(k_in_fwd<3>, grid, block, 0, st, a);
+      else if (Co == 1) hipLaunchKernelGGL(k_out_fwd<1>, grid, block, 0, st, a);
+      else if (Co == 2) hipLaunchKernelGGL(k_out_fwd<2>, grid, block, 0, st, a);
+      else hipLaunchKernelGGL(k_out_fwd<3>, grid, block, 0, st, a);
+    } else {
+      if (Co == 1) hipLaunchKernelGGL(k_out_bwd<1>, grid, block, 0, st, a);
+      else if (Co == 2) hipLaunchKernelGGL(k_out_bwd<2>, grid, block, 0, st, a);
+      else hipLaunchKernelGGL(k_out_bwd<3>, grid, block, 0, st, a);
+    }
+    PML_LAUNCH_CHECK();
+  }
   return PML_OK;
 }
 
@@ -1038,20 +1157,29 @@ extern "C" int pml_conv_small_wgrad(const void *x, const void *dz, float *dw, fl
                                     int Co, int B, int H, int W, void *stream) {
   if (B <= 0 || H <= 0 || W <= 0 || !x || !dz || !dw || !db)
     return fail(PML_E_INVALID, "bad weight-gradient arguments");
-  const int64_t P = static_cast<int64_t>(B) * H * W;
-  if (P >= (int64_t(1) << 30)) return fail(PML_E_INVALID, "too many pixels");
+  if (static_cast<int64_t>(H) * W * 128 >= (int64_t(1) << 31))
+    return fail(PML_E_INVALID, "image too large");
+  const bool in_layer = Co == 64 && Ci >= 1 && Ci <= 3;
+  const bool out_layer = Ci == 64 && Co >= 1 && Co <= 3;
+  if (!in_layer && !out_layer) return fail(PML_E_INVALID, "small layers are C (1..3) -> 64 or 64 -> C");
   hipStream_t st = as_stream(stream);
-  const dim3 grid(small_grid(P, 32, 4)), block(256);
-  const bf16 *xb = static_cast<const bf16 *>(x);
-  const bf16 *dzb = static_cast<const bf16 *>(dz);
-  if (Ci == 1 && Co == 64) hipLaunchKernelGGL(k_in_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
-  else if (Ci == 2 && Co == 64) hipLaunchKernelGGL(k_in_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
-  else if (Ci == 3 && Co == 64) hipLaunchKernelGGL(k_in_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
-  else if (Ci == 64 && Co == 1) hipLaunchKernelGGL(k_out_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
-  else if (Ci == 64 && Co == 2) hipLaunchKernelGGL(k_out_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
-  else if (Ci == 64 && Co == 3) hipLaunchKernelGGL(k_out_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, B, H, W);
-  else return fail(PML_E_INVALID, "small layers are C (1..3) -> 64 or 64 -> C");
-  PML_LAUNCH_CHECK();
+  const int per = small_chunk(H, W);
+  const int64_t hw = static_cast<int64_t>(H) * W;
+  for (int b0 = 0; b0 < B; b0 += per) {
+    const int nb = std::min(per, B - b0);
+    const int64_t px = b0 * hw;
+    const bf16 *xb = static_cast<const bf16 *>(x) + px * Ci;
+    const bf16 *dzb = static_cast<const bf16 *>(dz) + px * Co;
+    // two pixels per lane group and iteration: 32 pixels per block
+    const dim3 grid(small_grid(nb * hw, 32, 4)), block(256);
+    if (Ci == 1) hipLaunchKernelGGL(k_in_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
+    else if (Ci == 2) hipLaunchKernelGGL(k_in_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
+    else if (Ci == 3) hipLaunchKernelGGL(k_in_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
+    else if (Co == 1) hipLaunchKernelGGL(k_out_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
+    else if (Co == 2) hipLaunchKernelGGL(k_out_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
+    else hipLaunchKernelGGL(k_out_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
+    PML_LAUNCH_CHECK();
+  }
   return PML_OK;
 }
 
@@ -1082,15 +1210,27 @@ extern "C" int pml_surrogate_normalize(int precision, const void *u, void *x, fl
                                        int C, int64_t HW, void *stream) {
   if (B <= 0 || C <= 0 || HW <= 0 || !u || !x || !s)
     return fail(PML_E_INVALID, "bad normalisation arguments");
-  hipStream_t st = as_stream(stream);
-  if (precision == PML_F64)
-    hipLaunchKernelGGL(k_normalize<double>, dim3(B), dim3(256), 0, st,
-                       static_cast<const double *>(u), static_cast<bf16 *>(x), s, C, HW);
-  else if (precision == PML_F32)
-    hipLaunchKernelGGL(k_normalize<float>, dim3(B), dim3(256), 0, st,
-                       static_cast<const float *>(u), static_cast<bf16 *>(x), s, C, HW);
-  else
+  if (precision != PML_F64 && precision != PML_F32)
     return fail(PML_E_INVALID, "normalisation reads FP32 or FP64");
+  if (B > 65535) return fail(PML_E_INVALID, "too many samples");
+  hipStream_t st = as_stream(stream);
+  // about 8 elements per thread, spread over the whole chip
+  const int64_t n = static_cast<int64_t>(C) * HW;
+  const int per_sample = static_cast<int>(
+      std::max<int64_t>(1, std::min<int64_t>((n + 2047) / 2048, (8LL * cu_count() + B - 1) / B)));
+  const dim3 grid(per_sample, B);
+  hipLaunchKernelGGL(k_norm_clear, dim3((B + 255) / 256), dim3(256), 0, st, s, B);
+  if (precision == PML_F64) {
+    const double *ud = static_cast<const double *>(u);
+    hipLaunchKernelGGL(k_norm_max<double>, grid, dim3(256), 0, st, ud, s, C, HW);
+    hipLaunchKernelGGL(k_norm_scale<double>, grid, dim3(256), 0, st, ud, static_cast<bf16 *>(x), s,
+                       C, HW);
+  } else {
+    const float *uf = static_cast<const float *>(u);
+    hipLaunchKernelGGL(k_norm_max<float>, grid, dim3(256), 0, st, uf, s, C, HW);
+    hipLaunchKernelGGL(k_norm_scale<float>, grid, dim3(256), 0, st, uf, static_cast<bf16 *>(x), s,
+                       C, HW);
+  }
   PML_LAUNCH_CHECK();
   return PML_OK;
 }

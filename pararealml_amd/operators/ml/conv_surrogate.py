@@ -73,13 +73,20 @@ class ConvSurrogate:
         bf = dict(dtype=t.bfloat16, device="cuda")
         self.w32, self.b32, self.mw, self.vw, self.mb, self.vb = [], [], [], [], [], []
         self.gw, self.gb, self.plain, self.wf, self.wb = [], [], [], [], []
+        # all gradients are views of one buffer, cleared by one fill per step
+        sizes = [(9 * self.chans[k] * self.chans[k + 1], self.chans[k + 1]) for k in range(N_LAYERS)]
+        self.gflat = t.zeros(sum(n + co for n, co in sizes), **f32)
+        off = 0
         for layer in range(N_LAYERS):
             ci, co = self.chans[layer], self.chans[layer + 1]
             n = 9 * ci * co
-            for lst, shape in ((self.w32, n), (self.mw, n), (self.vw, n), (self.gw, n)):
+            for lst, shape in ((self.w32, n), (self.mw, n), (self.vw, n)):
                 lst.append(t.zeros(shape, **f32))
-            for lst in (self.b32, self.mb, self.vb, self.gb):
+            for lst in (self.b32, self.mb, self.vb):
                 lst.append(t.zeros(co, **f32))
+            self.gw.append(self.gflat[off:off + n])
+            self.gb.append(self.gflat[off + n:off + n + co])
+            off += n + co
             mid = ci == HIDDEN and co == HIDDEN
             self.plain.append(None if mid else t.zeros(n, **bf))
             self.wf.append(t.zeros(n, **bf) if mid else None)
@@ -224,8 +231,7 @@ class ConvSurrogate:
         nat.check(lib.pml_surrogate_normalize(
             nat.PML_F32, u.data_ptr(), ws["x"].data_ptr(), ws["s"].data_ptr(), batch, c, h * w, sp))
         self._forward(ws, batch, h, w, store_z=True)
-        for g in self.gw + self.gb:
-            g.zero_()
+        self.gflat.zero_()
         ws["loss"].zero_()
         nat.check(lib.pml_surrogate_loss(
             u.data_ptr(), ws["y"].data_ptr(), ws["s"].data_ptr(), target.data_ptr(),
