@@ -151,10 +151,20 @@ class ConvSurrogate:
         return net
 
     # ------------------------------------------------------------ workspaces
-    def _workspace(self, batch: int, h: int, w: int, training: bool):
-        key = (batch, h, w, training)
+    def _workspace(self, batch: int, h: int, w: int, training: bool, stream=None):
+        st = stream if stream is not None else self.stream
+        key = (batch, h, w, training, st.cuda_stream)
         if key in self._ws_cache:
             return self._ws_cache[key]
+        t = self.torch
+        with t.cuda.stream(st):  # the buffers belong to the stream that uses them
+            ws = self._alloc_workspace(batch, h, w, training)
+        if len(self._ws_cache) >= 4:
+            self._ws_cache.clear()
+        self._ws_cache[key] = ws
+        return ws
+
+    def _alloc_workspace(self, batch: int, h: int, w: int, training: bool):
         t = self.torch
         px = batch * h * w
         bf = dict(dtype=t.bfloat16, device="cuda")
@@ -171,14 +181,11 @@ class ConvSurrogate:
             ws["dy"] = t.empty(px * c, **bf)
             ws["dz"] = [t.empty(px * HIDDEN, **bf) for _ in range(2)]
             ws["loss"] = t.zeros(1, dtype=t.float64, device="cuda")
-        if len(self._ws_cache) >= 4:
-            self._ws_cache.clear()
-        self._ws_cache[key] = ws
         return ws
 
     # --------------------------------------------------------------- forward
-    def _forward(self, ws, batch, h, w, store_z: bool):
-        lib, sp = self.lib, _sp(self.stream)
+    def _forward(self, ws, batch, h, w, store_z: bool, stream=None):
+        lib, sp = self.lib, _sp(stream if stream is not None else self.stream)
         c = self.n_fields
         z = ws.get("z") if store_z else None
         # layer 0: C -> 64
@@ -196,26 +203,38 @@ class ConvSurrogate:
             self.b32[N_LAYERS - 1].data_ptr(), None, None, None, ws["y"].data_ptr(), HIDDEN,
             c, batch, h, w, sp))
 
-    def predict(self, u, out=None, steps: int = 1):
+    def predict(self, u, out=None, steps: int = 1, stream=None):
         """Applies the network ``steps`` times to field-major states
-        ``u`` (B, C, H, W), FP32 or FP64; returns a tensor of u's dtype."""
+        ``u`` (B, C, H, W), FP32 or FP64; returns a tensor of u's dtype.
+
+        Runs on ``stream`` (default: the surrogate's own stream), which must
+        be the stream that produced ``u`` and consumes the result; it first
+        waits for the surrogate's stream (where the parameters are updated),
+        and the surrogate's stream waits for it afterwards."""
         t = self.torch
         batch, c, h, w = u.shape
         if c != self.n_fields:
             raise ValueError(f"expected {self.n_fields} fields, got {c}")
         prec = nat.PML_F64 if u.dtype == t.float64 else nat.PML_F32
-        ws = self._workspace(batch, h, w, training=False)
-        cur = u.contiguous()
-        res = out if out is not None else t.empty_like(cur)
-        sp = _sp(self.stream)
+        st = stream if stream is not None else self.stream
+        other = st.cuda_stream != self.stream.cuda_stream
+        if other:
+            st.wait_stream(self.stream)
+        ws = self._workspace(batch, h, w, training=False, stream=st)
+        with t.cuda.stream(st):
+            cur = u.contiguous()
+            res = out if out is not None else t.empty_like(cur)
+        sp = _sp(st)
         for k in range(steps):
             nat.check(self.lib.pml_surrogate_normalize(
                 prec, cur.data_ptr(), ws["x"].data_ptr(), ws["s"].data_ptr(), batch, c, h * w, sp))
-            self._forward(ws, batch, h, w, store_z=False)
+            self._forward(ws, batch, h, w, store_z=False, stream=st)
             nat.check(self.lib.pml_surrogate_residual(
                 prec, cur.data_ptr(), ws["y"].data_ptr(), ws["s"].data_ptr(),
                 res.data_ptr(), batch, c, h * w, sp))
             cur = res
+        if other:
+            self.stream.wait_stream(st)
         return res
 
     # -------------------------------------------------------------- training

@@ -99,34 +99,37 @@ class LearnedPropagator:
         torch = self.torch
         sp = ctypes.c_void_p(self.stream.cuda_stream)
         net = self.op.surrogate
-        for k in range(last - first):
-            src = u_f64[k]
-            if self.op.coarsen:
-                nat.check(self.lib.pml_restrict2x(
-                    nat.PML_F64, src.data_ptr(), self.buf.data_ptr(), self.work.n_fields,
-                    self.work.n0, self.work.n1, sp))
-            else:
-                self.buf.view(-1).copy_(src)
-            self._dirichlet(self.buf, self.work, self.bc_work)
-            for _ in range(self.steps):
-                net.predict(self.buf, out=self.buf)
+        # every torch operation below (copies, the fallback test) runs on
+        # the propagator's stream, in order with its kernels
+        with torch.cuda.stream(self.stream):
+            for k in range(last - first):
+                src = u_f64[k]
+                if self.op.coarsen:
+                    nat.check(self.lib.pml_restrict2x(
+                        nat.PML_F64, src.data_ptr(), self.buf.data_ptr(), self.work.n_fields,
+                        self.work.n0, self.work.n1, sp))
+                else:
+                    self.buf.view(-1).copy_(src)
                 self._dirichlet(self.buf, self.work, self.bc_work)
-            dst = out_f64[k]
-            if self.op.coarsen:
-                nat.check(self.lib.pml_prolong2x(
-                    nat.PML_F64, self.buf.data_ptr(), dst.data_ptr(), self.work.n_fields,
-                    self.work.n0, self.work.n1, sp))
-                self._dirichlet(dst, self.fine, self.bc_fine)
-            else:
-                dst.copy_(self.buf.view(-1))
-            if self.fallback is not None:
-                bad = (~torch.isfinite(dst)).any() | (
-                    dst.abs().max() > self.op.guard * src.abs().max().clamp_min(1e-300)
-                )
-                if bool(bad.item()):
-                    self.op.fallbacks += 1
-                    self.fallback.run(u_f64[k : k + 1], first + k, first + k + 1,
-                                      out_f64[k : k + 1])
+                for _ in range(self.steps):
+                    net.predict(self.buf, out=self.buf, stream=self.stream)
+                    self._dirichlet(self.buf, self.work, self.bc_work)
+                dst = out_f64[k]
+                if self.op.coarsen:
+                    nat.check(self.lib.pml_prolong2x(
+                        nat.PML_F64, self.buf.data_ptr(), dst.data_ptr(), self.work.n_fields,
+                        self.work.n0, self.work.n1, sp))
+                    self._dirichlet(dst, self.fine, self.bc_fine)
+                else:
+                    dst.copy_(self.buf.view(-1))
+                if self.fallback is not None:
+                    bad = (~torch.isfinite(dst)).any() | (
+                        dst.abs().max() > self.op.guard * src.abs().max().clamp_min(1e-300)
+                    )
+                    if bool(bad.item()):
+                        self.op.fallbacks += 1
+                        self.fallback.run(u_f64[k : k + 1], first + k, first + k + 1,
+                                          out_f64[k : k + 1])
 
 
 class CorrectedCoarseOperator:
@@ -147,10 +150,20 @@ class CorrectedCoarseOperator:
     operator's response to perturbations (what sets Parareal's convergence)
     keeps their 2x-grid content; a 4x base discards it.  A
     non-finite slice, or one whose magnitude exceeds ``guard`` times its
-    input's, is recomputed by ``fallback`` and counted in ``fallbacks``."""
+    input's, is recomputed by ``fallback`` and counted in ``fallbacks``.
+
+    With ``freeze`` (the default) each slice's correction (the corrected
+    minus the base prediction on the 2x grid) is evaluated once, at the
+    slice's first coarse evaluation (Parareal's initial sweep), and reused:
+    later evaluations return the base prediction plus that fixed field.  The
+    operator is then affine per slice and the correction cancels in
+    Parareal's G(U_{k+1}) - G(U_k), so the iteration contracts like the base
+    solve from the network-corrected initial guess.  A live BF16 network in
+    every evaluation does not cancel below its rounding level (about 1e-5
+    relative here) and stalls the iteration there."""
 
     def __init__(self, base, surrogate: ConvSurrogate, fallback=None, guard: float = 10.0,
-                 base_levels: int = 2):
+                 base_levels: int = 2, freeze: bool = True):
         if base_levels not in (1, 2):
             raise ValueError("the base solve runs on the 2x (1) or 4x (2) coarsened grid")
         self.base = base
@@ -158,6 +171,7 @@ class CorrectedCoarseOperator:
         self.fallback = fallback
         self.guard = guard
         self.base_levels = base_levels
+        self.freeze = freeze
         self.fallbacks = 0
 
     @property
@@ -174,11 +188,12 @@ class CorrectedCoarseOperator:
             fb = self.fallback.slice_propagator(
                 cp, n_slices, int(round(slice_steps * self.d_t / self.fallback.d_t)), stream
             )
-        return CorrectedPropagator(self, cp, slice_steps, stream, fb)
+        return CorrectedPropagator(self, cp, n_slices, slice_steps, stream, fb)
 
 
 class CorrectedPropagator:
-    def __init__(self, op: CorrectedCoarseOperator, cp, steps: int, stream, fallback_prop):
+    def __init__(self, op: CorrectedCoarseOperator, cp, n_slices: int, steps: int, stream,
+                 fallback_prop):
         import torch
 
         from pararealml_amd.operators.parareal.propagators import FDMSlicePropagator
@@ -200,6 +215,12 @@ class CorrectedPropagator:
         self.b_in = torch.empty((1, self.basep.elems), **f64)
         self.b_out = torch.empty((1, self.basep.elems), **f64)
         self.lib = nat.device_lib()
+        # frozen per-slice corrections on the 2x grid (see CorrectedCoarseOperator)
+        self.corr = self.base_pred = None
+        if op.freeze:
+            self.corr = torch.zeros((n_slices, self.work.elems), **f64)
+            self.base_pred = torch.empty(self.work.elems, **f64)
+            self.have = np.zeros(n_slices, dtype=bool)
 
     def _dirichlet(self, t, p, bc):
         nat.check(self.lib.pml_apply_dirichlet(
@@ -210,35 +231,48 @@ class CorrectedPropagator:
         torch = self.torch
         sp = ctypes.c_void_p(self.stream.cuda_stream)
         w, b = self.work, self.basep
-        for k in range(last - first):
-            src = u_f64[k]
-            nat.check(self.lib.pml_restrict2x(nat.PML_F64, src.data_ptr(), self.buf.data_ptr(),
-                                              w.n_fields, w.n0, w.n1, sp))
-            self._dirichlet(self.buf, w, self.bc_work)
-            if self.op.base_levels == 2:
-                nat.check(self.lib.pml_restrict2x(nat.PML_F64, self.buf.data_ptr(),
-                                                  self.b_in.data_ptr(), b.n_fields, b.n0, b.n1,
-                                                  sp))
-                self.inner.run(self.b_in, 0, 1, self.b_out)
-                nat.check(self.lib.pml_prolong2x(nat.PML_F64, self.b_out.data_ptr(),
-                                                 self.buf.data_ptr(), b.n_fields, b.n0, b.n1, sp))
-            else:
-                self.inner.run(self.buf.view(1, -1), 0, 1, self.buf.view(1, -1))
-            self._dirichlet(self.buf, w, self.bc_work)
-            self.op.surrogate.predict(self.buf, out=self.buf)
-            self._dirichlet(self.buf, w, self.bc_work)
-            dst = out_f64[k]
-            nat.check(self.lib.pml_prolong2x(nat.PML_F64, self.buf.data_ptr(), dst.data_ptr(),
-                                             w.n_fields, w.n0, w.n1, sp))
-            self._dirichlet(dst, self.fine, self.bc_fine)
-            if self.fallback is not None:
-                bad = (~torch.isfinite(dst)).any() | (
-                    dst.abs().max() > self.op.guard * src.abs().max().clamp_min(1e-300)
-                )
-                if bool(bad.item()):
-                    self.op.fallbacks += 1
-                    self.fallback.run(u_f64[k : k + 1], first + k, first + k + 1,
-                                      out_f64[k : k + 1])
+        # every torch operation below (the fallback test) runs on the
+        # propagator's stream, in order with its kernels
+        with torch.cuda.stream(self.stream):
+            for k in range(last - first):
+                src = u_f64[k]
+                nat.check(self.lib.pml_restrict2x(nat.PML_F64, src.data_ptr(), self.buf.data_ptr(),
+                                                  w.n_fields, w.n0, w.n1, sp))
+                self._dirichlet(self.buf, w, self.bc_work)
+                if self.op.base_levels == 2:
+                    nat.check(self.lib.pml_restrict2x(nat.PML_F64, self.buf.data_ptr(),
+                                                      self.b_in.data_ptr(), b.n_fields, b.n0, b.n1,
+                                                      sp))
+                    self.inner.run(self.b_in, 0, 1, self.b_out)
+                    nat.check(self.lib.pml_prolong2x(nat.PML_F64, self.b_out.data_ptr(),
+                                                     self.buf.data_ptr(), b.n_fields, b.n0, b.n1, sp))
+                else:
+                    self.inner.run(self.buf.view(1, -1), 0, 1, self.buf.view(1, -1))
+                self._dirichlet(self.buf, w, self.bc_work)
+                n = first + k
+                if self.corr is None:
+                    self.op.surrogate.predict(self.buf, out=self.buf, stream=self.stream)
+                    self._dirichlet(self.buf, w, self.bc_work)
+                elif not self.have[n]:
+                    self.base_pred.copy_(self.buf.view(-1))
+                    self.op.surrogate.predict(self.buf, out=self.buf, stream=self.stream)
+                    self._dirichlet(self.buf, w, self.bc_work)
+                    torch.sub(self.buf.view(-1), self.base_pred, out=self.corr[n])
+                    self.have[n] = True
+                else:
+                    self.buf.view(-1).add_(self.corr[n])
+                dst = out_f64[k]
+                nat.check(self.lib.pml_prolong2x(nat.PML_F64, self.buf.data_ptr(), dst.data_ptr(),
+                                                 w.n_fields, w.n0, w.n1, sp))
+                self._dirichlet(dst, self.fine, self.bc_fine)
+                if self.fallback is not None:
+                    bad = (~torch.isfinite(dst)).any() | (
+                        dst.abs().max() > self.op.guard * src.abs().max().clamp_min(1e-300)
+                    )
+                    if bool(bad.item()):
+                        self.op.fallbacks += 1
+                        self.fallback.run(u_f64[k : k + 1], first + k, first + k + 1,
+                                          out_f64[k : k + 1])
 
 
 def corrected_pairs(base_op, cp, snaps, slice_dt: float, stream=None, base_levels: int = 2):

@@ -293,6 +293,65 @@ def test_corrected_coarse_with_identity_network_is_the_coarse_grid_fdm():
         CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=3)
 
 
+def test_corrected_propagator_applies_the_network_in_stream_order():
+    """A network whose output is the constant beta (zero weights, last bias
+    beta) turns the corrected prediction into the base prediction plus
+    s * beta, s = max |base prediction|, at every point whose prolongation
+    stencil is interior.  The surrogate is made on the default stream and
+    the propagator runs on the solver stream: the correction has to land
+    between the base solve and the prolongation."""
+    import torch
+
+    import pararealml_amd as pml
+    from pararealml_amd import _device as dev
+    from pararealml_amd.operators.fdm import (
+        FDMOperator,
+        ForwardEulerMethod,
+        ThreePointCentralDifferenceMethod,
+    )
+    from pararealml_amd.operators.ml import ConvSurrogate, CorrectedCoarseOperator
+
+    print("case: corrected propagator stream order", flush=True)
+    diff = ThreePointCentralDifferenceMethod()
+    cp = _diffusion_problem(64)
+    base = FDMOperator(ForwardEulerMethod(), diff, 2e-4)
+    beta = float(np.float32(0.01))
+    net = ConvSurrogate(1)
+    params = [(np.zeros((3, 3, ci, co)), np.zeros(co))
+              for ci, co in zip(net.chans[:-1], net.chans[1:])]
+    params[-1] = (params[-1][0], np.full(1, beta))
+    net.set_params(params)
+    stream = dev.solver_stream()
+    p_net = CorrectedCoarseOperator(base, net, base_levels=1).slice_propagator(cp, 1, 50, stream)
+    p_id = CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=1).slice_propagator(
+        cp, 1, 50, stream)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.5, 0.45]), 0.01 * np.eye(2))])
+    y0 = ic.discrete_y_0(True)
+    u = torch.tensor(np.moveaxis(y0, -1, 0).reshape(1, -1), dtype=torch.float64, device="cuda")
+    out_net = torch.zeros_like(u)
+    out_id = torch.zeros_like(u)
+    torch.cuda.synchronize()
+    for _ in range(3):  # repeated calls reuse the workspaces
+        p_net.run(u, 0, 1, out_net)
+        p_id.run(u, 0, 1, out_id)
+    torch.cuda.synchronize()
+    y_net = out_net.cpu().numpy().reshape(65, 65)
+    y_id = out_id.cpu().numpy().reshape(65, 65)
+    # the maximum of the 2x-grid state survives the prolongation
+    s = float(np.float32(np.abs(y_id).max()))
+    np.testing.assert_allclose(y_net[2:-2, 2:-2] - y_id[2:-2, 2:-2], s * beta, rtol=0, atol=1e-12)
+    # The correction was frozen at the first call: a perturbed input moves
+    # the output exactly as it moves the base solve's.
+    du = 1e-3 * torch.roll(u, 7, dims=1)
+    out_net2 = torch.zeros_like(u)
+    out_id2 = torch.zeros_like(u)
+    p_net.run(u + du, 0, 1, out_net2)
+    p_id.run(u + du, 0, 1, out_id2)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose((out_net2 - out_net).cpu().numpy(),
+                               (out_id2 - out_id).cpu().numpy(), rtol=0, atol=1e-13)
+
+
 def test_trained_correction_reduces_the_coarse_error_on_held_out_states():
     """What the network gains: trained on (cheap base prediction, fine
     solution) pairs, the corrected prediction over one slice is closer to
