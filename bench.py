@@ -57,6 +57,10 @@ def parse_args():
                     help="base solve on the 2x (1) or 4x (2) coarsened grid")
     ap.add_argument("--pr-traj", type=int, default=256, help="training trajectories")
     ap.add_argument("--pr-train-steps", type=int, default=3000)
+    ap.add_argument("--pr-train-batch", type=int, default=0,
+                    help="samples per training step (0: 8 on the fine grid, 32 on the 2x grid)")
+    ap.add_argument("--pr-correct-on", default="fine", choices=["fine", "coarse"],
+                    help="grid the learned correction acts on")
     ap.add_argument("--pr-target", type=float, default=1e-6)
     ap.add_argument("--pr-fine-precision", default="float32c")
     ap.add_argument("--pr-no-learned", action="store_true",
@@ -284,7 +288,9 @@ def parareal_workload(args):
                 f"{args.pr_coarse_steps} steps/slice on the 2x grid"
                 + ("" if args.pr_no_learned else
                    f", learned coarse = FE {args.pr_base_steps} steps/slice on the "
-                   f"{2 ** args.pr_base_levels}x grid + one conv-surrogate correction on the 2x grid")
+                   f"{2 ** args.pr_base_levels}x grid + one conv-surrogate correction on the "
+                   + ("2x grid" if args.pr_correct_on == "coarse" else "fine grid after the "
+                      "prolongation") + ", evaluated in the initial sweep and frozen")
             ),
             "target_rel_error": target,
         },
@@ -301,6 +307,7 @@ def parareal_workload(args):
                 "pairs": n_pairs,
                 "data_generation_s": t_data,
                 "train_steps": args.pr_train_steps,
+                "train_batch": args.pr_train_batch or (8 if args.pr_correct_on == "fine" else 32),
                 "train_s": t_train,
                 "loss_first": float(np.mean(losses[:10])),
                 "loss_last": float(np.mean(losses[-10:])),
@@ -332,6 +339,7 @@ def _learned_coarse_runs(args, cp, diff, n, S, g_fdm, run_parareal, log):
     # mixtures, one slice apart, restricted to the 2x grid.
     delta = 1.0 / S
     lv = args.pr_base_levels
+    on = args.pr_correct_on
     base = FDMOperator(ForwardEulerMethod(), diff, delta / args.pr_base_steps)
     base_cp = coarse_problem(cp) if lv == 1 else coarse_problem(coarse_problem(cp))
     base.tune(base_cp, 1)
@@ -341,8 +349,8 @@ def _learned_coarse_runs(args, cp, diff, n, S, g_fdm, run_parareal, log):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     y0s = gaussian_mixture_states(cp, args.pr_traj, seed=SEEDS[2])
-    snaps = generate_pairs(data_op, cp, y0s, 1.0, delta, coarsen=True)
-    inputs, targets = corrected_pairs(base, cp, snaps, delta, base_levels=lv)
+    snaps = generate_pairs(data_op, cp, y0s, 1.0, delta, coarsen=on == "coarse")
+    inputs, targets = corrected_pairs(base, cp, snaps, delta, base_levels=lv, correct_on=on)
     del snaps
     torch.cuda.synchronize()
     t_data = time.perf_counter() - t0
@@ -350,19 +358,20 @@ def _learned_coarse_runs(args, cp, diff, n, S, g_fdm, run_parareal, log):
     log(f"parareal: training on {inputs.shape[0]} pairs")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    losses = net.fit(inputs, targets, steps=args.pr_train_steps, batch=32, lr=1e-3,
+    batch = args.pr_train_batch or (8 if on == "fine" else 32)
+    losses = net.fit(inputs, targets, steps=args.pr_train_steps, batch=batch, lr=1e-3,
                      log_every=max(1, args.pr_train_steps // 10))
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     n_pairs = int(inputs.shape[0])
     del inputs, targets
     torch.cuda.empty_cache()
-    g_learned = CorrectedCoarseOperator(base, net, fallback=g_fdm, base_levels=lv)
+    g_learned = CorrectedCoarseOperator(base, net, fallback=g_fdm, base_levels=lv, correct_on=on)
     learned_res = run_parareal(g_learned, "learned coarse")
     # The same pipeline without the learned correction (an untrained
     # surrogate is exactly the identity): what the network gains.
     g_plain = CorrectedCoarseOperator(base, ConvSurrogate(1, seed=0), fallback=g_fdm,
-                                      base_levels=lv)
+                                      base_levels=lv, correct_on=on)
     plain_res = run_parareal(g_plain, "cheap coarse without the network")
     return learned_res, plain_res, n_pairs, t_data, t_train, losses, g_learned
 

@@ -133,8 +133,11 @@ class LearnedPropagator:
 
 
 class CorrectedCoarseOperator:
-    """Parareal coarse operator: a cheap FDM solve corrected by one surrogate
-    application per slice on the 2x-coarsened grid.
+    """Parareal coarse operator: a cheap FDM solve on the 2x-coarsened grid
+    corrected by one surrogate application per slice, on the 2x grid
+    (``correct_on = "coarse"``) or on the problem's own grid after the
+    prolongation (``correct_on = "fine"``, where the network also supplies
+    the detail that the 2x grid cannot carry).
 
     Per slice: restrict the state to the 2x grid (and, with
     ``base_levels = 2``, again to the 4x grid), advance it over the slice with
@@ -163,9 +166,13 @@ class CorrectedCoarseOperator:
     relative here) and stalls the iteration there."""
 
     def __init__(self, base, surrogate: ConvSurrogate, fallback=None, guard: float = 10.0,
-                 base_levels: int = 2, freeze: bool = True):
+                 base_levels: int = 2, freeze: bool = True, correct_on: str = "coarse"):
         if base_levels not in (1, 2):
             raise ValueError("the base solve runs on the 2x (1) or 4x (2) coarsened grid")
+        if correct_on not in ("coarse", "fine"):
+            raise ValueError("the correction acts on the 2x grid ('coarse') or the problem's "
+                             "grid ('fine')")
+        self.correct_on = correct_on
         self.base = base
         self.surrogate = surrogate
         self.fallback = fallback
@@ -215,17 +222,35 @@ class CorrectedPropagator:
         self.b_in = torch.empty((1, self.basep.elems), **f64)
         self.b_out = torch.empty((1, self.basep.elems), **f64)
         self.lib = nat.device_lib()
-        # frozen per-slice corrections on the 2x grid (see CorrectedCoarseOperator)
+        # frozen per-slice corrections on the corrected grid (see
+        # CorrectedCoarseOperator)
+        self.on_fine = op.correct_on == "fine"
+        elems = self.fine.elems if self.on_fine else self.work.elems
         self.corr = self.base_pred = None
         if op.freeze:
-            self.corr = torch.zeros((n_slices, self.work.elems), **f64)
-            self.base_pred = torch.empty(self.work.elems, **f64)
+            self.corr = torch.zeros((n_slices, elems), **f64)
+            self.base_pred = torch.empty(elems, **f64)
             self.have = np.zeros(n_slices, dtype=bool)
 
     def _dirichlet(self, t, p, bc):
         nat.check(self.lib.pml_apply_dirichlet(
             t.data_ptr(), bc.data_ptr(), p.n_fields, p.n0, p.n1, p.ndim, 1,
             ctypes.c_void_p(self.stream.cuda_stream)))
+
+    def _correct(self, x, n: int, p, bc):
+        """Applies slice n's correction to x ((1, C, H, W), FP64, Dirichlet
+        values imposed): the network, or its frozen result."""
+        if self.corr is None:
+            self.op.surrogate.predict(x, out=x, stream=self.stream)
+            self._dirichlet(x, p, bc)
+        elif not self.have[n]:
+            self.base_pred.copy_(x.view(-1))
+            self.op.surrogate.predict(x, out=x, stream=self.stream)
+            self._dirichlet(x, p, bc)
+            self.torch.sub(x.view(-1), self.base_pred, out=self.corr[n])
+            self.have[n] = True
+        else:
+            x.view(-1).add_(self.corr[n])
 
     def run(self, u_f64, first: int, last: int, out_f64):
         torch = self.torch
@@ -249,22 +274,16 @@ class CorrectedPropagator:
                 else:
                     self.inner.run(self.buf.view(1, -1), 0, 1, self.buf.view(1, -1))
                 self._dirichlet(self.buf, w, self.bc_work)
-                n = first + k
-                if self.corr is None:
-                    self.op.surrogate.predict(self.buf, out=self.buf, stream=self.stream)
-                    self._dirichlet(self.buf, w, self.bc_work)
-                elif not self.have[n]:
-                    self.base_pred.copy_(self.buf.view(-1))
-                    self.op.surrogate.predict(self.buf, out=self.buf, stream=self.stream)
-                    self._dirichlet(self.buf, w, self.bc_work)
-                    torch.sub(self.buf.view(-1), self.base_pred, out=self.corr[n])
-                    self.have[n] = True
-                else:
-                    self.buf.view(-1).add_(self.corr[n])
                 dst = out_f64[k]
+                if not self.on_fine:
+                    self._correct(self.buf, first + k, w, self.bc_work)
                 nat.check(self.lib.pml_prolong2x(nat.PML_F64, self.buf.data_ptr(), dst.data_ptr(),
                                                  w.n_fields, w.n0, w.n1, sp))
                 self._dirichlet(dst, self.fine, self.bc_fine)
+                if self.on_fine:
+                    fp = self.fine
+                    self._correct(dst.view(1, fp.n_fields, fp.n0, fp.n1), first + k, fp,
+                                  self.bc_fine)
                 if self.fallback is not None:
                     bad = (~torch.isfinite(dst)).any() | (
                         dst.abs().max() > self.op.guard * src.abs().max().clamp_min(1e-300)
@@ -275,21 +294,26 @@ class CorrectedPropagator:
                                           out_f64[k : k + 1])
 
 
-def corrected_pairs(base_op, cp, snaps, slice_dt: float, stream=None, base_levels: int = 2):
-    """Training pairs for ``CorrectedCoarseOperator`` from trajectories on
-    the 2x-coarsened grid (``snaps``: FP32 device tensor (T, N, C, H, W),
-    states one slice apart, as ``generate_pairs`` returns them): the input
-    is the base operator's prediction over one slice from state t (on the 2x
-    grid, or restricted to the 4x grid and prolonged back for
-    ``base_levels = 2``; Dirichlet values imposed), the target is state
-    t + 1.  Returns (inputs, targets), each ((T-1) N, C, H,
-    W) FP32."""
+def corrected_pairs(base_op, cp, snaps, slice_dt: float, stream=None, base_levels: int = 2,
+                    correct_on: str = "coarse"):
+    """Training pairs for ``CorrectedCoarseOperator`` from trajectories
+    (``snaps``: FP32 device tensor (T, N, C, H, W), states one slice apart,
+    as ``generate_pairs`` returns them: on the 2x-coarsened grid for
+    ``correct_on = "coarse"``, on the problem's grid for ``"fine"``).  The
+    input is the base operator's prediction over one slice from state t (run
+    on the 2x grid, or restricted to the 4x grid and prolonged back for
+    ``base_levels = 2``; for ``"fine"`` the state is first restricted to the
+    2x grid and the prediction prolonged to the problem's grid; Dirichlet
+    values imposed on every grid), the target is state t + 1.  Returns
+    (inputs, targets), each ((T-1) N, C, H, W) FP32."""
     import torch
 
     stream = stream if stream is not None else dev.solver_stream()
     n_t, n, c, h, w = snaps.shape
+    on_fine = correct_on == "fine"
     work_cp = coarse_problem(cp)
     wp = dev.device_problem(work_cp)
+    fp = dev.device_problem(cp)
     base_cp = coarse_problem(work_cp) if base_levels == 2 else work_cp
     bp = base_op.device_problem(base_cp)
     steps = int(round(slice_dt / base_op.d_t))
@@ -301,10 +325,16 @@ def corrected_pairs(base_op, cp, snaps, slice_dt: float, stream=None, base_level
         batch = dev.FDMBatch([bp], [0] * n, base_op.precision,
                              dev.integrator_id(base_op.integrator), stream)
         bc_work = torch.from_numpy(wp.bc).to("cuda")
+        bc_fine = torch.from_numpy(fp.bc).to("cuda")
         x4 = torch.empty(n * bp.elems, dtype=torch.float64, device="cuda")
-        y2 = torch.empty((n * c, h, w), dtype=torch.float64, device="cuda")
+        y2 = torch.empty((n * c, wp.n0, wp.n1), dtype=torch.float64, device="cuda")
+        yf = torch.empty((n * c, fp.n0, fp.n1), dtype=torch.float64, device="cuda")
         for t in range(n_t - 1):
             x2 = snaps[t].to(torch.float64).contiguous()
+            if on_fine:
+                nat.check(lib.pml_restrict2x(nat.PML_F64, x2.data_ptr(), y2.data_ptr(), n * c,
+                                             wp.n0, wp.n1, sp))
+                x2 = y2.clone()
             if base_levels == 2:
                 nat.check(lib.pml_restrict2x(nat.PML_F64, x2.data_ptr(), x4.data_ptr(), n * c,
                                              bp.n0, bp.n1, sp))
@@ -321,7 +351,14 @@ def corrected_pairs(base_op, cp, snaps, slice_dt: float, stream=None, base_level
                 y2.view(-1).copy_(y4)
             nat.check(lib.pml_apply_dirichlet(y2.data_ptr(), bc_work.data_ptr(), c, wp.n0,
                                               wp.n1, wp.ndim, n, sp))
-            inputs[t].copy_(y2.view(n, c, h, w))
+            if on_fine:
+                nat.check(lib.pml_prolong2x(nat.PML_F64, y2.data_ptr(), yf.data_ptr(), n * c,
+                                            wp.n0, wp.n1, sp))
+                nat.check(lib.pml_apply_dirichlet(yf.data_ptr(), bc_fine.data_ptr(), c, fp.n0,
+                                                  fp.n1, fp.ndim, n, sp))
+                inputs[t].copy_(yf.view(n, c, h, w))
+            else:
+                inputs[t].copy_(y2.view(n, c, h, w))
     torch.cuda.current_stream().wait_stream(stream)
     return inputs.reshape(-1, c, h, w), snaps[1:].reshape(-1, c, h, w)
 

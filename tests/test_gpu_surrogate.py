@@ -293,7 +293,8 @@ def test_corrected_coarse_with_identity_network_is_the_coarse_grid_fdm():
         CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=3)
 
 
-def test_corrected_propagator_applies_the_network_in_stream_order():
+@pytest.mark.parametrize("correct_on", ["coarse", "fine"])
+def test_corrected_propagator_applies_the_network_in_stream_order(correct_on):
     """A network whose output is the constant beta (zero weights, last bias
     beta) turns the corrected prediction into the base prediction plus
     s * beta, s = max |base prediction|, at every point whose prolongation
@@ -311,7 +312,7 @@ def test_corrected_propagator_applies_the_network_in_stream_order():
     )
     from pararealml_amd.operators.ml import ConvSurrogate, CorrectedCoarseOperator
 
-    print("case: corrected propagator stream order", flush=True)
+    print(f"case: corrected propagator stream order ({correct_on})", flush=True)
     diff = ThreePointCentralDifferenceMethod()
     cp = _diffusion_problem(64)
     base = FDMOperator(ForwardEulerMethod(), diff, 2e-4)
@@ -322,9 +323,10 @@ def test_corrected_propagator_applies_the_network_in_stream_order():
     params[-1] = (params[-1][0], np.full(1, beta))
     net.set_params(params)
     stream = dev.solver_stream()
-    p_net = CorrectedCoarseOperator(base, net, base_levels=1).slice_propagator(cp, 1, 50, stream)
-    p_id = CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=1).slice_propagator(
-        cp, 1, 50, stream)
+    p_net = CorrectedCoarseOperator(base, net, base_levels=1, correct_on=correct_on
+                                    ).slice_propagator(cp, 1, 50, stream)
+    p_id = CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=1, correct_on=correct_on
+                                   ).slice_propagator(cp, 1, 50, stream)
     ic = pml.GaussianInitialCondition(cp, [(np.array([0.5, 0.45]), 0.01 * np.eye(2))])
     y0 = ic.discrete_y_0(True)
     u = torch.tensor(np.moveaxis(y0, -1, 0).reshape(1, -1), dtype=torch.float64, device="cuda")
@@ -352,7 +354,8 @@ def test_corrected_propagator_applies_the_network_in_stream_order():
                                (out_id2 - out_id).cpu().numpy(), rtol=0, atol=1e-13)
 
 
-def test_trained_correction_reduces_the_coarse_error_on_held_out_states():
+@pytest.mark.parametrize("correct_on", ["coarse", "fine"])
+def test_trained_correction_reduces_the_coarse_error_on_held_out_states(correct_on):
     """What the network gains: trained on (cheap base prediction, fine
     solution) pairs, the corrected prediction over one slice is closer to
     the fine solution than the base prediction alone (at least 2x), on
@@ -372,7 +375,7 @@ def test_trained_correction_reduces_the_coarse_error_on_held_out_states():
         generate_pairs,
     )
 
-    print("case: corrected coarse gain", flush=True)
+    print(f"case: corrected coarse gain ({correct_on})", flush=True)
     n, slices = 64, 16
     diff = ThreePointCentralDifferenceMethod()
     cp = _diffusion_problem(n)
@@ -381,13 +384,16 @@ def test_trained_correction_reduces_the_coarse_error_on_held_out_states():
     # RK4 at 90% of its stability limit for the 2-D diffusion stencil
     data_steps = int(np.ceil(delta / (0.9 * 2.78 * (1.0 / n) ** 2 / (8 * 0.5))))
     data_op = FDMOperator(RK4(), diff, delta / data_steps, precision="float32")
-    train = generate_pairs(data_op, cp, gaussian_mixture_states(cp, 64, seed=1), 1.0, delta)
-    inp, tgt = corrected_pairs(base, cp, train, delta)
+    coarsen = correct_on == "coarse"
+    train = generate_pairs(data_op, cp, gaussian_mixture_states(cp, 64, seed=1), 1.0, delta,
+                           coarsen=coarsen)
+    inp, tgt = corrected_pairs(base, cp, train, delta, correct_on=correct_on)
     net = ConvSurrogate(1, seed=0)
     losses = net.fit(inp, tgt, steps=1500, batch=32, lr=1e-3)
     assert np.mean(losses[-20:]) < 0.05 * np.mean(losses[:5])
-    held = generate_pairs(data_op, cp, gaussian_mixture_states(cp, 4, seed=2), 0.5, delta)
-    h_in, h_tg = corrected_pairs(base, cp, held, delta)
+    held = generate_pairs(data_op, cp, gaussian_mixture_states(cp, 4, seed=2), 0.5, delta,
+                          coarsen=coarsen)
+    h_in, h_tg = corrected_pairs(base, cp, held, delta, correct_on=correct_on)
     x, t = h_in.double(), h_tg.double()
     pred = net.predict(x.clone())
     err_base = float(torch.linalg.norm(x - t) / torch.linalg.norm(t))
