@@ -61,6 +61,9 @@ def parse_args():
                     help="samples per training step (0: 8 on the fine grid, 32 on the 2x grid)")
     ap.add_argument("--pr-correct-on", default="fine", choices=["fine", "coarse"],
                     help="grid the learned correction acts on")
+    ap.add_argument("--pr-live", action="store_true",
+                    help="apply the network in every coarse evaluation instead of freezing "
+                         "each slice's correction after the initial sweep")
     ap.add_argument("--pr-target", type=float, default=1e-6)
     ap.add_argument("--pr-fine-precision", default="float32c")
     ap.add_argument("--pr-no-learned", action="store_true",
@@ -290,7 +293,9 @@ def parareal_workload(args):
                    f", learned coarse = FE {args.pr_base_steps} steps/slice on the "
                    f"{2 ** args.pr_base_levels}x grid + one conv-surrogate correction on the "
                    + ("2x grid" if args.pr_correct_on == "coarse" else "fine grid after the "
-                      "prolongation") + ", evaluated in the initial sweep and frozen")
+                      "prolongation")
+                   + (", applied in every coarse evaluation" if args.pr_live
+                      else ", evaluated in the initial sweep and frozen"))
             ),
             "target_rel_error": target,
         },
@@ -366,7 +371,8 @@ def _learned_coarse_runs(args, cp, diff, n, S, g_fdm, run_parareal, log):
     n_pairs = int(inputs.shape[0])
     del inputs, targets
     torch.cuda.empty_cache()
-    g_learned = CorrectedCoarseOperator(base, net, fallback=g_fdm, base_levels=lv, correct_on=on)
+    g_learned = CorrectedCoarseOperator(base, net, fallback=g_fdm, base_levels=lv, correct_on=on,
+                                        freeze=not args.pr_live)
     learned_res = run_parareal(g_learned, "learned coarse")
     # The same pipeline without the learned correction (an untrained
     # surrogate is exactly the identity): what the network gains.

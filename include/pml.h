@@ -221,15 +221,18 @@ int pml_prolong2x(int precision, const void *coarse, void *fine,
  *   pml_conv_small      : C -> 64 (C in 1..3) or 64 -> C layers, w =
  *                         [9][ci][co]; forward writes z_out/a_out (64
  *                         outputs) or y_out (FP32, C outputs); mode 1 is the
- *                         input gradient of a 64 -> C layer.
- *   pml_conv_small_wgrad: dw[9][ci][co] and db[co] += gradients (FP32).
+ *                         input gradient of a 64 -> C layer.  A C -> 64
+ *                         layer reads its input and its weights in FP32,
+ *                         a 64 -> C layer in BF16.
+ *   pml_conv_small_wgrad: dw[9][ci][co] and db[co] += gradients (FP32); x is
+ *                         FP32 for a C -> 64 layer, BF16 for 64 -> C.
  *   pml_bias_grad64     : db[64] += sum over pixels of dz.
  *   pml_adam            : Adam step on FP32 masters p/m/v with gradient g
  *                         (step = 1, 2, ...); writes BF16 weights in the
  *                         plain [9][ci][co], forward [9][co][ci] and
  *                         flipped backward layouts (each may be NULL).
  *   pml_surrogate_normalize: per-sample scale s = max|u| and x = u / s,
- *                         u field-major [B][C][HW] FP32/FP64 -> x NHWC.
+ *                         u field-major [B][C][HW] FP32/FP64 -> x NHWC FP32.
  *   pml_surrogate_residual : out = u + s y, u and out field-major FP32/FP64
  *                         (the identity part never passes through BF16;
  *                         out may alias u).

@@ -18,8 +18,10 @@ be checked:
   the normalised increment ``d = (t - u) / s``.
 
 Arithmetic is FP64, with BF16 round-to-nearest-even applied exactly where the
-device stores BF16 (weights, activations, pre-activations, activation
-gradients), so device and oracle differ only by FP32-vs-FP64 accumulation.
+device stores BF16 (weights of layers 1-5, activations, pre-activations,
+activation gradients; the first layer reads the normalised input and its
+weights unrounded, as the device does in FP32), so device and oracle differ
+only by FP32-vs-FP64 accumulation.
 
 Pinning: ``conv3x3`` is checked against a direct sum on tiny inputs and
 against linearity; the backward passes against finite differences of the
@@ -114,7 +116,7 @@ def init_params(c_in: int, c_out: int, seed: int = 0, hidden: int = HIDDEN):
 def forward(params, x, rounding=True):
     """Returns (y, cache) for normalised input x (B, H, W, C)."""
     r = bf16 if rounding else (lambda v: v)
-    a = r(x)
+    a = x  # the first layer reads the normalised input unrounded
     cache = [a]
     zs = []
     for layer, (w, b) in enumerate(params):
@@ -158,7 +160,7 @@ def adam_step(params, grads, state, lr, beta1=0.9, beta2=0.999, eps=1e-8):
     masters, m, v, t = state
     t = t + 1
     new_params, new_masters, new_m, new_v = [], [], [], []
-    for (mw, mb), (gw, gb), (mmw, mmb), (vw, vb) in zip(masters, grads, m, v):
+    for layer, ((mw, mb), (gw, gb), (mmw, mmb), (vw, vb)) in enumerate(zip(masters, grads, m, v)):
         out = []
         for p, g, mm, vv in ((mw, gw, mmw, vw), (mb, gb, mmb, vb)):
             mm = beta1 * mm + (1 - beta1) * g
@@ -171,7 +173,8 @@ def adam_step(params, grads, state, lr, beta1=0.9, beta2=0.999, eps=1e-8):
         new_masters.append((pw, pb))
         new_m.append((mw2, mb2))
         new_v.append((vw2, vb2))
-        new_params.append((bf16(pw), pb))
+        # the first layer computes with its FP32 master weights
+        new_params.append((pw if layer == 0 else bf16(pw), pb))
     return new_params, (new_masters, new_m, new_v, t)
 
 

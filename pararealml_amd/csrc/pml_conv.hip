@@ -16,7 +16,8 @@
 //    accumulated over a persistent pixel-tile loop and added to an FP32
 //    gradient buffer.
 //  * small layers (C_in -> 64 and 64 -> C_out with C <= 3): VALU kernels with
-//    vectorised 64-channel rows (a lane group per pixel).
+//    vectorised 64-channel rows (a lane group per pixel); the first layer
+//    reads the normalised input and its weights in FP32.
 //  * Adam on FP32 masters, writing the BF16 weight layouts the kernels read.
 #include <hip/hip_runtime.h>
 
@@ -466,8 +467,10 @@ __global__ __launch_bounds__(kWgradThreads) void k_wgrad64(WgradArgs a) {
 // which returns zero.  With one field a thread handles two pixels per
 // iteration, all their loads issued before the arithmetic.
 struct SmallArgs {
-  const bf16 *in;      // [B, H, W, Ci]
-  const bf16 *w;       // [9][Ci][Co]
+  const bf16 *in;      // [B, H, W, Ci] (64 -> Co layers)
+  const bf16 *w;       // [9][Ci][Co] (64 -> Co layers)
+  const float *in32;   // [B, H, W, Ci] FP32 (Ci -> 64 layers)
+  const float *w32;    // [9][Ci][64] FP32 (Ci -> 64 layers)
   const float *bias;   // [Co]
   const bf16 *zprev;   // backward: [B, H, W, Ci] pre-activation of the input
   bf16 *z_out;         // forward, Co == 64: pre-activation (may be null)
@@ -499,24 +502,26 @@ __device__ __forceinline__ float bf16_bits(unsigned short u) {
   return __builtin_bit_cast(float, static_cast<unsigned>(u) << 16);
 }
 
-template <int N>
-__device__ __forceinline__ void lds_fill(float *dst, const bf16 *src) {
+template <int N, typename T>
+__device__ __forceinline__ void lds_fill(float *dst, const T *src) {
   for (int i = threadIdx.x; i < N; i += blockDim.x) dst[i] = static_cast<float>(src[i]);
 }
 
 // Forward of a CI -> 64 layer with GELU: 8 lanes per pixel, 8 output
-// channels per lane.
+// channels per lane.  Input and weights are FP32 (the normalised state and
+// the master weights): small differences of the input survive into the
+// first features, which BF16 then rounds relative to their own size.
 template <int CI>
 __global__ __launch_bounds__(256) void k_in_fwd(SmallArgs a) {
   __shared__ float wl[9 * CI * 64];
   __shared__ float bl[64];
-  lds_fill<9 * CI * 64>(wl, a.w);
+  lds_fill<9 * CI * 64>(wl, a.w32);
   if (threadIdx.x < 64) bl[threadIdx.x] = a.bias[threadIdx.x];
   __syncthreads();
   constexpr int NP = CI == 1 ? 2 : 1;  // pixels per iteration
   const int j = threadIdx.x & 7;
   const int P = a.B * a.H * a.W;
-  const __amdgpu_buffer_rsrc_t r_in = conv_rsrc(a.in, P * CI * 2);
+  const __amdgpu_buffer_rsrc_t r_in = conv_rsrc(a.in32, P * CI * 4);
   const __amdgpu_buffer_rsrc_t r_z = conv_rsrc(a.z_out, a.z_out ? P * 128 : 0);
   const __amdgpu_buffer_rsrc_t r_a = conv_rsrc(a.a_out, P * 128);
   const int stride = gridDim.x * 32;
@@ -532,11 +537,12 @@ __global__ __launch_bounds__(256) void k_in_fwd(SmallArgs a) {
       obase[s] = in ? static_cast<unsigned>(p) * 128 + 16 * j : kDrop;
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
-        const unsigned off = tap_off(mask, t, static_cast<unsigned>(p) * CI * 2,
-                                     tap_delta<false>(t, a.W, CI * 2));
+        const unsigned off = tap_off(mask, t, static_cast<unsigned>(p) * CI * 4,
+                                     tap_delta<false>(t, a.W, CI * 4));
 #pragma unroll
         for (int c = 0; c < CI; ++c)
-          xv[s][t][c] = bf16_bits(__builtin_amdgcn_raw_buffer_load_b16(r_in, off + 2 * c, 0, 0));
+          xv[s][t][c] = __builtin_bit_cast(
+              float, __builtin_amdgcn_raw_buffer_load_b32(r_in, off + 4 * c, 0, 0));
       }
     }
 #pragma unroll
@@ -704,7 +710,7 @@ __device__ __forceinline__ void block_reduce16(float (&v)[NV], float (*red)[4][1
 // Weight and bias gradients of a CI -> 64 layer: dw[t][ci][co] and db[co]
 // summed over pixels.  16 lanes per pixel, 4 output channels per lane.
 template <int CI>
-__global__ __launch_bounds__(256) void k_in_wgrad(const bf16 *x, const bf16 *dz, float *dw,
+__global__ __launch_bounds__(256) void k_in_wgrad(const float *x, const bf16 *dz, float *dw,
                                                   float *db, int B, int H, int W) {
   constexpr int NA = 9 * CI * 4, NV = NA + 4;
   __shared__ float red[NV][4][16];
@@ -713,7 +719,7 @@ __global__ __launch_bounds__(256) void k_in_wgrad(const bf16 *x, const bf16 *dz,
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = 0.0f;
   const int P = B * H * W;
-  const __amdgpu_buffer_rsrc_t r_x = conv_rsrc(x, P * CI * 2);
+  const __amdgpu_buffer_rsrc_t r_x = conv_rsrc(x, P * CI * 4);
   const __amdgpu_buffer_rsrc_t r_dz = conv_rsrc(dz, P * 128);
   const int stride = gridDim.x * 16;
   for (int p0 = (blockIdx.x * 256 + threadIdx.x) >> 4; p0 < P; p0 += 2 * stride) {
@@ -731,11 +737,12 @@ __global__ __launch_bounds__(256) void k_in_wgrad(const bf16 *x, const bf16 *dz,
       for (int k = 0; k < 4; ++k) g[s][k] = static_cast<float>(gv[k]);
 #pragma unroll
       for (int t = 0; t < 9; ++t) {
-        const unsigned off = tap_off(mask, t, static_cast<unsigned>(p) * CI * 2,
-                                     tap_delta<false>(t, W, CI * 2));
+        const unsigned off = tap_off(mask, t, static_cast<unsigned>(p) * CI * 4,
+                                     tap_delta<false>(t, W, CI * 4));
 #pragma unroll
         for (int c = 0; c < CI; ++c)
-          xv[s][t][c] = bf16_bits(__builtin_amdgcn_raw_buffer_load_b16(r_x, off + 2 * c, 0, 0));
+          xv[s][t][c] = __builtin_bit_cast(
+              float, __builtin_amdgcn_raw_buffer_load_b32(r_x, off + 4 * c, 0, 0));
       }
     }
 #pragma unroll
@@ -890,7 +897,7 @@ __global__ __launch_bounds__(256) void k_adam(float *p, float *m, float *v, cons
 // Per-sample scale s = max |u| over all fields and the normalised BF16 NHWC
 // input x = u / s.  u is field-major [B][C][H*W] in FP64 or FP32.
 // Per-sample normalisation x = u / s, s = max |u| (at least 1e-30), field-
-// major u -> NHWC BF16 x, in three launches over the whole batch: clear s,
+// major u -> NHWC FP32 x, in three launches over the whole batch: clear s,
 // per-block maxima merged with an integer atomicMax on the float bits
 // (non-negative floats order like their bit patterns; fmaxf drops NaNs),
 // then the scaling pass.
@@ -918,7 +925,7 @@ __global__ __launch_bounds__(256) void k_norm_max(const T *u, float *s, int C, i
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_norm_scale(const T *u, bf16 *x, float *s, int C,
+__global__ __launch_bounds__(256) void k_norm_scale(const T *u, float *x, float *s, int C,
                                                     int64_t HW) {
   const int b = blockIdx.y;
   const float sc = fmaxf(s[b], 1e-30f);
@@ -930,7 +937,7 @@ __global__ __launch_bounds__(256) void k_norm_scale(const T *u, bf16 *x, float *
   for (int64_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += static_cast<int64_t>(gridDim.x) * 256) {
     const int c = static_cast<int>(i / HW);
     const int64_t px = i - c * HW;
-    x[(static_cast<int64_t>(b) * HW + px) * C + c] = static_cast<bf16>(static_cast<float>(ub[i]) / sc);
+    x[(static_cast<int64_t>(b) * HW + px) * C + c] = static_cast<float>(ub[i]) / sc;
   }
 }
 
@@ -1122,8 +1129,13 @@ extern "C" int pml_conv_small(int mode, const void *in, const void *w, const flo
     const int nb = std::min(per, B - b0);
     const int64_t px = b0 * hw;  // first pixel of the chunk
     SmallArgs a{};
-    a.in = static_cast<const bf16 *>(in) + px * Ci;
-    a.w = static_cast<const bf16 *>(w);
+    if (in_layer) {
+      a.in32 = static_cast<const float *>(in) + px * Ci;
+      a.w32 = static_cast<const float *>(w);
+    } else {
+      a.in = static_cast<const bf16 *>(in) + px * Ci;
+      a.w = static_cast<const bf16 *>(w);
+    }
     a.bias = bias;
     a.zprev = zprev ? static_cast<const bf16 *>(zprev) + px * 64 : nullptr;
     a.z_out = z_out ? static_cast<bf16 *>(z_out) + px * 64 : nullptr;
@@ -1169,12 +1181,13 @@ extern "C" int pml_conv_small_wgrad(const void *x, const void *dz, float *dw, fl
     const int nb = std::min(per, B - b0);
     const int64_t px = b0 * hw;
     const bf16 *xb = static_cast<const bf16 *>(x) + px * Ci;
+    const float *xf = static_cast<const float *>(x) + px * Ci;  // Ci -> 64 layers read FP32
     const bf16 *dzb = static_cast<const bf16 *>(dz) + px * Co;
     // two pixels per lane group and iteration: 32 pixels per block
     const dim3 grid(small_grid(nb * hw, 32, 4)), block(256);
-    if (Ci == 1) hipLaunchKernelGGL(k_in_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
-    else if (Ci == 2) hipLaunchKernelGGL(k_in_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
-    else if (Ci == 3) hipLaunchKernelGGL(k_in_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
+    if (Ci == 1) hipLaunchKernelGGL(k_in_wgrad<1>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W);
+    else if (Ci == 2) hipLaunchKernelGGL(k_in_wgrad<2>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W);
+    else if (Ci == 3) hipLaunchKernelGGL(k_in_wgrad<3>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W);
     else if (Co == 1) hipLaunchKernelGGL(k_out_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
     else if (Co == 2) hipLaunchKernelGGL(k_out_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
     else hipLaunchKernelGGL(k_out_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
@@ -1223,12 +1236,12 @@ extern "C" int pml_surrogate_normalize(int precision, const void *u, void *x, fl
   if (precision == PML_F64) {
     const double *ud = static_cast<const double *>(u);
     hipLaunchKernelGGL(k_norm_max<double>, grid, dim3(256), 0, st, ud, s, C, HW);
-    hipLaunchKernelGGL(k_norm_scale<double>, grid, dim3(256), 0, st, ud, static_cast<bf16 *>(x), s,
+    hipLaunchKernelGGL(k_norm_scale<double>, grid, dim3(256), 0, st, ud, static_cast<float *>(x), s,
                        C, HW);
   } else {
     const float *uf = static_cast<const float *>(u);
     hipLaunchKernelGGL(k_norm_max<float>, grid, dim3(256), 0, st, uf, s, C, HW);
-    hipLaunchKernelGGL(k_norm_scale<float>, grid, dim3(256), 0, st, uf, static_cast<bf16 *>(x), s,
+    hipLaunchKernelGGL(k_norm_scale<float>, grid, dim3(256), 0, st, uf, static_cast<float *>(x), s,
                        C, HW);
   }
   PML_LAUNCH_CHECK();

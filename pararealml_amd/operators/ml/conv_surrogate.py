@@ -54,6 +54,10 @@ class ConvSurrogate:
         self.stream = stream if stream is not None else torch.cuda.current_stream()
         self.chans = [n_fields] + [HIDDEN] * (N_LAYERS - 1) + [n_fields]
         self.step = 0
+        # The network learns the normalised increment divided by out_scale
+        # (set by fit() from the training data), so a small correction is
+        # still an O(1) output.
+        self.out_scale = 1.0
         rng = np.random.RandomState(seed)
         params = []
         for layer in range(N_LAYERS):
@@ -125,7 +129,8 @@ class ConvSurrogate:
         return out
 
     def state_dict(self) -> dict:
-        sd = {"n_fields": self.torch.tensor(self.n_fields), "step": self.torch.tensor(self.step)}
+        sd = {"n_fields": self.torch.tensor(self.n_fields), "step": self.torch.tensor(self.step),
+              "out_scale": self.torch.tensor(self.out_scale, dtype=self.torch.float64)}
         for layer in range(N_LAYERS):
             sd[f"w{layer}"] = self.w32[layer].cpu()
             sd[f"b{layer}"] = self.b32[layer].cpu()
@@ -136,6 +141,7 @@ class ConvSurrogate:
             self.w32[layer].copy_(sd[f"w{layer}"].to("cuda"))
             self.b32[layer].copy_(sd[f"b{layer}"].to("cuda"))
         self.step = int(sd.get("step", 0))
+        self.out_scale = float(sd.get("out_scale", 1.0))
         self._refresh_bf16()
 
     def save(self, path: str):
@@ -171,8 +177,9 @@ class ConvSurrogate:
         f32 = dict(dtype=t.float32, device="cuda")
         c = self.n_fields
         ws = {
-            "x": t.empty(px * c, **bf),
+            "x": t.empty(px * c, **f32),  # normalised input, FP32 (first layer)
             "s": t.empty(batch, **f32),
+            "so": t.empty(batch, **f32),  # s * out_scale
             "y": t.empty(px * c, **f32),
             "a": [t.empty(px * HIDDEN, **bf) for _ in range(N_LAYERS - 1)],
         }
@@ -183,6 +190,15 @@ class ConvSurrogate:
             ws["loss"] = t.zeros(1, dtype=t.float64, device="cuda")
         return ws
 
+    def _out_s(self, ws, stream):
+        """Per-sample output scale s * out_scale (the input scale s alone
+        when out_scale is 1)."""
+        if self.out_scale == 1.0:
+            return ws["s"]
+        with self.torch.cuda.stream(stream):
+            self.torch.mul(ws["s"], self.out_scale, out=ws["so"])
+        return ws["so"]
+
     # --------------------------------------------------------------- forward
     def _forward(self, ws, batch, h, w, store_z: bool, stream=None):
         lib, sp = self.lib, _sp(stream if stream is not None else self.stream)
@@ -190,7 +206,7 @@ class ConvSurrogate:
         z = ws.get("z") if store_z else None
         # layer 0: C -> 64
         nat.check(lib.pml_conv_small(
-            0, ws["x"].data_ptr(), self.plain[0].data_ptr(), self.b32[0].data_ptr(), None,
+            0, ws["x"].data_ptr(), self.w32[0].data_ptr(), self.b32[0].data_ptr(), None,
             z[0].data_ptr() if z else None, ws["a"][0].data_ptr(), None, c, HIDDEN,
             batch, h, w, sp))
         for layer in range(1, N_LAYERS - 1):
@@ -229,8 +245,9 @@ class ConvSurrogate:
             nat.check(self.lib.pml_surrogate_normalize(
                 prec, cur.data_ptr(), ws["x"].data_ptr(), ws["s"].data_ptr(), batch, c, h * w, sp))
             self._forward(ws, batch, h, w, store_z=False, stream=st)
+            so = self._out_s(ws, st)
             nat.check(self.lib.pml_surrogate_residual(
-                prec, cur.data_ptr(), ws["y"].data_ptr(), ws["s"].data_ptr(),
+                prec, cur.data_ptr(), ws["y"].data_ptr(), so.data_ptr(),
                 res.data_ptr(), batch, c, h * w, sp))
             cur = res
         if other:
@@ -252,8 +269,9 @@ class ConvSurrogate:
         self._forward(ws, batch, h, w, store_z=True)
         self.gflat.zero_()
         ws["loss"].zero_()
+        so = self._out_s(ws, self.stream)
         nat.check(lib.pml_surrogate_loss(
-            u.data_ptr(), ws["y"].data_ptr(), ws["s"].data_ptr(), target.data_ptr(),
+            u.data_ptr(), ws["y"].data_ptr(), so.data_ptr(), target.data_ptr(),
             ws["dy"].data_ptr(), ws["loss"].data_ptr(), batch, c, h * w, sp))
         last = N_LAYERS - 1
         # layer 5 (64 -> C): weight / bias gradients and the input gradient
@@ -298,13 +316,22 @@ class ConvSurrogate:
                 None, None, None, 0, 0, sp))
 
     def fit(self, inputs, targets, steps: int, batch: int = 32, lr: float = 1e-3,
-            seed: int = 0, log_every: int = 0) -> List[float]:
+            seed: int = 0, log_every: int = 0, scale_output: bool = True) -> List[float]:
         """Adam on random minibatches of (inputs, targets) (device tensors
-        (N, C, H, W)), cosine-decayed learning rate; returns the losses."""
+        (N, C, H, W)), cosine-decayed learning rate; returns the losses (in
+        units of the output scale).  With ``scale_output``, an untrained
+        network first sets ``out_scale`` to the RMS of the normalised
+        increments (t - u) / max|u| over up to 256 pairs."""
         t = self.torch
         gen = t.Generator(device="cpu")
         gen.manual_seed(seed)
         n = inputs.shape[0]
+        if scale_output and self.step == 0:
+            idx = t.randperm(n, generator=gen)[:256].to(inputs.device)
+            u, tg = inputs[idx].double(), targets[idx].double()
+            sc = u.abs().amax(dim=(1, 2, 3), keepdim=True).clamp_min(1e-30)
+            rms = float(((tg - u) / sc).pow(2).mean().sqrt())
+            self.out_scale = rms if rms > 0.0 and math.isfinite(rms) else 1.0
         losses = []
         for k in range(steps):
             idx = t.randint(0, n, (batch,), generator=gen).to(inputs.device)
