@@ -51,6 +51,9 @@ def parse_args():
     ap.add_argument("--pr-slices", type=int, default=128)
     ap.add_argument("--pr-fine-steps", type=int, default=40000, help="fine RK4 steps per slice")
     ap.add_argument("--pr-coarse-steps", type=int, default=800, help="coarse FE steps per slice")
+    ap.add_argument("--pr-same-grid-steps", type=int, default=1100,
+                    help="FE steps per slice of an extra FDM coarse operator on the fine grid "
+                         "(0: skip that run)")
     ap.add_argument("--pr-base-steps", type=int, default=320,
                     help="FE steps per slice of the learned operator's base solve")
     ap.add_argument("--pr-base-levels", type=int, default=1, choices=[1, 2],
@@ -161,10 +164,14 @@ def parareal_workload(args):
     RandomState(434342745); t in [0, 1] split into S slices; fine RK4 with
     the configured steps per slice; FDM coarse = forward Euler on the
     2x-coarsened grid; learned coarse = forward Euler with a large (still
-    stable) step on the 2x-coarsened grid (or the 4x one) corrected by one
-    conv-surrogate application per slice on the 2x grid, trained on trajectories of the device FDM solver from other seeded
+    stable) step on the 2x-coarsened grid, prolonged, corrected by one
+    conv-surrogate application per slice on the fine grid (or on the 2x
+    grid), the correction evaluated in the initial sweep and frozen;
+    trained on trajectories of the device FDM solver from other seeded
     Gaussian mixtures, with the FDM coarse operator as fallback; the same
-    cheap pipeline without the network is run as the ablation."""
+    cheap pipeline without the network is run as the ablation, and FDM
+    coarse forward Euler on the fine grid itself (the reference's
+    arrangement, no grid transfer) as a further comparison."""
     import torch
 
     import pararealml_amd as pml
@@ -211,6 +218,8 @@ def parareal_workload(args):
         FDMOperator(RK4(), diff, dt_f, precision=prec).tune(cp, 1)
     FDMOperator(RK4(), diff, dt_f, precision=args.pr_fine_precision).tune(cp, S)
     FDMOperator(ForwardEulerMethod(), diff, dt_c).tune(coarse_problem(cp), 1)
+    if args.pr_same_grid_steps:
+        FDMOperator(ForwardEulerMethod(), diff, 1.0 / (S * args.pr_same_grid_steps)).tune(cp, 1)
 
     log("parareal: FP64 serial fine reference")
     t_ref, ref = timed_serial("float64")
@@ -259,6 +268,13 @@ def parareal_workload(args):
 
     g_fdm = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), diff, dt_c))
     fdm_res = run_parareal(g_fdm, "FDM coarse")
+    # The reference's own arrangement: the coarse FDM operator on the fine
+    # grid (no grid transfer), forward Euler near its stability limit.
+    same_res = None
+    if args.pr_same_grid_steps:
+        same_res = run_parareal(
+            FDMOperator(ForwardEulerMethod(), diff, 1.0 / (S * args.pr_same_grid_steps)),
+            "FDM coarse on the fine grid")
 
     learned_res = plain_res = None
     n_pairs = 0
@@ -304,6 +320,9 @@ def parareal_workload(args):
             "serial_fine_s": t_serial,
             "serial_fine_error": err_serial,
             "fdm_coarse": fdm_res,
+            "fdm_coarse_same_grid": (
+                dict(same_res, fe_steps_per_slice=args.pr_same_grid_steps) if same_res else None
+            ),
             "learned_coarse": learned_res,
             "learned_fallbacks": g_learned.fallbacks if g_learned else None,
             "cheap_coarse_without_network": plain_res,
