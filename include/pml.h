@@ -61,6 +61,10 @@ extern "C" {
 #define PML_EQ_CONVECTION_DIFFUSION 3 /* params: d, vel0, vel1            */
 #define PML_EQ_WAVE 4                 /* params: c^2                      */
 #define PML_EQ_BURGERS 5              /* params: 1/re                     */
+#define PML_EQ_NAVIER_STOKES 6        /* params: 1/re; fields w, psi, u, v;
+                                         psi (LHS.Y_LAPLACIAN) is left
+                                         unchanged by the step and re-solved
+                                         by the caller (pml_jacobi_sweep) */
 
 /* ---- integrators ------------------------------------------------------- */
 #define PML_INT_FE 0

@@ -340,10 +340,16 @@ def fdm_solve(
     dtype=np.float64,
     keep: str = "all",
     max_steps: Optional[int] = None,
+    poisson_tol: float = 1e-3,
+    poisson_rng: Optional[np.random.RandomState] = None,
 ):
     """FDMOperator.solve for static boundary conditions
     (fdm_operator.py:48-165, with LHS.Y fields recomputed from the pre-step
-    y after the integrator step).  ``keep="last"`` returns only the final
+    y after the integrator step and LHS.Y_LAPLACIAN fields solved by Jacobi
+    from the pre-step right-hand side, starting from uniform random values
+    drawn from ``poisson_rng`` (fdm_operator.py:141-161,
+    numerical_differentiator.py:872-927); ``poisson_tol`` is the
+    differentiator's tolerance).  ``keep="last"`` returns only the final
     state; ``max_steps`` truncates the run (used for bounded CPU timing).
 
     Returns ``(t[1:], trajectory)`` with trajectory of shape
@@ -357,8 +363,7 @@ def fdm_solve(
     d_idx = system.equation_indices_by_type(LHS.D_Y_OVER_D_T)
     y_idx = system.equation_indices_by_type(LHS.Y)
     lap_idx = system.equation_indices_by_type(LHS.Y_LAPLACIAN)
-    if lap_idx:
-        raise NotImplementedError("oracle covers D_Y_OVER_D_T and Y equations")
+    rng = poisson_rng if poisson_rng is not None else np.random.RandomState(0)
 
     if diff_eq.x_dimension:
         mesh = cp.mesh
@@ -394,6 +399,13 @@ def fdm_solve(
             yc = None if y_cons is None else y_cons[y_idx]
             y_next[..., y_idx] = apply_constraints_last_axis(
                 yc, rhs.evaluate(y_idx, t_i, y, dbc)
+            )
+        if lap_idx:
+            yc = None if y_cons is None else y_cons[lap_idx]
+            lap = rhs.evaluate(lap_idx, t_i, y, dbc)
+            y_next[..., lap_idx] = anti_laplacian(
+                lap, d_x, yc, None if dbc is None else dbc[:, lap_idx],
+                y_init=rng.random_sample(lap.shape).astype(dtype), tol=poisson_tol,
             )
         y = y_next
         if keep == "all":

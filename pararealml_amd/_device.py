@@ -56,6 +56,9 @@ class DeviceProblem:
     dx0: float
     dx1: float
     bc: np.ndarray  # FP64, n_fields * 4 * (n0 + n1)
+    # LHS.Y_LAPLACIAN equations (field, source field, c): lap(field) = c *
+    # source, re-solved after every step by FDMOperator.solve
+    y_laplacian: Tuple[Tuple[int, int, float], ...] = ()
 
     @property
     def plane(self) -> int:
@@ -142,7 +145,8 @@ def device_problem(cp) -> DeviceProblem:
         blocks.append(np.concatenate(dirichlet + neumann).astype(np.float64))
     bc = np.concatenate(blocks)
     assert bc.size == n_fields * 4 * (n0 + n1)
-    return DeviceProblem(name, tuple(params), n_fields, ndim, n0, n1, dx0, dx1, bc)
+    return DeviceProblem(name, tuple(params), n_fields, ndim, n0, n1, dx0, dx1, bc,
+                         tuple(diff_eq.device_y_laplacian()))
 
 
 _streams = {}
@@ -194,11 +198,17 @@ class FDMBatch:
         precision: str,
         integrator: int,
         stream=None,
+        poisson_phase: bool = False,
     ):
         import torch
 
         self.lib = nat.device_lib()
         first = problems[0]
+        if first.y_laplacian and not poisson_phase:
+            raise NotImplementedError(
+                "equations with LHS.Y_LAPLACIAN components run through FDMOperator.solve, "
+                "which solves their Poisson equations after every step"
+            )
         for p in problems:
             if (p.eq_name, p.params, p.ndim, p.n_fields) != (
                 first.eq_name,

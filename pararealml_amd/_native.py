@@ -27,6 +27,7 @@ PML_EQ = {
     "convection_diffusion": 3,
     "wave": 4,
     "burgers": 5,
+    "navier_stokes": 6,
 }
 
 PML_FLAG_GRAPH = 1

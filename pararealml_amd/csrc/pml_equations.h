@@ -129,6 +129,27 @@ struct EqShallowWater {
   __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
 };
 
+// Navier-Stokes in vorticity / stream function (w, psi, u, v), parameter
+// 1/re (differential_equation.py:822-850): w_t = lap(w)/re - (u w_x0 +
+// v w_x1); u = psi_x1 and v = -psi_x0 are algebraic (from the pre-step
+// state); psi is an LHS.Y_LAPLACIAN field (lap(psi) = -w), re-solved by the
+// host after each step, so its right-hand side here is zero and the step
+// leaves it unchanged.
+struct EqNavierStokes {
+  static constexpr int F = 4;
+  static constexpr unsigned kLap = 0x1u, kD = 0x3u, kAlg = 0xCu;
+  static constexpr bool kInvariant = false;
+  template <typename C>
+  __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
+    r[0] = p[0] * q[0].lap - (q[2].v * q[0].d0 + q[3].v * q[0].d1);
+  }
+  template <typename C>
+  __device__ __forceinline__ static void alg(const C *, const Quant<C> *q, C *r) {
+    r[2] = q[1].d1;
+    r[3] = -q[1].d0;
+  }
+};
+
 // Integrators (pararealml/operators/fdm/numerical_integrator.py:42-132).
 struct IntFE {
   static constexpr int kStages = 1;

@@ -16,15 +16,16 @@ namespace pml {
 static const std::vector<Variant> *find_variants(int eq, int integ, int prec, int ndim,
                                                  int *n_fields) {
   using Table = const std::vector<Variant> *(*)(int, int, int, int *);
-  Table a = nullptr, b = nullptr;
+  Table a = nullptr, b = nullptr, c = nullptr;
   switch (prec) {
-    case PML_F32: a = fdm_table_f32_a; b = fdm_table_f32_b; break;
-    case PML_F64: a = fdm_table_f64_a; b = fdm_table_f64_b; break;
-    case PML_F32C: a = fdm_table_f32c_a; b = fdm_table_f32c_b; break;
+    case PML_F32: a = fdm_table_f32_a; b = fdm_table_f32_b; c = fdm_table_f32_c; break;
+    case PML_F64: a = fdm_table_f64_a; b = fdm_table_f64_b; c = fdm_table_f64_c; break;
+    case PML_F32C: a = fdm_table_f32c_a; b = fdm_table_f32c_b; c = fdm_table_f32c_c; break;
     default: return nullptr;
   }
   const std::vector<Variant> *v = a(eq, integ, ndim, n_fields);
   if (!v) v = b(eq, integ, ndim, n_fields);
+  if (!v) v = c(eq, integ, ndim, n_fields);
   return v;
 }
 

@@ -249,6 +249,7 @@ __device__ __forceinline__ void all_quants(Quant<C> *q, const Getter &Y, bool lo
                                            bool lo0, bool hi0, bool ax0, const double *bcg,
                                            int64_t fstride, int gi, int gj, int n0, int n1,
                                            const StencilK<C> &K) {
+  static_assert(Eq::F <= 4, "the stencil quantities are unrolled for up to 4 fields");
   const int64_t noff = 2 * static_cast<int64_t>(n0 + n1);
 #define PML_FQ(F_)                                                                        \
   if constexpr (F_ < Eq::F) {                                                             \
@@ -259,6 +260,7 @@ __device__ __forceinline__ void all_quants(Quant<C> *q, const Getter &Y, bool lo
   PML_FQ(0)
   PML_FQ(1)
   PML_FQ(2)
+  PML_FQ(3)
 #undef PML_FQ
 }
 
@@ -615,6 +617,7 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
           PML_FQR(0)
           PML_FQR(1)
           PML_FQR(2)
+          PML_FQR(3)
 #undef PML_FQR
         }
         finish(i, q, w[sc], wl[sc], dv);
@@ -631,6 +634,7 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
           PML_FQR(0)
           PML_FQR(1)
           PML_FQR(2)
+          PML_FQR(3)
 #undef PML_FQR
         }
         finish(i, q, w[sc], wl[sc], nullptr);
@@ -886,6 +890,7 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
       PML_FQS(0)
       PML_FQS(1)
       PML_FQS(2)
+      PML_FQS(3)
 #undef PML_FQS
       C rr[F], ra[F];
 #pragma unroll
@@ -1118,6 +1123,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
       PML_FQK(0)
       PML_FQK(1)
       PML_FQK(2)
+      PML_FQK(3)
 #undef PML_FQK
       C rr[F];
 #pragma unroll
@@ -1343,6 +1349,7 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
       PML_FQN(0)
       PML_FQN(1)
       PML_FQN(2)
+      PML_FQN(3)
 #undef PML_FQN
       C rr[F], ra[F];
 #pragma unroll
@@ -1729,6 +1736,13 @@ const std::vector<Variant> &variants_for() {
           out.push_back(PML_TILE(EQ, IN, PR, 64, 32, 4, 512, "tile-ox64-oy32-s4"));
         }
       }
+    } else if constexpr (EQ::kAlg != 0u) {
+      // multi-stage steps with algebraic fields: the LDS tile kernel only
+      if constexpr (wide) {
+        out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
+      } else {
+        out.push_back(PML_TILE(EQ, IN, PR, 64 - 2 * S, 16, 1, 256, "tile-ox64m-oy16-s1"));
+      }
     } else {
       out.push_back(PML_STREAMRK(EQ, IN, PR, 8, 1, "streamrk-rw8-pd1"));
       out.push_back(PML_STREAMRK(EQ, IN, PR, 16, 1, "streamrk-rw16-pd1"));
@@ -1767,10 +1781,14 @@ PML_DECLARE_TABLE(fdm_table_f64_a)
 PML_DECLARE_TABLE(fdm_table_f64_b)
 PML_DECLARE_TABLE(fdm_table_f32c_a)
 PML_DECLARE_TABLE(fdm_table_f32c_b)
+PML_DECLARE_TABLE(fdm_table_f32_c)
+PML_DECLARE_TABLE(fdm_table_f64_c)
+PML_DECLARE_TABLE(fdm_table_f32c_c)
 #undef PML_DECLARE_TABLE
 
 // Group a: diffusion, Cahn-Hilliard, convection-diffusion.
 // Group b: shallow water, wave, Burgers.
+// Group c: Navier-Stokes (2-D).
 #define PML_DEFINE_TABLE_A(NAME, PR)                                                   \
   const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields) {       \
     (void)ndim;                                                                        \
@@ -1801,6 +1819,12 @@ PML_DECLARE_TABLE(fdm_table_f32c_b)
         return variants_by_int<EqBurgers2, PR>(integ);                                 \
       default: return nullptr;                                                         \
     }                                                                                  \
+  }
+#define PML_DEFINE_TABLE_C(NAME, PR)                                                   \
+  const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields) {       \
+    if (eq != PML_EQ_NAVIER_STOKES || ndim != 2) return nullptr;                       \
+    *n_fields = 4;                                                                     \
+    return variants_by_int<EqNavierStokes, PR>(integ);                                 \
   }
 
 }  // namespace pml

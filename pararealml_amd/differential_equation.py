@@ -170,6 +170,12 @@ class DifferentialEquation(ABC):
         this equation, else None."""
         return None
 
+    def device_y_laplacian(self) -> List[Tuple[int, int, float]]:
+        """The LHS.Y_LAPLACIAN equations as (field, source field, c): the
+        field's Laplacian is c times the source field (the device solves
+        them after each step); empty when there are none."""
+        return []
+
     def _validate_equations(self):
         system = self.symbolic_equation_system
         if len(system.rhs) != self._y_dimension:
@@ -567,3 +573,10 @@ class NavierStokesEquation(DifferentialEquation):
             ],
             [LHS.D_Y_OVER_D_T, LHS.Y_LAPLACIAN, LHS.Y, LHS.Y],
         )
+
+    def device_parameters(self):
+        return "navier_stokes", (1.0 / float(self._re),)
+
+    def device_y_laplacian(self):
+        # lap(psi) = -w: (LHS.Y_LAPLACIAN field, source field, coefficient)
+        return [(1, 0, -1.0)]

@@ -11,7 +11,8 @@ the pre-step state after each step as in fdm_operator.py:127-139.
 
 Device coverage: Cartesian 1-D and 2-D problems with static boundary
 conditions, the equations with fused kernels (diffusion, convection-
-diffusion, wave, Cahn-Hilliard, Burgers, shallow water), the explicit
+diffusion, wave, Cahn-Hilliard, Burgers, shallow water, Navier-Stokes with
+its stream function re-solved by device Jacobi sweeps), the explicit
 integrators and ``ThreePointCentralDifferenceMethod``.  Anything else raises
 ``NotImplementedError``; there is no host fallback.
 
@@ -142,6 +143,8 @@ class FDMOperator(Operator):
 
         import torch
 
+        if problem.y_laplacian:
+            return self._solve_with_poisson(ivp, problem, integ, t, n_steps, every, y0)
         stream = dev.solver_stream()
         stream.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(stream):
@@ -166,3 +169,58 @@ class FDMOperator(Operator):
             vertex_oriented=True,
             d_t=self._d_t * every,
         )
+
+    def _solve_with_poisson(self, ivp, problem, integ, t, n_steps, every, y0) -> Solution:
+        """The time loop for equations with LHS.Y_LAPLACIAN components
+        (fdm_operator.py:141-161): each step runs the fused kernel (the
+        D_Y_OVER_D_T fields advance, the LHS.Y fields are recomputed from the
+        pre-step state, the Y_LAPLACIAN fields stay), then every
+        Y_LAPLACIAN field is re-solved from the pre-step state's right-hand
+        side by the differentiator's device Jacobi iteration, with the
+        field's own constraints.  The iteration starts from the field's
+        current value; the reference starts from uniform random values, so
+        its result is defined only up to the Jacobi tolerance, which bounds
+        the difference between the two."""
+        import torch
+
+        if self._precision == "float32c":
+            raise NotImplementedError(
+                "LHS.Y_LAPLACIAN equations run in float32 or float64"
+            )
+        cp = ivp.constrained_problem
+        mesh = cp.mesh
+        plane = problem.plane
+        shape = (problem.n0, problem.n1, 1)
+        y_cons = cp.static_y_vertex_constraints
+        d_cons = cp.static_boundary_vertex_constraints[1]
+        n_snap = n_steps // every
+        stream = dev.solver_stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            batch = dev.FDMBatch([problem], [0], self._precision, integ, stream,
+                                 poisson_phase=True)
+            batch.load_host([y0])
+            traj = torch.empty((n_snap, problem.elems), dtype=torch.float64, device="cuda")
+            tile = self.tile_config_for(problem)
+            state = batch.hi  # field-major, the batch precision
+            for i in range(n_steps):
+                sources = {
+                    f: (c * state[src * plane:(src + 1) * plane]).view(shape)
+                    for f, src, c in problem.y_laplacian
+                }
+                batch.advance(self._d_t, 1, tile_config=tile, use_graph=False)
+                for f, lap in sources.items():
+                    cur = state[f * plane:(f + 1) * plane].view(shape)
+                    new = self._differentiator.anti_laplacian(
+                        lap, mesh,
+                        [None if y_cons is None else y_cons[f]],
+                        None if d_cons is None else d_cons[:, f:f + 1],
+                        y_init=cur,
+                    )
+                    cur.copy_(new)
+                if (i + 1) % every == 0:
+                    batch.state_f64(traj[(i + 1) // every - 1])
+            flat = traj.cpu().numpy()
+        flat = flat.reshape(n_snap, problem.n_fields, problem.n0, problem.n1)
+        y = problem.from_device_layout(flat, mesh.vertices_shape)
+        return Solution(ivp, t[every::every], y, vertex_oriented=True, d_t=self._d_t * every)

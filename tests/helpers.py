@@ -83,3 +83,23 @@ def shallow_water(n, seed=0, h=1.0, v=0.01, g=9.81, length=10.0):
         cp, [(mean, 0.25 * np.eye(2))] * 3, [0.1, 0.0, 0.0]
     )
     return cp, ic
+
+
+def navier_stokes_cavity(n=16, re=100.0):
+    """Lid-driven cavity in (w, psi, u, v) on [0, 1]^2 with (n + 1)^2
+    vertices: every field Dirichlet 0 except u = 1 on the upper edge of
+    axis 1; initial vorticity a Gaussian bump, psi = u = v = 0 inside."""
+    eq = pml.NavierStokesEquation(re)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / n, 1.0 / n])
+    wall = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 4)), is_static=True)
+    lid = pml.DirichletBoundaryCondition(
+        lambda x, t: np.column_stack(
+            [np.zeros(len(x)), np.zeros(len(x)), np.ones(len(x)), np.zeros(len(x))]
+        ),
+        is_static=True,
+    )
+    cp = pml.ConstrainedProblem(eq, mesh, [(wall, wall), (wall, lid)])
+    x0, x1 = mesh.vertex_coordinate_grids
+    y0 = np.zeros(mesh.vertices_shape + (4,))
+    y0[..., 0] = np.exp(-((x0 - 0.5) ** 2 + (x1 - 0.6) ** 2) / 0.02)
+    return cp, y0
