@@ -26,6 +26,8 @@
  *                             _next_anti_laplacian_estimate (Cartesian) and the
  *                             constraint application of anti_laplacian
  *                             (numerical_differentiator.py:872-927, 1097-1186)
+ *   pml_jacobi_solve       <- the anti_laplacian loop itself
+ *                             (numerical_differentiator.py:904-925)
  *   pml_parareal_update    <- the per-slice update U[j+1] = G[j] + corr[j] and
  *                             the RMS update norms of PararealOperator
  *                             (operators/parareal/parareal_operator.py:53-100,
@@ -64,7 +66,7 @@ extern "C" {
 #define PML_EQ_NAVIER_STOKES 6        /* params: 1/re; fields w, psi, u, v;
                                          psi (LHS.Y_LAPLACIAN) is left
                                          unchanged by the step and re-solved
-                                         by the caller (pml_jacobi_sweep) */
+                                         by the caller (pml_jacobi_solve) */
 
 /* ---- integrators ------------------------------------------------------- */
 #define PML_INT_FE 0
@@ -166,6 +168,23 @@ int pml_jacobi_sweep(int precision, const void *y_old, void *y_new,
                      const double *dx, const double *const *ghost_lo,
                      const double *const *ghost_hi, const double *ycons,
                      double *sumsq_out, void *stream);
+/*
+ * The whole Jacobi loop of anti_laplacian (numerical_differentiator.py:
+ * 872-927): sweeps (as pml_jacobi_sweep, y -> work -> y ...) until
+ * ||y_new - y_old||_2 <= tol, with the stopping test on the device (a sweep
+ * after convergence does nothing) and the host reading the loop state once
+ * per `batch` sweeps.  y holds the start (constraints already applied) and
+ * receives the result; work is a second array of the same size; state is a
+ * device buffer of at least 32 bytes (8-byte aligned).  *sweeps_out = the
+ * sweeps taken, or -(sweeps issued) if max_sweeps ran out first; *diff_out
+ * (may be NULL) = the last update norm.
+ */
+int pml_jacobi_solve(int precision, void *y, void *work, const void *lap,
+                     const int64_t *shape, int n_fields, const double *dx,
+                     const double *const *ghost_lo, const double *const *ghost_hi,
+                     const double *ycons, double tol, int64_t max_sweeps,
+                     int64_t batch, void *state, int64_t *sweeps_out,
+                     double *diff_out, void *stream);
 
 /*
  * Parareal, all FP64.

@@ -92,9 +92,22 @@ struct JacobiGeom {
 // One Jacobi update of the Cartesian anti-Laplacian
 // (numerical_differentiator.py:1097-1186) followed by the y constraints
 // and a sum of squared updates (872-927).
+// State of a device-side Jacobi loop (pml_jacobi_solve): the sweep's
+// sum of squared updates, a done flag and the number of sweeps taken.
+struct JacobiState {
+  double sumsq;
+  double last_diff;
+  long long sweeps;
+  int done;
+  int pad;
+};
+
 template <typename T>
 __global__ void k_jacobi(const T *y, T *yn, const T *lap, int64_t total, int nf,
-                         JacobiGeom g, const double *ycons, double *sumsq) {
+                         JacobiGeom g, const double *ycons, double *sumsq,
+                         const int *done) {
+  // A loop that has converged turns its remaining sweeps into no-ops.
+  if (done && *done) return;
   __shared__ double red[256];
   double acc2 = 0.0;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
@@ -150,6 +163,29 @@ __global__ void k_jacobi(const T *y, T *yn, const T *lap, int64_t total, int nf,
   if (threadIdx.x == 0) atomicAdd(sumsq, red[0]);
 }
 
+__global__ void k_jacobi_init(JacobiState *st) {
+  if (threadIdx.x == 0) {
+    st->sumsq = 0.0;
+    st->last_diff = 0.0;
+    st->sweeps = 0;
+    st->done = 0;
+    st->pad = 0;
+  }
+}
+
+// After a sweep: count it, stop once ||y_new - y_old||_2 <= tol (the
+// reference's loop condition), clear the accumulator for the next sweep.
+__global__ void k_jacobi_check(JacobiState *st, double tol) {
+  if (threadIdx.x == 0) {
+    if (!st->done) {
+      st->sweeps += 1;
+      st->last_diff = sqrt(st->sumsq);
+      if (st->last_diff <= tol) st->done = 1;
+    }
+    st->sumsq = 0.0;
+  }
+}
+
 }  // namespace pml
 
 using namespace pml;
@@ -194,12 +230,11 @@ extern "C" int pml_stencil_second(int precision, const void *y, void *out, int64
   return PML_OK;
 }
 
-extern "C" int pml_jacobi_sweep(int precision, const void *y_old, void *y_new, const void *lap,
-                                const int64_t *shape, int n_fields, const double *dx,
-                                const double *const *ghost_lo, const double *const *ghost_hi,
-                                const double *ycons, double *sumsq_out, void *stream) {
-  if (!y_old || !y_new || !lap || !shape || !dx || !sumsq_out || n_fields <= 0)
-    return fail(PML_E_INVALID, "bad Jacobi arguments");
+// Geometry of a Jacobi sweep over a (shape[0], shape[1], shape[2], n_fields)
+// array; returns PML_OK or the error, and the element count in *total.
+static int jacobi_geom(const int64_t *shape, int n_fields, const double *dx,
+                       const double *const *ghost_lo, const double *const *ghost_hi,
+                       JacobiGeom *out, int64_t *total_out) {
   JacobiGeom g{};
   int64_t total = n_fields;
   g.coef = 0.0;
@@ -222,19 +257,93 @@ extern "C" int pml_jacobi_sweep(int precision, const void *y_old, void *y_new, c
   g.stride[2] = n_fields;
   g.stride[1] = shape[2] * n_fields;
   g.stride[0] = shape[1] * shape[2] * n_fields;
+  *out = g;
+  *total_out = total;
+  return PML_OK;
+}
+
+extern "C" int pml_jacobi_sweep(int precision, const void *y_old, void *y_new, const void *lap,
+                                const int64_t *shape, int n_fields, const double *dx,
+                                const double *const *ghost_lo, const double *const *ghost_hi,
+                                const double *ycons, double *sumsq_out, void *stream) {
+  if (!y_old || !y_new || !lap || !shape || !dx || !sumsq_out || n_fields <= 0)
+    return fail(PML_E_INVALID, "bad Jacobi arguments");
+  JacobiGeom g{};
+  int64_t total = 0;
+  if (int rc = jacobi_geom(shape, n_fields, dx, ghost_lo, ghost_hi, &g, &total); rc != PML_OK)
+    return rc;
   hipStream_t st = as_stream(stream);
   int nb = grid_for(total);
   if (nb > 2048) nb = 2048;
   if (precision == PML_F64)
     hipLaunchKernelGGL(k_jacobi<double>, dim3(nb), dim3(256), 0, st, (const double *)y_old,
                        (double *)y_new, (const double *)lap, total, n_fields, g, ycons,
-                       sumsq_out);
+                       sumsq_out, nullptr);
   else if (precision == PML_F32)
     hipLaunchKernelGGL(k_jacobi<float>, dim3(nb), dim3(256), 0, st, (const float *)y_old,
                        (float *)y_new, (const float *)lap, total, n_fields, g, ycons,
-                       sumsq_out);
+                       sumsq_out, nullptr);
   else
     return fail(PML_E_INVALID, "Jacobi sweep takes FP32 or FP64 arrays");
   PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_jacobi_solve(int precision, void *y, void *work, const void *lap,
+                                const int64_t *shape, int n_fields, const double *dx,
+                                const double *const *ghost_lo, const double *const *ghost_hi,
+                                const double *ycons, double tol, int64_t max_sweeps,
+                                int64_t batch, void *state, int64_t *sweeps_out,
+                                double *diff_out, void *stream) {
+  if (!y || !work || !lap || !shape || !dx || !state || !sweeps_out || n_fields <= 0 ||
+      max_sweeps <= 0 || batch <= 0 || !(tol >= 0.0))
+    return fail(PML_E_INVALID, "bad Jacobi solve arguments");
+  if (precision != PML_F64 && precision != PML_F32)
+    return fail(PML_E_INVALID, "Jacobi solve takes FP32 or FP64 arrays");
+  JacobiGeom g{};
+  int64_t total = 0;
+  if (int rc = jacobi_geom(shape, n_fields, dx, ghost_lo, ghost_hi, &g, &total); rc != PML_OK)
+    return rc;
+  hipStream_t st = as_stream(stream);
+  JacobiState *js = static_cast<JacobiState *>(state);
+  int nb = grid_for(total);
+  if (nb > 2048) nb = 2048;
+  hipLaunchKernelGGL(k_jacobi_init, dim3(1), dim3(64), 0, st, js);
+  JacobiState host{};
+  int64_t issued = 0;
+  while (issued < max_sweeps) {
+    const int64_t n = std::min<int64_t>(batch, max_sweeps - issued);
+    for (int64_t k = 0; k < n; ++k) {
+      // sweep i (0-based) reads buffer i % 2 (y, work) and writes the other
+      const bool odd = ((issued + k) & 1) != 0;
+      const void *src = odd ? work : y;
+      void *dst = odd ? y : work;
+      if (precision == PML_F64)
+        hipLaunchKernelGGL(k_jacobi<double>, dim3(nb), dim3(256), 0, st,
+                           static_cast<const double *>(src), static_cast<double *>(dst),
+                           static_cast<const double *>(lap), total, n_fields, g, ycons,
+                           &js->sumsq, &js->done);
+      else
+        hipLaunchKernelGGL(k_jacobi<float>, dim3(nb), dim3(256), 0, st,
+                           static_cast<const float *>(src), static_cast<float *>(dst),
+                           static_cast<const float *>(lap), total, n_fields, g, ycons,
+                           &js->sumsq, &js->done);
+      hipLaunchKernelGGL(k_jacobi_check, dim3(1), dim3(64), 0, st, js, tol);
+    }
+    issued += n;
+    PML_LAUNCH_CHECK();
+    PML_HIP_CHECK(hipMemcpyAsync(&host, js, sizeof(host), hipMemcpyDeviceToHost, st));
+    PML_HIP_CHECK(hipStreamSynchronize(st));
+    if (host.done) break;
+  }
+  // The converged iterate is the output of sweep `sweeps` (1-based): it is
+  // in work when that count is odd.
+  const size_t esz = precision == PML_F64 ? 8 : 4;
+  if (host.done && (host.sweeps & 1))
+    PML_HIP_CHECK(hipMemcpyAsync(y, work, total * esz, hipMemcpyDeviceToDevice, st));
+  else if (!host.done && (issued & 1))
+    PML_HIP_CHECK(hipMemcpyAsync(y, work, total * esz, hipMemcpyDeviceToDevice, st));
+  *sweeps_out = host.done ? host.sweeps : -issued;
+  if (diff_out) *diff_out = host.last_diff;
   return PML_OK;
 }

@@ -138,10 +138,16 @@ def _second(src: "_Arr", t, out, axis: int, d_x: float, d_x2: float, pairs, mode
 class NumericalDifferentiator(ABC):
     """Validation and public interface (numerical_differentiator.py:14-996)."""
 
-    def __init__(self, tol: float = 1e-3):
+    def __init__(self, tol: float = 1e-3, *, max_sweeps: int = 10_000_000):
         if tol < 0.0:
             raise ValueError("tolerance must be non-negative")
+        if max_sweeps < 1:
+            raise ValueError("max_sweeps must be positive")
         self._tol = tol
+        # The reference's Jacobi loop has no bound; this one raises
+        # RuntimeError after max_sweeps sweeps without reaching tol.
+        self._max_sweeps = int(max_sweeps)
+        self.last_jacobi_sweeps = 0
 
     # ---- validation helpers (numerical_differentiator.py:929-996) ---------
     @staticmethod
@@ -190,8 +196,8 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
     """Second-order central differences on gfx950
     (numerical_differentiator.py:999-1242)."""
 
-    def __init__(self, tol: float = 1e-3):
-        super().__init__(tol)
+    def __init__(self, tol: float = 1e-3, *, max_sweeps: int = 10_000_000):
+        super().__init__(tol, max_sweeps=max_sweeps)
 
     # ---------------------------------------------------------------- public
     def gradient(self, y, mesh: Mesh, x_axis: int, derivative_boundary_constraints=None):
@@ -388,24 +394,30 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
         if ycons is not None:
             mask = ~torch.isnan(ycons)
             cur.view(-1)[mask] = ycons[mask].to(cur.dtype)
-        nxt = torch.empty_like(cur)
-        sumsq = torch.zeros(1, dtype=torch.float64, device=cur.device)
+        work = torch.empty_like(cur)
+        state = torch.zeros(4, dtype=torch.float64, device=cur.device)
         lib = nat.device_lib()
         c_shape = (ctypes.c_int64 * 3)(*spatial)
         c_dx = (ctypes.c_double * 3)(*dxs)
         c_lo = (ctypes.c_void_p * 3)(*glo)
         c_hi = (ctypes.c_void_p * 3)(*ghi)
-        diff = np.inf
-        while diff > self._tol:
-            sumsq.zero_()
-            nat.check(
-                lib.pml_jacobi_sweep(
-                    lap.prec, cur.data_ptr(), nxt.data_ptr(), lap.t.data_ptr(), c_shape,
-                    n_fields, c_dx, c_lo, c_hi,
-                    None if ycons is None else ycons.data_ptr(),
-                    sumsq.data_ptr(), _stream(),
-                )
+        sweeps = ctypes.c_int64(0)
+        diff = ctypes.c_double(0.0)
+        # the loop and its stopping test run on the device; the host looks
+        # at the loop state once per 64 sweeps
+        nat.check(
+            lib.pml_jacobi_solve(
+                lap.prec, cur.data_ptr(), work.data_ptr(), lap.t.data_ptr(), c_shape,
+                n_fields, c_dx, c_lo, c_hi,
+                None if ycons is None else ycons.data_ptr(),
+                float(self._tol), int(self._max_sweeps), 64, state.data_ptr(),
+                ctypes.byref(sweeps), ctypes.byref(diff), _stream(),
             )
-            cur, nxt = nxt, cur
-            diff = float(np.sqrt(sumsq.item()))
+        )
+        if sweeps.value < 0:
+            raise RuntimeError(
+                f"the Jacobi iteration did not reach tol={self._tol} in "
+                f"{-sweeps.value} sweeps (last update norm {diff.value:.3e})"
+            )
+        self.last_jacobi_sweeps = int(sweeps.value)
         return lap.give(cur)
