@@ -977,11 +977,27 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
 // pipeline: while walking down, stage s works on the row s-1 above stage 1,
 // so every stage input Y_s stays in a three-row register ring and each row
 // of the state crosses HBM once per step.  Stage s is valid on one column
-// fewer per side than stage s-1: lane l owns column j0 - (S-1) + l and the
-// step's result is written by lanes S-1 .. 64-S (strips of 66 - 2S
-// columns).  The per-stage arithmetic, stage weights and Dirichlet rules are
-// those of fdm_fused (bitwise-identical results); boundary rules are
-// selects as in fdm_stream2, units are specialised on touching an edge.
+// fewer per side than stage s-1.  With one or two fields, lane l owns column
+// j0 - (S-1) + l and lanes 0 / 63 also load the halo columns beside the
+// wavefront, so the result is written by lanes S-1 .. 64-S (strips of
+// 66 - 2S columns).  With three or more fields registers limit occupancy:
+// lane l owns column j0 - S + l and loads no halo, the result is written by
+// lanes S .. 63-S (strips of 64 - 2S columns), which keeps the kernel at
+// three waves per SIMD.  The axis-1 edge values of a row are held in one
+// register per field (lower edge in lane k, upper in lane 32 + k).  The
+// per-stage arithmetic, stage weights and Dirichlet rules are those of
+// fdm_fused (bitwise-identical results); boundary rules are selects as in
+// fdm_stream2, units are specialised on touching an edge.
+template <class Eq, class Integ>
+constexpr bool stream_rk_halo() {
+  return Eq::F < 3;
+}
+// Lanes left of a stream_rk strip's first output column; strips are
+// 64 - 2 * lead columns wide.
+template <class Eq, class Integ>
+constexpr int stream_rk_lead() {
+  return stream_rk_halo<Eq, Integ>() ? Integ::kStages - 1 : Integ::kStages;
+}
 template <class Eq, class Integ, class P, int RW, int PD, bool SNAP>
 __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
   using C = typename P::C;
@@ -992,10 +1008,11 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
   static_assert(Eq::kAlg == 0u, "no algebraic fields");
   constexpr int NW = PD + S + 1;        // input ring slots
   constexpr int NK = S;                 // k-accumulator ring slots (RK4)
-  constexpr int OW = 66 - 2 * S;        // output columns per strip
   constexpr int NT = RW + 2 * (S - 1);  // pipeline iterations
+  constexpr bool kHalo = stream_rk_halo<Eq, Integ>();
+  constexpr int kLead = stream_rk_lead<Eq, Integ>();
   static_assert(PD >= 1, "the ring must prefetch at least one row");
-  static_assert(NT <= 64, "one lane per stage-1 row of the unit");
+  static_assert(NT <= 32, "one lane per stage-1 row of the unit and edge");
   const int lane = threadIdx.x & 63;
   const int unit = __builtin_amdgcn_readfirstlane(xcd_tile(blockIdx.x, gridDim.x) * 4 +
                                                   static_cast<int>(threadIdx.x >> 6));
@@ -1015,7 +1032,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
   const C hdt = static_cast<C>(a.dt * 0.5);
   const int i_beg = U.i0;
   const int i_end = min(U.i0 + RW, n0);
-  const int c0 = U.j0 - (S - 1);  // column of lane 0
+  const int c0 = U.j0 - kLead;  // column of lane 0
   const int col = c0 + lane;
   const bool lo1 = col == 0, hi1 = col == n1 - 1;
   const int cc = min(max(col, 0), n1 - 1);
@@ -1033,7 +1050,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
   const unsigned jb = static_cast<unsigned>(col) * esz;  // columns < 0 read 0
   const int hcol = lane == 0 ? c0 - 1 : (lane == 63 ? c0 + 64 : col);
   const unsigned hb = static_cast<unsigned>(hcol) * esz;
-  const unsigned sb = (lane >= S - 1 && lane <= 64 - S) ? jb : 0x80000000u;
+  const unsigned sb = (lane >= kLead && lane <= 63 - kLead) ? jb : 0x80000000u;
   const unsigned row_bytes = static_cast<unsigned>(n1) * esz;
 
   auto load = [&](int r, C *v, C *vl, C *h) {
@@ -1046,7 +1063,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
       v[f] = buf_load<C, 1>(rs, jb).v[0];
       if (P::kComp)
         vl[f] = buf_load<C, 1>(row_rsrc(in_lo + static_cast<size_t>(f) * plane + ro, nb), jb).v[0];
-      h[f] = buf_load<C, 1>(rs, hb).v[0];
+      if (kHalo) h[f] = buf_load<C, 1>(rs, hb).v[0];
     }
   };
   auto store_row = [&](int r, const C *vh, const C *vl) {
@@ -1066,13 +1083,14 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
   auto run = [&](auto edge_tag) {
     constexpr bool kE = decltype(edge_tag)::value;
     // Edge values: the lane's column on rows 0 / n0-1, and the axis-1 edge
-    // values of stage-1 row r_first + k in lane k.
-    C rn_lo[F], rn_hi[F], pe_lo[F], pe_hi[F];
-    D rd_lo[F], rd_hi[F], pd_lo[F], pd_hi[F];
+    // values of stage-1 row r_first + k, lower edge in lane k and upper edge
+    // in lane 32 + k (one register for both).
+    C rn_lo[F], rn_hi[F], pe[F];
+    D rd_lo[F], rd_hi[F], pd[F];
 #pragma unroll
     for (int f = 0; f < F; ++f) {
-      rn_lo[f] = rn_hi[f] = pe_lo[f] = pe_hi[f] = static_cast<C>(NAN);
-      rd_lo[f] = rd_hi[f] = pd_lo[f] = pd_hi[f] = static_cast<D>(NAN);
+      rn_lo[f] = rn_hi[f] = pe[f] = static_cast<C>(NAN);
+      rd_lo[f] = rd_hi[f] = pd[f] = static_cast<D>(NAN);
       if (kE) {
         const double *fb = bcg + f * fstride;
         const double *nb = fb + 2 * (n0 + n1);
@@ -1082,11 +1100,9 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
           rn_lo[f] = static_cast<C>(nb[cc]);
           rn_hi[f] = static_cast<C>(nb[n1 + cc]);
         }
-        const int rk = min(max(r_first + lane, 0), n0 - 1);
-        pd_lo[f] = static_cast<D>(fb[2 * n1 + rk]);
-        pd_hi[f] = static_cast<D>(fb[2 * n1 + n0 + rk]);
-        pe_lo[f] = static_cast<C>(nb[2 * n1 + rk]);
-        pe_hi[f] = static_cast<C>(nb[2 * n1 + n0 + rk]);
+        const int rk = min(max(r_first + (lane & 31), 0), n0 - 1) + (lane < 32 ? 0 : n0);
+        pd[f] = static_cast<D>(fb[2 * n1 + rk]);
+        pe[f] = static_cast<C>(nb[2 * n1 + rk]);
       }
     }
     // dt-weighted right-hand side of row r = r_first + k from stage-input
@@ -1102,12 +1118,12 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
         e_lo[f] = e_hi[f] = static_cast<C>(NAN);
         dv[f] = static_cast<D>(NAN);
         if (kE) {
-          e_lo[f] = lane_value(pe_lo[f], k);
-          e_hi[f] = lane_value(pe_hi[f], k);
+          e_lo[f] = lane_value(pe[f], k);
+          e_hi[f] = lane_value(pe[f], 32 + k);
           D v = lo0 ? rd_lo[f] : (hi0 ? rd_hi[f] : static_cast<D>(NAN));
           v = (v == v) ? v
-                       : (l1 ? lane_value(pd_lo[f], k)
-                             : (h1 ? lane_value(pd_hi[f], k) : static_cast<D>(NAN)));
+                       : (l1 ? lane_value(pd[f], k)
+                             : (h1 ? lane_value(pd[f], 32 + k) : static_cast<D>(NAN)));
           dv[f] = v;
         }
       }
@@ -1137,17 +1153,18 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
     // (s >= 2) of the row produced at iteration t in slot t mod 3.  k ring:
     // the accumulator of the row whose stage 1 ran at iteration t, slot
     // t mod NK.
-    C w[NW][F], wl[NW][F], wh[NW][F];
+    C w[NW][F], wl[NW][F], wh[kHalo ? NW : 1][F];
     C y[S > 1 ? S - 1 : 1][3][F];
     C kacc[NK][F];
 #pragma unroll
-    for (int m = 0; m <= PD + 1; ++m) load(i_beg - S + m, w[m % NW], wl[m % NW], wh[m % NW]);
+    for (int m = 0; m <= PD + 1; ++m)
+      load(i_beg - S + m, w[m % NW], wl[m % NW], wh[kHalo ? m % NW : 0]);
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
       if (i_beg - 2 * (S - 1) + t >= i_end) break;  // stage S is past the strip
       {
         const int m = t + 2 + PD;
-        load(i_beg - S + m, w[m % NW], wl[m % NW], wh[m % NW]);
+        load(i_beg - S + m, w[m % NW], wl[m % NW], wh[kHalo ? m % NW : 0]);
       }
 #pragma unroll
       for (int s = 1; s <= S; ++s) {
@@ -1157,8 +1174,9 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
         C kk[F];
         D dv[F];
         if (s == 1) {
-          stage_k(k, w[t % NW], w[(t + 1) % NW], w[(t + 2) % NW], wh[(t + 1) % NW],
-                  S == 2 ? hdt : dt, kk, dv);
+          // Stage 1 is valid on every lane when the halo columns are loaded.
+          stage_k(k, w[t % NW], w[(t + 1) % NW], w[(t + 2) % NW],
+                  kHalo ? wh[(t + 1) % NW] : w[(t + 1) % NW], S == 2 ? hdt : dt, kk, dv);
         } else {
           // Y_s rows r-1, r, r+1 were produced at iterations t-2, t-1, t.
           const C *yu = y[s - 2][(t + 1) % 3], *yc = y[s - 2][(t + 2) % 3],
@@ -1695,7 +1713,7 @@ struct Variant {
   Variant {                                                                               \
     &fdm_stream_rk<EQ, IN, PR, RW, PD, false>, &fdm_stream_rk<EQ, IN, PR, RW, PD, false>, \
         &fdm_stream_rk<EQ, IN, PR, RW, PD, true>, &fdm_stream_rk<EQ, IN, PR, RW, PD, true>, \
-        kStreamKind, 66 - 2 * IN::kStages, RW, 1, 256, 1, NAME                            \
+        kStreamKind, 64 - 2 * stream_rk_lead<EQ, IN>(), RW, 1, 256, 1, NAME               \
   }
 #define PML_STREAMN(EQ, IN, PR, RW, PD, NS, NAME)                                         \
   Variant {                                                                               \

@@ -280,35 +280,51 @@ def _mixed_bc_diffusion(n0, n1):
     return cp, ic
 
 
-@pytest.mark.parametrize("integ", ["fe", "midpoint", "rk4"])
-@pytest.mark.parametrize("prec", ["float64", "float32", "float32c"])
-@pytest.mark.parametrize("shapes", [((21, 130), (34, 66)), ((21, 130), (17, 33))])
-def test_stream_variants_identical_with_boundaries_and_snapshots(integ, prec, shapes):
-    """Every variant (one and two columns per lane, ring depths, two steps
-    per launch), eager and graph launches, on a ragged batch with Dirichlet,
-    Neumann and Cauchy edges, strips that end inside the grid and snapshots
-    (an odd step count, so fused variants also run a single-step launch):
-    bitwise-identical states and snapshots.  The second batch has an odd row length, where the
-    two-column variants hand over to the default one.  FP64 matches the
-    oracle per grid."""
-    print(f"case: stream variants {integ} {prec} {shapes}", flush=True)
+def _mixed_bc_shallow_water(n0, n1):
+    """Shallow water with Dirichlet / Neumann rows (axis 0) and Neumann /
+    Cauchy columns (axis 1) on an n0 x n1 vertex grid."""
+    eq = pml.ShallowWaterEquation(h=1.0, v=0.01, g=9.81)
+    mesh = pml.Mesh([(0.0, 0.1 * (n0 - 1)), (0.0, 0.125 * (n1 - 1))], [0.1, 0.125])
+    bcs = [
+        (
+            pml.DirichletBoundaryCondition(lambda x, t: np.full((len(x), 3), 0.05), is_static=True),
+            pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True),
+        ),
+        (
+            pml.NeumannBoundaryCondition(lambda x, t: np.full((len(x), 3), -0.02), is_static=True),
+            pml.CauchyBoundaryCondition(
+                lambda x, t: np.full((len(x), 3), 1.0),
+                lambda x, t: np.repeat(x[:, :1] * 0.01, 3, axis=1),
+                is_static=True,
+            ),
+        ),
+    ]
+    cp = pml.ConstrainedProblem(eq, mesh, bcs)
+    centre = np.array([0.05 * (n0 - 1), 0.06 * (n1 - 1)])
+    ic = pml.GaussianInitialCondition(cp, [(centre, 0.3 * np.eye(2))] * 3, [0.2, 0.0, 0.0])
+    return cp, ic
+
+
+def _check_stream_variants(eq_name, cases, integ, prec, d_t):
+    """Runs every kernel variant, eager and graph, on the ragged batch of
+    `cases` with snapshots; asserts bitwise-identical states and snapshots
+    and, in FP64, the oracle per grid."""
     from pararealml_amd import _device as dev
     from pararealml_amd import _native as nat
 
-    cases = [_mixed_bc_diffusion(*s) for s in shapes]
     problems = [dev.device_problem(cp) for cp, _ in cases]
     y0s = [ic.discrete_y_0(True) for _, ic in cases]
     lib = nat.device_lib()
     int_id = {"fe": nat.PML_INT_FE, "midpoint": nat.PML_INT_MIDPOINT, "rk4": nat.PML_INT_RK4}[integ]
-    n_cfg = lib.pml_fdm_configs(nat.PML_EQ["diffusion"], int_id, dev.precision_id(prec))
-    names = [lib.pml_fdm_config_name(nat.PML_EQ["diffusion"], int_id,
+    n_cfg = lib.pml_fdm_configs(nat.PML_EQ[eq_name], int_id, dev.precision_id(prec))
+    names = [lib.pml_fdm_config_name(nat.PML_EQ[eq_name], int_id,
                                      dev.precision_id(prec), c).decode() for c in range(n_cfg)]
     assert any(("-c2-" if integ == "fe" else "streamrk-") in n for n in names)
-    d_t, steps, every = 2e-4, 37, 4
+    steps, every = 37, 4
     outs = []
     for cfg in range(n_cfg):
         for graph in (False, True):
-            b = dev.FDMBatch(problems, [0, 1], prec, int_id, dev.solver_stream())
+            b = dev.FDMBatch(problems, list(range(len(cases))), prec, int_id, dev.solver_stream())
             b.load_host(y0s)
             snaps = b.new_snapshots(steps // every)
             b.advance(d_t, steps, snapshots=snaps, snapshot_every=every, tile_config=cfg,
@@ -325,6 +341,35 @@ def test_stream_variants_identical_with_boundaries_and_snapshots(integ, prec, sh
                                      keep="last")
             got = problems[k].from_device_layout(b.grid_view(flat, k), cp.mesh.vertices_shape)
             assert rel_err(got, y_ref[0]) <= 1e-12
+
+
+@pytest.mark.parametrize("integ", ["fe", "midpoint", "rk4"])
+@pytest.mark.parametrize("prec", ["float64", "float32", "float32c"])
+@pytest.mark.parametrize("shapes", [((21, 130), (34, 66)), ((21, 130), (17, 33))])
+def test_stream_variants_identical_with_boundaries_and_snapshots(integ, prec, shapes):
+    """Every variant (one and two columns per lane, ring depths, two steps
+    per launch), eager and graph launches, on a ragged batch with Dirichlet,
+    Neumann and Cauchy edges, strips that end inside the grid and snapshots
+    (an odd step count, so fused variants also run a single-step launch):
+    bitwise-identical states and snapshots.  The second batch has an odd row length, where the
+    two-column variants hand over to the default one.  FP64 matches the
+    oracle per grid."""
+    print(f"case: stream variants {integ} {prec} {shapes}", flush=True)
+    _check_stream_variants("diffusion", [_mixed_bc_diffusion(*s) for s in shapes], integ, prec,
+                           2e-4)
+
+
+@pytest.mark.parametrize("integ", ["midpoint", "rk4"])
+@pytest.mark.parametrize("prec", ["float64", "float32", "float32c"])
+def test_shallow_water_stream_variants_identical(integ, prec):
+    """Three fields take the streaming multi-stage kernel's halo-free layout
+    (strips of 64 - 2S columns): every variant, eager and graph, on a ragged
+    batch with mixed edges and snapshots gives bitwise-identical results,
+    and FP64 matches the oracle per grid."""
+    print(f"case: shallow water stream variants {integ} {prec}", flush=True)
+    cases = [_mixed_bc_shallow_water(21, 130), _mixed_bc_shallow_water(34, 66),
+             _mixed_bc_shallow_water(17, 57)]
+    _check_stream_variants("shallow_water", cases, integ, prec, 2e-3)
 
 
 def test_ragged_batch_matches_per_grid():
