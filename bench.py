@@ -43,7 +43,7 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="ch256",
-                    choices=["ch256", "parareal", "copy", "surrogate"])
+                    choices=["ch256", "parareal", "swe-parareal", "copy", "surrogate"])
     ap.add_argument("--sur-batch", type=int, default=32)
     ap.add_argument("--sur-size", type=int, default=129, help="grid vertices per axis")
     # Parareal workload (2-D diffusion, learned vs FDM coarse)
@@ -71,6 +71,18 @@ def parse_args():
     ap.add_argument("--pr-fine-precision", default="float32c")
     ap.add_argument("--pr-no-learned", action="store_true",
                     help="skip the learned coarse operator (FDM coarse and serial only)")
+    # Shallow-water Parareal workload (learned vs FDM coarse on 1024^2)
+    ap.add_argument("--swe-n", type=int, default=1024, help="cells per axis (vertices n + 1)")
+    ap.add_argument("--swe-slices", type=int, default=64)
+    ap.add_argument("--swe-dt", type=float, default=1e-3, help="fine RK4 step")
+    ap.add_argument("--swe-fine-steps", type=int, default=40, help="fine RK4 steps per slice")
+    ap.add_argument("--swe-coarse-steps", type=int, default=8,
+                    help="coarse RK4 steps per slice on the 2x grid")
+    ap.add_argument("--swe-traj", type=int, default=8, help="training trajectories")
+    ap.add_argument("--swe-train-steps", type=int, default=600)
+    ap.add_argument("--swe-train-batch", type=int, default=4)
+    ap.add_argument("--swe-target", type=float, default=1e-5)
+    ap.add_argument("--swe-fine-precision", default="float32c")
     ap.add_argument("--tile-config", type=int, default=None)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
@@ -342,6 +354,206 @@ def parareal_workload(args):
     print(json.dumps(line), flush=True)
 
 
+def _swe_bumps(cp, n, rng):
+    """n shallow-water states: eta = 0.1 N(x; mu, 0.25 I) with mu uniform in
+    the middle half of each axis, u = v = 0; shape (n, *vertices, 3)."""
+    import pararealml_amd as pml
+
+    lo = [iv[0] for iv in cp.mesh.x_intervals]
+    span = [iv[1] - iv[0] for iv in cp.mesh.x_intervals]
+    out = []
+    for _ in range(n):
+        mean = np.array([lo[a] + span[a] * rng.uniform(0.25, 0.75) for a in range(2)])
+        ic = pml.GaussianInitialCondition(cp, [(mean, 0.25 * np.eye(2))] * 3, [0.1, 0.0, 0.0])
+        out.append(ic.discrete_y_0(True))
+    return np.stack(out)
+
+
+def swe_parareal_workload(args):
+    """Parareal for the shallow-water equations on one GPU, judged on all
+    three fields against the FP64 serial fine solution at the slice ends.
+
+    ShallowWaterEquation(h=1, v=0.01, g=9.81) on [-5, 5]^2 with (n + 1)^2
+    vertices, zero-Neumann (reflective) edges on eta, u and v; eta_0 =
+    0.1 N(x; mu, 0.25 I) with mu from RandomState(1462316) in the middle half
+    of the domain, u = v = 0.  S slices of `--swe-fine-steps` fine RK4 steps
+    of `--swe-dt`; FDM coarse = RK4 with `--swe-coarse-steps` steps per slice
+    on the 2x-coarsened grid; learned coarse = that coarse solve, prolonged
+    and corrected on the fine grid by one 3-field conv-surrogate application
+    per slice (evaluated in the initial sweep and frozen), trained on fine
+    trajectories from other seeded bumps, FDM coarse as fallback; the same
+    pipeline without the network is the ablation."""
+    import torch
+
+    import pararealml_amd as pml
+    from pararealml_amd import _device as dev
+    from pararealml_amd.operators.fdm import RK4, FDMOperator, ThreePointCentralDifferenceMethod
+    from pararealml_amd.operators.ml import (
+        ConvSurrogate,
+        CorrectedCoarseOperator,
+        corrected_pairs,
+        generate_pairs,
+    )
+    from pararealml_amd.operators.ml.learned_coarse import coarse_problem
+    from pararealml_amd.operators.parareal import CoarseGridFDMOperator, PararealOperator
+
+    torch.cuda.set_device(0)
+    n, S = args.swe_n, args.swe_slices
+    dt_f = args.swe_dt
+    delta = dt_f * args.swe_fine_steps
+    t_end = delta * S
+    target = args.swe_target
+    diff = ThreePointCentralDifferenceMethod()
+    eq = pml.ShallowWaterEquation(h=1.0, v=0.01, g=9.81)
+    mesh = pml.Mesh([(-5.0, 5.0), (-5.0, 5.0)], [10.0 / n, 10.0 / n])
+    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    rng = np.random.RandomState(SEEDS[2])
+    y0 = _swe_bumps(cp, 1, rng)[0]
+    ivp = pml.InitialValueProblem(cp, (0.0, t_end), pml.DiscreteInitialCondition(cp, y0, True))
+    problem = dev.device_problem(cp)
+    fp = args.swe_fine_precision
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+
+    log("swe-parareal: tuning kernel variants")
+    for prec in ("float64", fp):
+        FDMOperator(RK4(), diff, dt_f, precision=prec).tune(cp, 1)
+    FDMOperator(RK4(), diff, dt_f, precision=fp).tune(cp, S)
+    dt_c = delta / args.swe_coarse_steps
+    FDMOperator(RK4(), diff, dt_c).tune(coarse_problem(cp), 1)
+
+    def timed_serial(precision):
+        op = FDMOperator(RK4(), diff, dt_f, precision=precision,
+                         snapshot_every=args.swe_fine_steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sol = op.solve(ivp)
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, sol.discrete_y()
+
+    log("swe-parareal: FP64 serial fine reference")
+    t_ref, ref = timed_serial("float64")
+    # Slice-end states in the device layout (field-major), as the Parareal
+    # monitor sees them.
+    ref_dev = torch.stack([
+        torch.from_numpy(problem.to_device_layout(ref[k]).reshape(3, -1)) for k in range(S)
+    ]).cuda()
+    ref_norm = torch.linalg.norm(ref_dev, dim=2)
+
+    def field_errors(end_points):
+        e = torch.linalg.norm(end_points.view(S, 3, -1) - ref_dev, dim=2) / ref_norm
+        return e.max(dim=0).values.tolist()
+
+    log("swe-parareal: serial fine at the fine precision")
+    t_serial, y_serial = timed_serial(fp)
+    ser = torch.stack([
+        torch.from_numpy(problem.to_device_layout(y_serial[k]).reshape(3, -1)) for k in range(S)
+    ]).cuda()
+    err_serial = field_errors(ser)
+    del ser
+
+    def run_parareal(g, name):
+        history = []
+
+        def monitor(i, u):
+            e = field_errors(u[1:])
+            history.append((i + 1, e))
+            return max(e) <= target
+
+        f = FDMOperator(RK4(), diff, dt_f, precision=fp)
+        p = PararealOperator(f, g, 0.0, S, n_slices=S, output="endpoints", monitor=monitor)
+        log(f"swe-parareal: {name}")
+        p.solve(ivp)
+        st = p.last_stats
+        hit = next((k for k, (it, e) in enumerate(history) if max(e) <= target), None)
+        return {
+            "iterations_to_target": history[hit][0] if hit is not None else None,
+            "time_to_target_s": st["iteration_wall_s"][hit] if hit is not None else None,
+            "field_errors_per_iteration": [e for _, e in history],
+            "iteration_wall_s": st["iteration_wall_s"],
+            "fine_s": st["fine_s"],
+            "coarse_s": st["coarse_s"],
+            "correction_s": st["correction_s"],
+        }
+
+    base = FDMOperator(RK4(), diff, dt_c)
+    g_fdm = CoarseGridFDMOperator(base)
+    fdm_res = run_parareal(g_fdm, "FDM coarse")
+
+    log(f"swe-parareal: {args.swe_traj} training trajectories")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    y0s = _swe_bumps(cp, args.swe_traj, rng)
+    data_op = FDMOperator(RK4(), diff, dt_f, precision="float32")
+    snaps = generate_pairs(data_op, cp, y0s, t_end, delta, coarsen=False)
+    inputs, targets = corrected_pairs(base, cp, snaps, delta, base_levels=1, correct_on="fine")
+    del snaps
+    torch.cuda.synchronize()
+    t_data = time.perf_counter() - t0
+    net = ConvSurrogate(3, seed=0)
+    log(f"swe-parareal: training on {inputs.shape[0]} pairs")
+    t0 = time.perf_counter()
+    losses = net.fit(inputs, targets, steps=args.swe_train_steps, batch=args.swe_train_batch,
+                     lr=1e-3, log_every=max(1, args.swe_train_steps // 10))
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    n_pairs = int(inputs.shape[0])
+    del inputs, targets
+    torch.cuda.empty_cache()
+    g_learned = CorrectedCoarseOperator(base, net, fallback=g_fdm, base_levels=1,
+                                        correct_on="fine")
+    learned_res = run_parareal(g_learned, "learned coarse")
+    g_plain = CorrectedCoarseOperator(base, ConvSurrogate(3, seed=0), fallback=g_fdm,
+                                      base_levels=1, correct_on="fine")
+    plain_res = run_parareal(g_plain, "coarse pipeline without the network")
+
+    line = {
+        "metric": METRIC,
+        "value": learned_res["time_to_target_s"],
+        "unit": "s (learned-coarse Parareal time to target error, all three fields)",
+        "n_gpus": 1,
+        "steps": S * args.swe_fine_steps,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": False,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": fp,
+        "data": "synthetic (seeded Gaussian-bump initial heights)",
+        "config": {
+            "workload": (
+                f"Parareal shallow water h=1 v=0.01 g=9.81, {n + 1}^2 vertices on [-5,5]^2, "
+                f"reflective Neumann edges, t in [0,{t_end:g}], {S} slices, fine RK4 d_t={dt_f:g} "
+                f"({args.swe_fine_steps} steps/slice), FDM coarse RK4 {args.swe_coarse_steps} "
+                "steps/slice on the 2x grid, learned coarse = that solve + one 3-field "
+                "conv-surrogate correction per slice on the fine grid, frozen"
+            ),
+            "target_rel_error": target,
+        },
+        "parareal": {
+            "serial_fine_fp64_s": t_ref,
+            "serial_fine_s": t_serial,
+            "serial_fine_field_errors": err_serial,
+            "fdm_coarse": fdm_res,
+            "learned_coarse": learned_res,
+            "learned_fallbacks": g_learned.fallbacks,
+            "coarse_pipeline_without_network": plain_res,
+            "training": {
+                "trajectories": args.swe_traj,
+                "pairs": n_pairs,
+                "data_generation_s": t_data,
+                "train_steps": args.swe_train_steps,
+                "train_batch": args.swe_train_batch,
+                "train_s": t_train,
+                "loss_first": float(np.mean(losses[:10])),
+                "loss_last": float(np.mean(losses[-10:])),
+            },
+        },
+        "host": host_info(),
+    }
+    print(json.dumps(line), flush=True)
+
+
 def _learned_coarse_runs(args, cp, diff, n, S, g_fdm, run_parareal, log):
     """The learned coarse operator's data generation, training and
     Parareal runs (and the ablation without the network)."""
@@ -490,6 +702,9 @@ def main():
 
     if args.workload == "parareal":
         parareal_workload(args)
+        return
+    if args.workload == "swe-parareal":
+        swe_parareal_workload(args)
         return
     if args.workload == "copy":
         copy_workload(args)
