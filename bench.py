@@ -14,6 +14,10 @@ the stepping kernel (algorithmic bytes = 2 fields x (read + write) x 4 B =
 16 B per cell-step, against 8 TB/s), the FP32 error against an FP64 run of
 the same batch, and the CPU baseline (the NumPy oracle, FP64, one core, on a
 bounded sample of the same workload).
+
+Other workloads: ``parareal`` (2-D diffusion, learned vs FDM coarse),
+``swe-parareal`` (shallow water on 1025^2, 64 slices), ``surrogate`` (conv
+surrogate training throughput) and ``copy`` (the HBM counter calibration).
 """
 
 import argparse
