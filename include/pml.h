@@ -22,6 +22,11 @@
  *   pml_stencil_second     <- ThreePointCentralDifferenceMethod._second_derivative
  *                             (same axis) + _add_halos_along_axis
  *                             (numerical_differentiator.py:1059-1095, 1188-1242)
+ *   pml_weighted_add       <- the coordinate factors of the polar, cylindrical
+ *                             and spherical branches of gradient, hessian,
+ *                             divergence, curl, laplacian, vector_laplacian
+ *                             (numerical_differentiator.py:158-173, 229-308,
+ *                             348-400, 482-590, 636-725, 771-870)
  *   pml_jacobi_sweep       <- ThreePointCentralDifferenceMethod.
  *                             _next_anti_laplacian_estimate (Cartesian) and the
  *                             constraint application of anti_laplacian
@@ -154,6 +159,15 @@ int pml_stencil_second(int precision, const void *y, void *out, int64_t outer,
                        int64_t n, int64_t inner, double dx, double dx2,
                        const double *lo, const double *hi, int mode,
                        void *stream);
+
+/*
+ * out (=, +=, -=) coef * x over `count` elements (mode as above); x, coef
+ * and out share the precision.  The curvilinear operators of
+ * NumericalDifferentiator combine stencil results and field components with
+ * coordinate factors (1/r, cos(phi)/sin(phi), ...) through this call.
+ */
+int pml_weighted_add(int precision, const void *x, const void *coef, void *out,
+                     int64_t count, int mode, void *stream);
 
 /*
  * One Jacobi sweep of the Cartesian anti-Laplacian on up to 3 spatial axes

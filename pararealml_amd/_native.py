@@ -79,6 +79,9 @@ SIGNATURES = {
         [c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_double, c_double,
          c_void_p, c_void_p, c_int, c_void_p],
     ),
+    "pml_weighted_add": (
+        c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p]
+    ),
     "pml_jacobi_sweep": (
         c_int,
         [c_int, c_void_p, c_void_p, c_void_p, POINTER(c_int64), c_int,

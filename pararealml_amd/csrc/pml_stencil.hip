@@ -186,6 +186,17 @@ __global__ void k_jacobi_check(JacobiState *st, double tol) {
   }
 }
 
+// out (=, +=, -=) coef * x, element by element: the coordinate-dependent
+// factors of the curvilinear operators (numerical_differentiator.py:158-173,
+// 229-308, 348-400, 482-590, 636-725, 771-870) applied to stencil results
+// and to field components.
+template <typename T>
+__global__ void k_weighted_add(const T *x, const T *coef, T *out, int64_t count, int mode) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
+       t += (int64_t)gridDim.x * blockDim.x)
+    emit(out, t, coef[t] * x[t], mode);
+}
+
 }  // namespace pml
 
 using namespace pml;
@@ -206,6 +217,23 @@ extern "C" int pml_stencil_first(int precision, const void *y, void *out, int64_
                        (float *)out, outer, n, inner, 2.0 * dx, lo, hi, mode);
   else
     return fail(PML_E_INVALID, "stencils take FP32 or FP64 arrays");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_weighted_add(int precision, const void *x, const void *coef, void *out,
+                                int64_t count, int mode, void *stream) {
+  if (count <= 0 || !x || !coef || !out) return fail(PML_E_INVALID, "bad weighted-add arguments");
+  hipStream_t st = as_stream(stream);
+  const int nb = grid_for(count);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_weighted_add<double>, dim3(nb), dim3(256), 0, st, (const double *)x,
+                       (const double *)coef, (double *)out, count, mode);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_weighted_add<float>, dim3(nb), dim3(256), 0, st, (const float *)x,
+                       (const float *)coef, (float *)out, count, mode);
+  else
+    return fail(PML_E_INVALID, "weighted add takes FP32 or FP64 arrays");
   PML_LAUNCH_CHECK();
   return PML_OK;
 }

@@ -18,9 +18,13 @@ branches) and the stencil semantics follow
 
 Inputs may be NumPy arrays (copied to HBM, results copied back) or torch
 tensors already on the GPU (results stay there).  Arrays are in the
-reference layout, channel-last ``(*vertices_shape, y_dim)``.  Curvilinear
-coordinate systems are not covered by the device kernels and raise
-``NotImplementedError``.
+reference layout, channel-last ``(*vertices_shape, y_dim)``.  Polar,
+cylindrical and spherical meshes follow the reference's curvilinear
+branches: each operator is a sum of stencil results and field components
+weighted by coordinate factors (1/r, 1/(r sin phi), cos(phi)/sin(phi), ...),
+every term evaluated on the device (``pml_weighted_add``); the factors are
+computed once per call from the mesh coordinates.  The curvilinear
+anti-Laplacian raises ``NotImplementedError``.
 """
 
 from abc import ABC
@@ -132,6 +136,79 @@ def _second(src: "_Arr", t, out, axis: int, d_x: float, d_x2: float, pairs, mode
     )
 
 
+def _weights(src: "_Arr", coef, shape):
+    import torch
+
+    dtype = np.float64 if src.prec == nat.PML_F64 else np.float32
+    arr = np.ascontiguousarray(np.broadcast_to(coef, tuple(shape)), dtype=dtype)
+    return torch.from_numpy(arr).to("cuda")
+
+
+class _Sum:
+    """``out`` = a sum of terms, each a stencil result or a field component
+    times an optional per-vertex factor; every term runs on the device (the
+    first term writes, the rest accumulate)."""
+
+    def __init__(self, src: "_Arr", shape):
+        self.a = src
+        self.shape = tuple(shape)
+        self.out = src.new(self.shape)
+        self.mode = 0
+        self._tmp = None
+
+    def _scratch(self):
+        if self._tmp is None:
+            self._tmp = self.a.new(self.shape)
+        return self._tmp
+
+    def _weighted(self, x, coef):
+        c = _weights(self.a, coef, self.shape)
+        nat.check(
+            nat.device_lib().pml_weighted_add(
+                self.a.prec, x.data_ptr(), c.data_ptr(), self.out.data_ptr(), self.out.numel(),
+                self.mode, _stream(),
+            )
+        )
+        self.mode = 1
+
+    def first(self, y, axis: int, d_x: float, pairs, coef=None):
+        if coef is None or (isinstance(coef, float) and coef == -1.0 and self.mode == 1):
+            _first(self.a, y, self.out, axis, d_x, pairs, -1 if coef is not None else self.mode)
+            self.mode = 1
+        else:
+            t = self._scratch()
+            _first(self.a, y, t, axis, d_x, pairs, 0)
+            self._weighted(t, coef)
+
+    def second(self, y, axis: int, d_x: float, pairs, coef=None):
+        if coef is None:
+            _second(self.a, y, self.out, axis, d_x, d_x, pairs, self.mode)
+            self.mode = 1
+        else:
+            t = self._scratch()
+            _second(self.a, y, t, axis, d_x, d_x, pairs, 0)
+            self._weighted(t, coef)
+
+    def mixed(self, y, axis1: int, axis2: int, d_x1: float, d_x2: float, pairs, coef=None):
+        inner = self.a.new(self.shape)
+        _first(self.a, y, inner, axis1, d_x1, pairs, 0)
+        self.first(inner, axis2, d_x2, [None] * self.shape[-1], coef)
+
+    def value(self, y, coef):
+        self._weighted(y, coef)
+
+
+def _polar_factors(mesh: Mesh):
+    """r, sin(phi), cos(phi) on the vertices as (*vertices, 1) arrays; the
+    angle terms are None outside spherical coordinates."""
+    grids = mesh.vertex_coordinate_grids
+    r = grids[0][..., np.newaxis]
+    if mesh.coordinate_system_type == CoordinateSystem.SPHERICAL:
+        phi = grids[2][..., np.newaxis]
+        return r, np.sin(phi), np.cos(phi)
+    return r, None, None
+
+
 # --------------------------------------------------------------------------
 
 
@@ -182,7 +259,7 @@ class NumericalDifferentiator(ABC):
     def _require_cartesian(mesh: Mesh):
         if mesh.coordinate_system_type != CoordinateSystem.CARTESIAN:
             raise NotImplementedError(
-                "the device stencils cover Cartesian meshes; "
+                "the device Jacobi solve covers Cartesian meshes; "
                 f"got {mesh.coordinate_system_type.name}"
             )
 
@@ -211,12 +288,19 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
         dbc = self._verify_and_get_derivative_boundary_constraints(
             derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
         )
-        self._require_cartesian(mesh)
         self._require_points(y.shape, x_axis)
         a = _Arr(y)
-        out = a.new(a.t.shape)
-        _first(a, a.t, out, x_axis, mesh.d_x[x_axis], dbc[x_axis], 0)
-        return a.give(out)
+        out = _Sum(a, a.t.shape)
+        cs = mesh.coordinate_system_type
+        coef = None
+        if cs != CoordinateSystem.CARTESIAN:
+            r, sin_phi, _ = _polar_factors(mesh)
+            if cs == CoordinateSystem.SPHERICAL:
+                coef = {0: None, 1: 1.0 / (r * sin_phi), 2: 1.0 / r}[x_axis]
+            elif x_axis == 1:
+                coef = 1.0 / r
+        out.first(a.t, x_axis, mesh.d_x[x_axis], dbc[x_axis], coef)
+        return a.give(out.out)
 
     def hessian(
         self, y, mesh: Mesh, x_axis1: int, x_axis2: int, derivative_boundary_constraints=None
@@ -232,40 +316,98 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
         dbc = self._verify_and_get_derivative_boundary_constraints(
             derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
         )
-        self._require_cartesian(mesh)
+        self._require_points(y.shape, x_axis1)
+        self._require_points(y.shape, x_axis2)
         a = _Arr(y)
-        out = a.new(a.t.shape)
-        if x_axis1 == x_axis2:
-            self._require_points(y.shape, x_axis1)
-            _second(a, a.t, out, x_axis1, mesh.d_x[x_axis1], mesh.d_x[x_axis2],
-                    dbc[x_axis1], 0)
+        out = _Sum(a, a.t.shape)
+        d = mesh.d_x
+
+        def second(coef=None):
+            if x_axis1 == x_axis2:
+                out.second(a.t, x_axis1, d[x_axis1], dbc[x_axis1], coef)
+            else:
+                out.mixed(a.t, x_axis1, x_axis2, d[x_axis1], d[x_axis2], dbc[x_axis1], coef)
+
+        def first(axis, coef):
+            self._require_points(y.shape, axis)
+            out.first(a.t, axis, d[axis], dbc[axis], coef)
+
+        cs = mesh.coordinate_system_type
+        axes = {x_axis1, x_axis2}
+        if cs == CoordinateSystem.CARTESIAN:
+            second()
+        elif cs == CoordinateSystem.SPHERICAL:
+            r, sin_phi, cos_phi = _polar_factors(mesh)
+            if axes == {0}:
+                second()
+            elif axes == {1}:
+                second(1.0 / (r * sin_phi) ** 2)
+                first(0, 1.0 / r)
+                first(2, cos_phi / (r * r * sin_phi))
+            elif axes == {2}:
+                second(1.0 / (r * r))
+                first(0, 1.0 / r)
+            elif axes == {0, 1}:
+                second(1.0 / (r * sin_phi))
+                first(1, -1.0 / (r * r * sin_phi))
+            elif axes == {0, 2}:
+                second(1.0 / r)
+                first(2, -1.0 / (r * r))
+            else:
+                second(1.0 / (r * r * sin_phi))
+                first(1, -cos_phi / (r * sin_phi) ** 2)
         else:
-            self._require_points(y.shape, x_axis1)
-            self._require_points(y.shape, x_axis2)
-            tmp = a.new(a.t.shape)
-            _first(a, a.t, tmp, x_axis1, mesh.d_x[x_axis1], dbc[x_axis1], 0)
-            _first(a, tmp, out, x_axis2, mesh.d_x[x_axis2], [None] * y.shape[-1], 0)
-        return a.give(out)
+            r, _, _ = _polar_factors(mesh)
+            if axes <= {0, 2}:
+                second()
+            elif axes == {1}:
+                second(1.0 / (r * r))
+                first(0, 1.0 / r)
+            elif axes == {0, 1}:
+                second(1.0 / r)
+                first(1, -1.0 / (r * r))
+            else:
+                second(1.0 / r)
+        return a.give(out.out)
 
     def divergence(self, y, mesh: Mesh, derivative_boundary_constraints=None):
-        """numerical_differentiator.py:310-400 (Cartesian: sum of the
-        component derivatives accumulated in component order)."""
+        """numerical_differentiator.py:310-400 (Cartesian: the component
+        derivatives accumulated in component order)."""
         self._verify_input_is_a_vector_field(y, mesh)
         dbc = self._verify_and_get_derivative_boundary_constraints(
             derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
         )
-        self._require_cartesian(mesh)
-        a = _Arr(y)
-        out = a.new(tuple(a.t.shape[:-1]) + (1,))
-        out.zero_()
         for i in range(y.shape[-1]):
             self._require_points(y.shape, i)
-            comp = a.t[..., i : i + 1].contiguous()
-            _first(a, comp, out, i, mesh.d_x[i], dbc[i, i : i + 1], 1)
-        return a.give(out)
+        a = _Arr(y)
+        out = _Sum(a, tuple(a.t.shape[:-1]) + (1,))
+        comp = [a.t[..., i : i + 1].contiguous() for i in range(y.shape[-1])]
+
+        def d(i, coef=None):
+            out.first(comp[i], i, mesh.d_x[i], dbc[i, i : i + 1], coef)
+
+        cs = mesh.coordinate_system_type
+        if cs == CoordinateSystem.CARTESIAN:
+            for i in range(y.shape[-1]):
+                d(i)
+        elif cs == CoordinateSystem.SPHERICAL:
+            r, sin_phi, cos_phi = _polar_factors(mesh)
+            d(0)
+            d(2, 1.0 / r)
+            out.value(comp[0], 2.0 / r)
+            d(1, 1.0 / (r * sin_phi))
+            out.value(comp[2], cos_phi / (r * sin_phi))
+        else:
+            r, _, _ = _polar_factors(mesh)
+            d(0)
+            out.value(comp[0], 1.0 / r)
+            d(1, 1.0 / r)
+            if cs == CoordinateSystem.CYLINDRICAL:
+                d(2)
+        return a.give(out.out)
 
     def curl(self, y, mesh: Mesh, curl_ind: int = 0, derivative_boundary_constraints=None):
-        """numerical_differentiator.py:402-590 (Cartesian)."""
+        """numerical_differentiator.py:402-590."""
         self._verify_input_is_a_vector_field(y, mesh)
         if not (2 <= mesh.dimensions <= 3):
             raise ValueError(f"number of x dimensions ({mesh.dimensions}) must be 2 or 3")
@@ -279,47 +421,98 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
         dbc = self._verify_and_get_derivative_boundary_constraints(
             derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
         )
-        self._require_cartesian(mesh)
-        if mesh.dimensions == 2 or curl_ind == 2:
-            (pc, pa), (mc, ma) = (1, 0), (0, 1)
-        elif curl_ind == 0:
-            (pc, pa), (mc, ma) = (2, 1), (1, 2)
-        else:
-            (pc, pa), (mc, ma) = (0, 2), (2, 0)
-        self._require_points(y.shape, pa)
-        self._require_points(y.shape, ma)
         a = _Arr(y)
-        out = a.new(tuple(a.t.shape[:-1]) + (1,))
-        plus = a.t[..., pc : pc + 1].contiguous()
-        minus = a.t[..., mc : mc + 1].contiguous()
-        _first(a, plus, out, pa, mesh.d_x[pa], dbc[pa, pc : pc + 1], 0)
-        _first(a, minus, out, ma, mesh.d_x[ma], dbc[ma, mc : mc + 1], -1)
-        return a.give(out)
+        out = _Sum(a, tuple(a.t.shape[:-1]) + (1,))
+        comp = [a.t[..., i : i + 1].contiguous() for i in range(y.shape[-1])]
+
+        def d(c, axis, coef=None):
+            """coef * d(y_c)/d(x_axis), the constraints of (axis, c)."""
+            self._require_points(y.shape, axis)
+            out.first(comp[c], axis, mesh.d_x[axis], dbc[axis, c : c + 1], coef)
+
+        cs = mesh.coordinate_system_type
+        planar = mesh.dimensions == 2 or curl_ind == 2
+        if cs == CoordinateSystem.CARTESIAN:
+            if planar:
+                (pc, pa), (mc, ma) = (1, 0), (0, 1)
+            elif curl_ind == 0:
+                (pc, pa), (mc, ma) = (2, 1), (1, 2)
+            else:
+                (pc, pa), (mc, ma) = (0, 2), (2, 0)
+            d(pc, pa)
+            d(mc, ma, -1.0)
+        elif cs == CoordinateSystem.SPHERICAL:
+            r, sin_phi, cos_phi = _polar_factors(mesh)
+            if curl_ind == 0:
+                d(1, 2, 1.0 / r)
+                out.value(comp[1], cos_phi / (r * sin_phi))
+                d(2, 1, -1.0 / (r * sin_phi))
+            elif curl_ind == 1:
+                d(2, 0)
+                out.value(comp[2], 1.0 / r)
+                d(0, 2, -1.0 / r)
+            else:
+                d(1, 0, -1.0)
+                d(0, 1, 1.0 / (r * sin_phi))
+                out.value(comp[1], -1.0 / r)
+        else:
+            r, _, _ = _polar_factors(mesh)
+            if cs == CoordinateSystem.POLAR or curl_ind == 2:
+                d(1, 0)
+                out.value(comp[1], 1.0 / r)
+                d(0, 1, -1.0 / r)
+            elif curl_ind == 0:
+                d(2, 1, 1.0 / r)
+                d(1, 2, -1.0)
+            else:
+                d(0, 2)
+                d(2, 0, -1.0)
+        return a.give(out.out)
 
     def laplacian(self, y, mesh: Mesh, derivative_boundary_constraints=None):
-        """Element-wise scalar Laplacian, accumulated over axes in order
-        (numerical_differentiator.py:592-634)."""
+        """Element-wise scalar Laplacian (numerical_differentiator.py:592-725;
+        Cartesian: accumulated over axes in order)."""
         self._verify_input_shape_matches_mesh(y, mesh)
         dbc = self._verify_and_get_derivative_boundary_constraints(
             derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
         )
-        self._require_cartesian(mesh)
         a = _Arr(y)
         return a.give(self._laplacian_dev(a, a.t, mesh, dbc))
 
     def _laplacian_dev(self, a, t, mesh, dbc):
-        out = a.new(t.shape)
+        out = _Sum(a, t.shape)
+        self._laplacian_terms(out, t, mesh, dbc)
+        return out.out
+
+    def _laplacian_terms(self, out, t, mesh, dbc):
         for axis in range(t.dim() - 1):
             self._require_points(t.shape, axis)
-            _second(a, t, out, axis, mesh.d_x[axis], mesh.d_x[axis], dbc[axis, :],
-                    0 if axis == 0 else 1)
-        return out
+        d = mesh.d_x
+        cs = mesh.coordinate_system_type
+        if cs == CoordinateSystem.CARTESIAN:
+            for axis in range(t.dim() - 1):
+                out.second(t, axis, d[axis], dbc[axis, :])
+        elif cs == CoordinateSystem.SPHERICAL:
+            r, sin_phi, cos_phi = _polar_factors(mesh)
+            out.second(t, 0, d[0], dbc[0, :])
+            out.first(t, 0, d[0], dbc[0, :], 2.0 / r)
+            out.second(t, 2, d[2], dbc[2, :], 1.0 / (r * r))
+            out.first(t, 2, d[2], dbc[2, :], cos_phi / (r * r * sin_phi))
+            out.second(t, 1, d[1], dbc[1, :], 1.0 / (r * sin_phi) ** 2)
+        else:
+            r, _, _ = _polar_factors(mesh)
+            out.second(t, 0, d[0], dbc[0, :])
+            out.second(t, 1, d[1], dbc[1, :], 1.0 / (r * r))
+            out.first(t, 0, d[0], dbc[0, :], 1.0 / r)
+            if cs == CoordinateSystem.CYLINDRICAL:
+                out.second(t, 2, d[2], dbc[2, :])
 
     def vector_laplacian(
         self, y, mesh: Mesh, vector_laplacian_ind: int, derivative_boundary_constraints=None
     ):
-        """numerical_differentiator.py:727-870 (Cartesian: the scalar
-        Laplacian of the selected component)."""
+        """numerical_differentiator.py:727-870: the scalar Laplacian of the
+        selected component plus, in curvilinear coordinates, the coupling
+        terms of the other components."""
         self._verify_input_is_a_vector_field(y, mesh)
         if not (0 <= vector_laplacian_ind < mesh.dimensions):
             raise ValueError(
@@ -330,11 +523,43 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
         dbc = self._verify_and_get_derivative_boundary_constraints(
             derivative_boundary_constraints, mesh.dimensions, y.shape[-1]
         )
-        self._require_cartesian(mesh)
         k = vector_laplacian_ind
         a = _Arr(y)
-        comp = a.t[..., k : k + 1].contiguous()
-        return a.give(self._laplacian_dev(a, comp, mesh, dbc[:, k : k + 1]))
+        comp = [a.t[..., i : i + 1].contiguous() for i in range(y.shape[-1])]
+        out = _Sum(a, comp[k].shape)
+        self._laplacian_terms(out, comp[k], mesh, dbc[:, k : k + 1])
+        cs = mesh.coordinate_system_type
+        if cs == CoordinateSystem.CARTESIAN:
+            return a.give(out.out)
+
+        def d(c, axis, coef):
+            self._require_points(y.shape, axis)
+            out.first(comp[c], axis, mesh.d_x[axis], dbc[axis, c : c + 1], coef)
+
+        r, sin_phi, cos_phi = _polar_factors(mesh)
+        r2 = r * r
+        if cs == CoordinateSystem.SPHERICAL:
+            if k == 1:
+                out.value(comp[0], -2.0 / r2)
+                d(2, 2, -2.0 / r2)
+                out.value(comp[2], -2.0 * cos_phi / (r2 * sin_phi))
+                d(1, 1, -2.0 / (r2 * sin_phi))
+            elif k == 2:
+                d(0, 1, 2.0 / (r2 * sin_phi))
+                d(2, 1, 2.0 * cos_phi / (r2 * sin_phi * sin_phi))
+                out.value(comp[1], -1.0 / (r2 * sin_phi * sin_phi))
+            else:
+                d(0, 2, 2.0 / r2)
+                out.value(comp[2], -1.0 / (r2 * sin_phi * sin_phi))
+                d(1, 1, -2.0 * cos_phi / (r2 * sin_phi * sin_phi))
+        else:
+            if k == 0:
+                out.value(comp[0], -1.0 / r2)
+                d(1, 1, -2.0 / r2)
+            elif k == 1:
+                out.value(comp[1], -1.0 / r2)
+                d(0, 1, 2.0 / r2)
+        return a.give(out.out)
 
     def anti_laplacian(
         self,
