@@ -31,6 +31,9 @@
  *                             _next_anti_laplacian_estimate (Cartesian) and the
  *                             constraint application of anti_laplacian
  *                             (numerical_differentiator.py:872-927, 1097-1186)
+ *   pml_jacobi_solve_weighted <- the polar / cylindrical / spherical branches
+ *                             of _next_anti_laplacian_estimate
+ *                             (numerical_differentiator.py:1097-1186)
  *   pml_jacobi_solve       <- the anti_laplacian loop itself
  *                             (numerical_differentiator.py:904-925)
  *   pml_parareal_update    <- the per-slice update U[j+1] = G[j] + corr[j] and
@@ -199,6 +202,25 @@ int pml_jacobi_solve(int precision, void *y, void *work, const void *lap,
                      const double *ycons, double tol, int64_t max_sweeps,
                      int64_t batch, void *state, int64_t *sweeps_out,
                      double *diff_out, void *stream);
+
+/*
+ * pml_jacobi_solve with per-vertex weights, for polar, cylindrical and
+ * spherical meshes (numerical_differentiator.py:1097-1186): each sweep sets
+ * y_new = (sum_a w_sum[a] (prev_a + next_a) + w_diff[a] (next_a - prev_a)
+ * - lap) / w_den, where prev_a / next_a are the axis-a neighbours (ghosts
+ * as in pml_jacobi_sweep).  w_sum[a] (required for every axis with more
+ * than one vertex), w_diff[a] (NULL entries or a NULL array: no term) and
+ * w_den are FP64 arrays over the vertices (shape[0] x shape[1] x shape[2],
+ * C order), shared by all fields.
+ */
+int pml_jacobi_solve_weighted(int precision, void *y, void *work, const void *lap,
+                              const int64_t *shape, int n_fields, const double *dx,
+                              const double *const *ghost_lo,
+                              const double *const *ghost_hi, const double *ycons,
+                              double tol, int64_t max_sweeps, int64_t batch,
+                              void *state, int64_t *sweeps_out, double *diff_out,
+                              const double *const *w_sum, const double *const *w_diff,
+                              const double *w_den, void *stream);
 
 /*
  * Parareal, all FP64.

@@ -95,6 +95,13 @@ SIGNATURES = {
          c_double, c_int64, c_int64, c_void_p, POINTER(c_int64), POINTER(c_double),
          c_void_p],
     ),
+    "pml_jacobi_solve_weighted": (
+        c_int,
+        [c_int, c_void_p, c_void_p, c_void_p, POINTER(c_int64), c_int,
+         POINTER(c_double), POINTER(c_void_p), POINTER(c_void_p), c_void_p,
+         c_double, c_int64, c_int64, c_void_p, POINTER(c_int64), POINTER(c_double),
+         POINTER(c_void_p), POINTER(c_void_p), c_void_p, c_void_p],
+    ),
     "pml_parareal_correction": (
         c_int,
         [c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_int64,

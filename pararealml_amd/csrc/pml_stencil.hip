@@ -87,6 +87,11 @@ struct JacobiGeom {
   const double *ghi[3];
   int active[3];
   double coef;  // sum over active axes of 2 / dx^2
+  // Curvilinear weights per vertex (NULL: Cartesian): the update is
+  // (sum_a wsum[a] (prev + next) + wdiff[a] (next - prev) - lap) / wden.
+  const double *wsum[3];
+  const double *wdiff[3];
+  const double *wden;
 };
 
 // One Jacobi update of the Cartesian anti-Laplacian
@@ -142,10 +147,15 @@ __global__ void k_jacobi(const T *y, T *yn, const T *lap, int64_t total, int nf,
         const double gv = g.ghi[a] ? g.ghi[a][slab] : NAN;
         next = (gv == gv) ? y[t - s] + static_cast<T>(2.0 * g.dx[a]) * static_cast<T>(gv) : T(0);
       }
-      acc = acc + (prev + next) / static_cast<T>(g.dxsq[a]);
+      if (g.wden) {
+        acc = acc + static_cast<T>(g.wsum[a][cell]) * (prev + next);
+        if (g.wdiff[a]) acc = acc + static_cast<T>(g.wdiff[a][cell]) * (next - prev);
+      } else {
+        acc = acc + (prev + next) / static_cast<T>(g.dxsq[a]);
+      }
     }
     acc = acc - lap[t];
-    T v = acc / static_cast<T>(g.coef);
+    T v = acc / static_cast<T>(g.wden ? g.wden[cell] : g.coef);
     if (ycons) {
       const double c = ycons[t];
       if (c == c) v = static_cast<T>(c);
@@ -317,12 +327,13 @@ extern "C" int pml_jacobi_sweep(int precision, const void *y_old, void *y_new, c
   return PML_OK;
 }
 
-extern "C" int pml_jacobi_solve(int precision, void *y, void *work, const void *lap,
-                                const int64_t *shape, int n_fields, const double *dx,
-                                const double *const *ghost_lo, const double *const *ghost_hi,
-                                const double *ycons, double tol, int64_t max_sweeps,
-                                int64_t batch, void *state, int64_t *sweeps_out,
-                                double *diff_out, void *stream) {
+static int jacobi_solve(int precision, void *y, void *work, const void *lap,
+                        const int64_t *shape, int n_fields, const double *dx,
+                        const double *const *ghost_lo, const double *const *ghost_hi,
+                        const double *ycons, double tol, int64_t max_sweeps, int64_t batch,
+                        void *state, int64_t *sweeps_out, double *diff_out,
+                        const double *const *w_sum, const double *const *w_diff,
+                        const double *w_den, void *stream) {
   if (!y || !work || !lap || !shape || !dx || !state || !sweeps_out || n_fields <= 0 ||
       max_sweeps <= 0 || batch <= 0 || !(tol >= 0.0))
     return fail(PML_E_INVALID, "bad Jacobi solve arguments");
@@ -332,6 +343,16 @@ extern "C" int pml_jacobi_solve(int precision, void *y, void *work, const void *
   int64_t total = 0;
   if (int rc = jacobi_geom(shape, n_fields, dx, ghost_lo, ghost_hi, &g, &total); rc != PML_OK)
     return rc;
+  if (w_den) {
+    if (!w_sum) return fail(PML_E_INVALID, "weighted Jacobi needs w_sum");
+    for (int a = 0; a < 3; ++a) {
+      if (g.active[a] && !w_sum[a])
+        return fail(PML_E_INVALID, "weighted Jacobi needs w_sum for every axis");
+      g.wsum[a] = w_sum[a];
+      g.wdiff[a] = w_diff ? w_diff[a] : nullptr;
+    }
+    g.wden = w_den;
+  }
   hipStream_t st = as_stream(stream);
   JacobiState *js = static_cast<JacobiState *>(state);
   int nb = grid_for(total);
@@ -374,4 +395,30 @@ extern "C" int pml_jacobi_solve(int precision, void *y, void *work, const void *
   *sweeps_out = host.done ? host.sweeps : -issued;
   if (diff_out) *diff_out = host.last_diff;
   return PML_OK;
+}
+
+extern "C" int pml_jacobi_solve(int precision, void *y, void *work, const void *lap,
+                                const int64_t *shape, int n_fields, const double *dx,
+                                const double *const *ghost_lo, const double *const *ghost_hi,
+                                const double *ycons, double tol, int64_t max_sweeps,
+                                int64_t batch, void *state, int64_t *sweeps_out,
+                                double *diff_out, void *stream) {
+  return jacobi_solve(precision, y, work, lap, shape, n_fields, dx, ghost_lo, ghost_hi, ycons,
+                      tol, max_sweeps, batch, state, sweeps_out, diff_out, nullptr, nullptr,
+                      nullptr, stream);
+}
+
+extern "C" int pml_jacobi_solve_weighted(int precision, void *y, void *work, const void *lap,
+                                         const int64_t *shape, int n_fields, const double *dx,
+                                         const double *const *ghost_lo,
+                                         const double *const *ghost_hi, const double *ycons,
+                                         double tol, int64_t max_sweeps, int64_t batch,
+                                         void *state, int64_t *sweeps_out, double *diff_out,
+                                         const double *const *w_sum,
+                                         const double *const *w_diff, const double *w_den,
+                                         void *stream) {
+  if (!w_sum || !w_den) return fail(PML_E_INVALID, "weighted Jacobi needs w_sum and w_den");
+  return jacobi_solve(precision, y, work, lap, shape, n_fields, dx, ghost_lo, ghost_hi, ycons,
+                      tol, max_sweeps, batch, state, sweeps_out, diff_out, w_sum, w_diff, w_den,
+                      stream);
 }

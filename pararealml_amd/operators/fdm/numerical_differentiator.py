@@ -23,8 +23,8 @@ cylindrical and spherical meshes follow the reference's curvilinear
 branches: each operator is a sum of stencil results and field components
 weighted by coordinate factors (1/r, 1/(r sin phi), cos(phi)/sin(phi), ...),
 every term evaluated on the device (``pml_weighted_add``); the factors are
-computed once per call from the mesh coordinates.  The curvilinear
-anti-Laplacian raises ``NotImplementedError``.
+computed once per call from the mesh coordinates (for the anti-Laplacian:
+per-vertex Jacobi weights, ``pml_jacobi_solve_weighted``).
 """
 
 from abc import ABC
@@ -198,6 +198,37 @@ class _Sum:
         self._weighted(y, coef)
 
 
+def _jacobi_weights(mesh: Mesh):
+    """Per-vertex weights of the curvilinear Jacobi update
+    (numerical_differentiator.py:1097-1186): the new estimate is
+    (sum_a w_sum[a] (prev + next) + w_diff[a] (next - prev) - laplacian)
+    / w_den.  Lists of three FP64 vertex arrays (None: no term) and the
+    denominator array."""
+    r, sin_phi, cos_phi = _polar_factors(mesh)
+    r, sin_phi, cos_phi = (None if a is None else a[..., 0] for a in (r, sin_phi, cos_phi))
+    dx = list(mesh.d_x) + [1.0] * (3 - mesh.dimensions)
+    sq = [d * d for d in dx]
+    r2 = r * r
+    ones = np.ones(mesh.vertices_shape)
+    w_sum = [ones / sq[0], None, None]
+    w_diff = [None, None, None]
+    if mesh.coordinate_system_type == CoordinateSystem.SPHERICAL:
+        r2s2 = r2 * sin_phi * sin_phi
+        w_diff[0] = 1.0 / (dx[0] * r)
+        w_sum[1] = 1.0 / (sq[1] * r2s2)
+        w_sum[2] = 1.0 / (sq[2] * r2)
+        w_diff[2] = cos_phi / (2.0 * dx[2] * sin_phi * r2)
+        w_den = 2.0 / sq[0] + 2.0 / (sq[1] * r2s2) + 2.0 / (sq[2] * r2)
+    else:
+        w_diff[0] = 1.0 / (2.0 * dx[0] * r)
+        w_sum[1] = 1.0 / (sq[1] * r2)
+        w_den = 2.0 / sq[0] + 2.0 / (sq[1] * r2)
+        if mesh.coordinate_system_type == CoordinateSystem.CYLINDRICAL:
+            w_sum[2] = ones / sq[2]
+            w_den = w_den + 2.0 / sq[2]
+    return w_sum, w_diff, w_den
+
+
 def _polar_factors(mesh: Mesh):
     """r, sin(phi), cos(phi) on the vertices as (*vertices, 1) arrays; the
     angle terms are None outside spherical coordinates."""
@@ -254,14 +285,6 @@ class NumericalDifferentiator(ABC):
                 f"{(x_axes, y_elements)} but got {dbc.shape}"
             )
         return dbc
-
-    @staticmethod
-    def _require_cartesian(mesh: Mesh):
-        if mesh.coordinate_system_type != CoordinateSystem.CARTESIAN:
-            raise NotImplementedError(
-                "the device Jacobi solve covers Cartesian meshes; "
-                f"got {mesh.coordinate_system_type.name}"
-            )
 
     @staticmethod
     def _require_points(shape, axis):
@@ -577,7 +600,6 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
         dbc = self._verify_and_get_derivative_boundary_constraints(
             derivative_boundary_constraints, mesh.dimensions, laplacian.shape[-1]
         )
-        self._require_cartesian(mesh)
         if mesh.dimensions > 3:
             raise NotImplementedError("the device Jacobi sweep covers up to 3 axes")
         if not np.all(np.array(laplacian.shape[:-1]) > 2):
@@ -630,15 +652,26 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
         diff = ctypes.c_double(0.0)
         # the loop and its stopping test run on the device; the host looks
         # at the loop state once per 64 sweeps
-        nat.check(
-            lib.pml_jacobi_solve(
-                lap.prec, cur.data_ptr(), work.data_ptr(), lap.t.data_ptr(), c_shape,
-                n_fields, c_dx, c_lo, c_hi,
-                None if ycons is None else ycons.data_ptr(),
-                float(self._tol), int(self._max_sweeps), 64, state.data_ptr(),
-                ctypes.byref(sweeps), ctypes.byref(diff), _stream(),
-            )
+        args = (
+            lap.prec, cur.data_ptr(), work.data_ptr(), lap.t.data_ptr(), c_shape,
+            n_fields, c_dx, c_lo, c_hi,
+            None if ycons is None else ycons.data_ptr(),
+            float(self._tol), int(self._max_sweeps), 64, state.data_ptr(),
+            ctypes.byref(sweeps), ctypes.byref(diff),
         )
+        if mesh.coordinate_system_type == CoordinateSystem.CARTESIAN:
+            nat.check(lib.pml_jacobi_solve(*args, _stream()))
+        else:
+            w_sum, w_diff, w_den = _jacobi_weights(mesh)
+            dev = [None if w is None else torch.from_numpy(np.ascontiguousarray(w).ravel())
+                   .to("cuda") for w in w_sum + w_diff + [w_den]]
+            ptr = [None if w is None else w.data_ptr() for w in dev]
+            nat.check(
+                lib.pml_jacobi_solve_weighted(
+                    *args, (ctypes.c_void_p * 3)(*ptr[0:3]), (ctypes.c_void_p * 3)(*ptr[3:6]),
+                    ptr[6], _stream(),
+                )
+            )
         if sweeps.value < 0:
             raise RuntimeError(
                 f"the Jacobi iteration did not reach tol={self._tol} in "

@@ -3,9 +3,10 @@
 Pinned three ways: the reference's own known-answer arrays
 (tests/operators/fdm/test_numerical_differentiator.py:670-2457, written to
 tests/golden/curvilinear_kats.json by make_curvilinear_kats.py), the
-reference's property tests (Hessian symmetry, Laplacian = Hessian trace),
-and closed forms that three-point central differences reproduce exactly at
-interior vertices (quadratics in one coordinate).
+reference's property tests (Hessian symmetry, Laplacian = Hessian trace,
+the anti-Laplacian round trips), and closed forms that three-point central
+differences reproduce exactly at interior vertices (quadratics in one
+coordinate).
 
 Tolerances: the KATs use the reference's own check (numpy.allclose defaults,
 rtol 1e-5 and atol 1e-8; several expected values are printed to 8 digits);
@@ -130,7 +131,73 @@ def test_curvilinear_fp32_device_tensors():
     _close(got.double().cpu().numpy(), ref, 1e-5)
 
 
-def test_curvilinear_anti_laplacian_unsupported():
-    mesh = Mesh([(1.0, 5.0), (0.0, 2.0)], [2.0, 1.0], CoordinateSystem.POLAR)
-    with pytest.raises(NotImplementedError):
-        DIFF.anti_laplacian(np.zeros((3, 3, 1)), mesh, None)
+def _edge_constraint(shape, lo0, hi0, lo1, hi1):
+    """Dirichlet values on the faces of axes 0 and 1 (None: free)."""
+    from pararealml_amd.constraint import Constraint
+
+    value = np.full(tuple(shape) + (1,), np.nan)
+    for sl, v in (((0,), lo0), ((-1,), hi0), ((slice(None), 0), lo1), ((slice(None), -1), hi1)):
+        if v is not None:
+            value[sl] = v
+    mask = ~np.isnan(value)
+    return Constraint(value[mask], mask)
+
+
+@pytest.mark.parametrize("system", ["POLAR", "CYLINDRICAL", "SPHERICAL"])
+def test_curvilinear_anti_laplacian_round_trip(system):
+    """The reference's test_3pcfdm_<system>_anti_laplacian: Dirichlet faces,
+    tol 1e-12 (1e-10 in 3-D); the Laplacian of the result is the input and the result is
+    the original field (numpy.allclose, as the reference)."""
+    print(f"case: anti-laplacian {system}", flush=True)
+    # 3-D grids hold 16,000 values, whose FP64 update norm bottoms out near
+    # 1e-12: they stop at 1e-10.
+    diff = ThreePointCentralDifferenceMethod(1e-12 if system == "POLAR" else 1e-10,
+                                             max_sweeps=500_000)
+    rng = np.random.RandomState(11)
+    if system == "POLAR":
+        mesh = Mesh([(1.0, 2.9), (0.0, 0.95)], [0.1, 0.05], CoordinateSystem.POLAR)
+    elif system == "CYLINDRICAL":
+        mesh = Mesh([(1.0, 2.9), (0.0, 0.95), (-0.5, 0.5)], [0.1, 0.05, 0.1],
+                    CoordinateSystem.CYLINDRICAL)
+    else:
+        mesh = Mesh([(1.0, 2.9), (0.0, 0.95), (0.5, 1.45)], [0.1, 0.05, 0.05],
+                    CoordinateSystem.SPHERICAL)
+    shape = mesh.vertices_shape
+    y = rng.random_sample(tuple(shape) + (2,))
+    c = _edge_constraint(shape, 1.0, 2.0, 3.0, 42.0)
+    if mesh.dimensions == 3:
+        from pararealml_amd.constraint import Constraint
+
+        value = np.full(tuple(shape) + (1,), np.nan)
+        value[0], value[-1], value[:, 0], value[:, -1] = 1.0, 2.0, 3.0, 42.0
+        value[:, :, 0], value[:, :, -1] = -1.0, 5.0
+        mask = ~np.isnan(value)
+        c = Constraint(value[mask], mask)
+    c.apply(y[..., :1])
+    c.apply(y[..., 1:])
+    lap = diff.laplacian(y, mesh)
+    anti = diff.anti_laplacian(lap, mesh, [c, c])
+    assert np.allclose(diff.laplacian(anti, mesh), lap)
+    assert np.allclose(anti, y)
+
+
+def test_polar_anti_laplacian_with_derivative_constraints():
+    """The reference's test_3pcfdm_polar_anti_laplacian_with_derivative_constraints."""
+    from pararealml_amd.constraint import Constraint
+
+    print("case: polar anti-laplacian with derivative constraints", flush=True)
+    diff = ThreePointCentralDifferenceMethod(1e-12, max_sweeps=500_000)
+    rng = np.random.RandomState(12)
+    mesh = Mesh([(1.0, 2.9), (0.0, 0.95)], [0.1, 0.05], CoordinateSystem.POLAR)
+    y = rng.random_sample((20, 20, 2))
+    c0 = _edge_constraint((20, 20), 1.0, None, 3.0, None)
+    c1 = _edge_constraint((20, 20), -2.0, 2.0, 5.0, 4.0)
+    c0.apply(y[..., :1])
+    c1.apply(y[..., 1:])
+    dbc = np.empty((2, 2), dtype=object)
+    dbc[0, 0] = (None, Constraint(np.full(20, -3.0), np.ones((1, 20, 1), dtype=bool)))
+    dbc[1, 0] = (None, Constraint(np.full(20, 0.0), np.ones((20, 1, 1), dtype=bool)))
+    lap = diff.laplacian(y, mesh, dbc)
+    anti = diff.anti_laplacian(lap, mesh, [c0, c1], dbc)
+    assert np.allclose(diff.laplacian(anti, mesh, dbc), lap)
+    assert np.allclose(anti, y)

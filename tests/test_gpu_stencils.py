@@ -150,6 +150,3 @@ def test_device_differentiator_validation():
     mesh = pml.Mesh([(0.0, 1.0), (0.0, 2.0), (0.0, 1.0)], [1.0, 1.0, 1.0])
     with pytest.raises(ValueError):
         DIFF.gradient(np.arange(0.0, 12.0).reshape((2, 3, 2)), mesh, 0)
-    polar = pml.Mesh([(1.0, 5.0), (0.0, 2 * np.pi)], [2.0, np.pi], pml.CoordinateSystem.POLAR)
-    with pytest.raises(NotImplementedError):
-        DIFF.anti_laplacian(np.zeros((3, 3, 1)), polar, None)
