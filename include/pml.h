@@ -27,6 +27,9 @@
  *                             divergence, curl, laplacian, vector_laplacian
  *                             (numerical_differentiator.py:158-173, 229-308,
  *                             348-400, 482-590, 636-725, 771-870)
+ *   pml_set_values         <- apply_constraints_along_last_axis
+ *                             (constraint.py:104-131) on a full channel-last
+ *                             state (curvilinear stepping)
  *   pml_jacobi_sweep       <- ThreePointCentralDifferenceMethod.
  *                             _next_anti_laplacian_estimate (Cartesian) and the
  *                             constraint application of anti_laplacian
@@ -171,6 +174,14 @@ int pml_stencil_second(int precision, const void *y, void *out, int64_t outer,
  */
 int pml_weighted_add(int precision, const void *x, const void *coef, void *out,
                      int64_t count, int mode, void *stream);
+
+/*
+ * y[i] = values[i] wherever values[i] (FP64) is not NaN, over `count`
+ * elements: the static y constraints of a channel-last state in the
+ * curvilinear stepping path of FDMOperator.
+ */
+int pml_set_values(int precision, void *y, const double *values, int64_t count,
+                   void *stream);
 
 /*
  * One Jacobi sweep of the Cartesian anti-Laplacian on up to 3 spatial axes

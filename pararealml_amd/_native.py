@@ -82,6 +82,7 @@ SIGNATURES = {
     "pml_weighted_add": (
         c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p]
     ),
+    "pml_set_values": (c_int, [c_int, c_void_p, c_void_p, c_int64, c_void_p]),
     "pml_jacobi_sweep": (
         c_int,
         [c_int, c_void_p, c_void_p, c_void_p, POINTER(c_int64), c_int,

@@ -207,6 +207,17 @@ __global__ void k_weighted_add(const T *x, const T *coef, T *out, int64_t count,
     emit(out, t, coef[t] * x[t], mode);
 }
 
+// y[t] = v[t] where v[t] is not NaN: the static y constraints of a
+// channel-last state (constraint.py:104-131 over the full grid).
+template <typename T>
+__global__ void k_set_values(T *y, const double *v, int64_t count) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const double c = v[t];
+    if (c == c) y[t] = static_cast<T>(c);
+  }
+}
+
 }  // namespace pml
 
 using namespace pml;
@@ -244,6 +255,23 @@ extern "C" int pml_weighted_add(int precision, const void *x, const void *coef, 
                        (const float *)coef, (float *)out, count, mode);
   else
     return fail(PML_E_INVALID, "weighted add takes FP32 or FP64 arrays");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_set_values(int precision, void *y, const double *values, int64_t count,
+                              void *stream) {
+  if (count <= 0 || !y || !values) return fail(PML_E_INVALID, "bad set-values arguments");
+  hipStream_t st = as_stream(stream);
+  const int nb = grid_for(count);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_set_values<double>, dim3(nb), dim3(256), 0, st, (double *)y, values,
+                       count);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_set_values<float>, dim3(nb), dim3(256), 0, st, (float *)y, values,
+                       count);
+  else
+    return fail(PML_E_INVALID, "set values takes FP32 or FP64 arrays");
   PML_LAUNCH_CHECK();
   return PML_OK;
 }

@@ -13,8 +13,10 @@ Device coverage: Cartesian 1-D and 2-D problems with static boundary
 conditions, the equations with fused kernels (diffusion, convection-
 diffusion, wave, Cahn-Hilliard, Burgers, shallow water, Navier-Stokes with
 its stream function re-solved by device Jacobi sweeps), the explicit
-integrators and ``ThreePointCentralDifferenceMethod``.  Anything else raises
-``NotImplementedError``; there is no host fallback.
+integrators and ``ThreePointCentralDifferenceMethod``; on polar,
+cylindrical and spherical meshes, the diffusion and wave equations through
+the device operators of the differentiator (``_solve_curvilinear``).
+Anything else raises ``NotImplementedError``; there is no host fallback.
 
 Extensions over the reference (keyword-only, defaults reproduce it):
 ``precision`` ("float64", "float32" or "float32c" = compensated FP32),
@@ -27,7 +29,9 @@ from typing import Optional
 import numpy as np
 
 from pararealml_amd import _device as dev
+from pararealml_amd import _native as nat
 from pararealml_amd.initial_value_problem import InitialValueProblem
+from pararealml_amd.mesh import CoordinateSystem
 from pararealml_amd.operator import Operator, discretize_time_domain
 from pararealml_amd.operators.fdm.numerical_differentiator import (
     NumericalDifferentiator,
@@ -124,6 +128,8 @@ class FDMOperator(Operator):
 
     def solve(self, ivp: InitialValueProblem, parallel_enabled: bool = True) -> Solution:
         cp = ivp.constrained_problem
+        if cp.mesh is not None and cp.mesh.coordinate_system_type != CoordinateSystem.CARTESIAN:
+            return self._solve_curvilinear(ivp)
         problem = self.device_problem(cp)
         integ = dev.integrator_id(self._integrator)
         t = discretize_time_domain(ivp.t_interval, self._d_t)
@@ -169,6 +175,147 @@ class FDMOperator(Operator):
             vertex_oriented=True,
             d_t=self._d_t * every,
         )
+
+    def _solve_curvilinear(self, ivp: InitialValueProblem) -> Solution:
+        """Polar, cylindrical and spherical meshes (fdm_operator.py:48-165 with
+        the curvilinear branches of the differentiator).  The state stays on
+        the device in the reference's channel-last layout; each stage's
+        right-hand side is composed from the device operators of
+        ThreePointCentralDifferenceMethod (the Laplacian of component k with
+        the derivative constraints of component k, as
+        fdm_symbol_mapper.py:120-125), the stage sums run through
+        ``pml_weighted_add`` and the static y constraints through
+        ``pml_set_values``, with the stage arithmetic of
+        numerical_integrator.py:42-132.  Covers equations whose right-hand
+        side is a Laplacian form (diffusion, wave) in FP64 or FP32; other
+        equations raise ``NotImplementedError``.  Called directly, it also
+        runs Cartesian meshes (the tests compare it with the fused kernels)."""
+        import torch
+
+        cp = ivp.constrained_problem
+        eq = cp.differential_equation
+        if not isinstance(self._differentiator, ThreePointCentralDifferenceMethod):
+            raise NotImplementedError(
+                "the device path implements ThreePointCentralDifferenceMethod"
+            )
+        if not cp.are_all_boundary_conditions_static:
+            raise NotImplementedError("curvilinear meshes need static boundary conditions")
+        params = eq.device_parameters()
+        family = params[0] if params else None
+        if family not in ("diffusion", "wave"):
+            raise NotImplementedError(
+                "curvilinear meshes run the diffusion and wave equations; got "
+                f"{type(eq).__name__}"
+            )
+        if self._precision not in ("float64", "float32"):
+            raise NotImplementedError("curvilinear meshes run in float64 or float32")
+        integ = dev.integrator_id(self._integrator)
+        t = discretize_time_domain(ivp.t_interval, self._d_t)
+        n_steps = len(t) - 1
+        every = self._snapshot_every
+        if n_steps % every:
+            raise ValueError(
+                f"snapshot_every ({every}) must divide the number of steps ({n_steps})"
+            )
+        n_snap = n_steps // every
+        if n_snap == 0:
+            return Solution(
+                ivp, t[1:], np.empty((0,) + cp.y_vertices_shape), vertex_oriented=True,
+                d_t=self._d_t,
+            )
+        mesh = cp.mesh
+        diff = self._differentiator
+        dtype = torch.float64 if self._precision == "float64" else torch.float32
+        prec = nat.PML_F64 if dtype == torch.float64 else nat.PML_F32
+        shape = tuple(cp.y_vertices_shape)
+        count = int(np.prod(shape))
+        lib = nat.device_lib()
+
+        # full-grid static y constraints, NaN where free
+        full = np.full(shape, np.nan)
+        for f, c in enumerate(cp.static_y_vertex_constraints
+                              if cp.static_y_vertex_constraints is not None else []):
+            if c is not None:
+                full[..., f : f + 1][..., c.mask] = c.values
+        ycons = torch.from_numpy(full.ravel()).to("cuda")
+        dbc = cp.static_boundary_vertex_constraints[1]
+        if dbc is None:
+            dbc = np.empty((mesh.dimensions, shape[-1]), dtype=object)
+
+        def sptr():
+            return torch.cuda.current_stream().cuda_stream
+
+        def full_of(v):
+            return torch.full(shape, float(v), dtype=dtype, device="cuda")
+
+        def axpy(out, x, coef, mode):
+            """out (=, +=) coef * x on the device (coef: a full tensor)."""
+            nat.check(lib.pml_weighted_add(prec, x.data_ptr(), coef.data_ptr(), out.data_ptr(),
+                                           count, mode, sptr()))
+
+        def constrain(y):
+            nat.check(lib.pml_set_values(prec, y.data_ptr(), ycons.data_ptr(), count, sptr()))
+            return y
+
+        coef_p = full_of(params[1][0])
+        coef_f = torch.full(shape[:-1] + (1,), float(params[1][0]), dtype=dtype, device="cuda")
+
+        def rhs(y):
+            if family == "diffusion":
+                lap = diff.laplacian(y, mesh, dbc)
+                out = torch.empty_like(y)
+                axpy(out, lap, coef_p, 0)
+                return out
+            # wave: (y1, c^2 lap(y0))
+            lap = diff.laplacian(y[..., :1].contiguous(), mesh, dbc[:, :1])
+            acc = torch.empty_like(lap)
+            nat.check(lib.pml_weighted_add(prec, lap.data_ptr(), coef_f.data_ptr(),
+                                           acc.data_ptr(), lap.numel(), 0, sptr()))
+            return torch.cat([y[..., 1:2], acc], dim=-1).contiguous()
+
+        dt = self._d_t
+        c_dt, c_half, c_one, c_two, c_sixth = (full_of(v) for v in
+                                                (dt, 0.5, 1.0, 2.0, 1.0 / 6.0))
+        y = torch.from_numpy(np.ascontiguousarray(ivp.initial_condition.discrete_y_0(True))) \
+            .to(device="cuda", dtype=dtype)
+        snaps = []
+        for step in range(n_steps):
+            k1 = torch.empty_like(y)
+            axpy(k1, rhs(y), c_dt, 0)
+            if integ == nat.PML_INT_FE:
+                nxt = y.clone()
+                axpy(nxt, k1, c_one, 1)
+            elif integ == nat.PML_INT_MIDPOINT:
+                half = y.clone()
+                axpy(half, k1, c_half, 1)
+                k2 = torch.empty_like(y)
+                axpy(k2, rhs(constrain(half)), c_dt, 0)
+                nxt = y.clone()
+                axpy(nxt, k2, c_one, 1)
+            else:
+                st = y.clone()
+                axpy(st, k1, c_half, 1)
+                k2 = torch.empty_like(y)
+                axpy(k2, rhs(constrain(st)), c_dt, 0)
+                st = y.clone()
+                axpy(st, k2, c_half, 1)
+                k3 = torch.empty_like(y)
+                axpy(k3, rhs(constrain(st)), c_dt, 0)
+                st = y.clone()
+                axpy(st, k3, c_one, 1)
+                k4 = torch.empty_like(y)
+                axpy(k4, rhs(constrain(st)), c_dt, 0)
+                acc = k1.clone()
+                axpy(acc, k2, c_two, 1)
+                axpy(acc, k3, c_two, 1)
+                axpy(acc, k4, c_one, 1)
+                nxt = y.clone()
+                axpy(nxt, acc, c_sixth, 1)
+            y = constrain(nxt)
+            if (step + 1) % every == 0:
+                snaps.append(y.clone())
+        out = torch.stack(snaps).to(torch.float64).cpu().numpy()
+        return Solution(ivp, t[every::every], out, vertex_oriented=True, d_t=dt * every)
 
     def _solve_with_poisson(self, ivp, problem, integ, t, n_steps, every, y0) -> Solution:
         """The time loop for equations with LHS.Y_LAPLACIAN components
