@@ -14,8 +14,9 @@ conditions, the equations with fused kernels (diffusion, convection-
 diffusion, wave, Cahn-Hilliard, Burgers, shallow water, Navier-Stokes with
 its stream function re-solved by device Jacobi sweeps), the explicit
 integrators and ``ThreePointCentralDifferenceMethod``; on polar,
-cylindrical and spherical meshes, the diffusion and wave equations through
-the device operators of the differentiator (``_solve_curvilinear``).
+cylindrical and spherical meshes, the diffusion, wave and (2-D) shallow-water
+equations through the device operators of the differentiator
+(``_solve_curvilinear``).
 Anything else raises ``NotImplementedError``; there is no host fallback.
 
 Extensions over the reference (keyword-only, defaults reproduce it):
@@ -187,8 +188,8 @@ class FDMOperator(Operator):
         ``pml_weighted_add`` and the static y constraints through
         ``pml_set_values``, with the stage arithmetic of
         numerical_integrator.py:42-132.  Covers equations whose right-hand
-        side is a Laplacian form (diffusion, wave) in FP64 or FP32; other
-        equations raise ``NotImplementedError``.  Called directly, it also
+        side the loop composes (diffusion, wave, shallow water) in FP64 or
+        FP32; other equations raise ``NotImplementedError``.  Called directly, it also
         runs Cartesian meshes (the tests compare it with the fused kernels)."""
         import torch
 
@@ -202,10 +203,10 @@ class FDMOperator(Operator):
             raise NotImplementedError("curvilinear meshes need static boundary conditions")
         params = eq.device_parameters()
         family = params[0] if params else None
-        if family not in ("diffusion", "wave"):
+        if family not in ("diffusion", "wave", "shallow_water"):
             raise NotImplementedError(
-                "curvilinear meshes run the diffusion and wave equations; got "
-                f"{type(eq).__name__}"
+                "curvilinear meshes run the diffusion, wave and shallow-water "
+                f"equations; got {type(eq).__name__}"
             )
         if self._precision not in ("float64", "float32"):
             raise NotImplementedError("curvilinear meshes run in float64 or float32")
@@ -260,7 +261,45 @@ class FDMOperator(Operator):
         coef_p = full_of(params[1][0])
         coef_f = torch.full(shape[:-1] + (1,), float(params[1][0]), dtype=dtype, device="cuda")
 
+        def comp(y, k):
+            return y[..., k : k + 1].contiguous()
+
+        def field_full(v):
+            return torch.full(shape[:-1] + (1,), float(v), dtype=dtype, device="cuda")
+
+        def sub_sum(out, terms):
+            """out = sum of sign * coef * x over (coef, x, sign): coef is a
+            field-sized tensor (a constant or a field), the first term is
+            positive and writes, the rest add or subtract."""
+            for i, (c, x, sign) in enumerate(terms):
+                mode = 0 if i == 0 else sign
+                nat.check(lib.pml_weighted_add(prec, x.data_ptr(), c.data_ptr(), out.data_ptr(),
+                                               out.numel(), mode, sptr()))
+
+        def rhs_shallow_water(y):
+            """differential_equation.py:797-819 (h, b, v, f, g)."""
+            ys = [comp(y, k) for k in range(3)]
+            grad = [[diff.gradient(ys[k], mesh, a, dbc[:, k : k + 1]) for a in range(2)]
+                    for k in range(3)]
+            div = diff.divergence(y[..., 1:3].contiguous(), mesh, dbc[:, 1:3])
+            lap = [None] + [diff.laplacian(ys[k], mesh, dbc[:, k : k + 1]) for k in (1, 2)]
+            outs = [torch.empty_like(ys[0]) for _ in range(3)]
+            sub_sum(outs[0], [(c_mh, div, 1), (ys[0], grad[1][0], -1), (ys[1], grad[0][0], -1),
+                              (ys[0], grad[2][1], -1), (ys[2], grad[0][1], -1)])
+            sub_sum(outs[1], [(c_v, lap[1], 1), (ys[1], grad[1][0], -1), (ys[2], grad[1][1], -1),
+                              (c_mg, grad[0][0], 1), (c_mb, ys[1], 1), (c_f, ys[2], 1)])
+            sub_sum(outs[2], [(c_v, lap[2], 1), (ys[1], grad[2][0], -1), (ys[2], grad[2][1], -1),
+                              (c_mg, grad[0][1], 1), (c_mb, ys[2], 1), (c_mf, ys[1], 1)])
+            return torch.cat(outs, dim=-1).contiguous()
+
+        if family == "shallow_water":
+            h_, b_, v_, f_, g_ = params[1]
+            c_mh, c_v, c_mg, c_mb, c_f, c_mf = (field_full(x) for x in
+                                                (-h_, v_, -g_, -b_, f_, -f_))
+
         def rhs(y):
+            if family == "shallow_water":
+                return rhs_shallow_water(y)
             if family == "diffusion":
                 lap = diff.laplacian(y, mesh, dbc)
                 out = torch.empty_like(y)
