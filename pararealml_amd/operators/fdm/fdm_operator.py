@@ -14,9 +14,9 @@ conditions, the equations with fused kernels (diffusion, convection-
 diffusion, wave, Cahn-Hilliard, Burgers, shallow water, Navier-Stokes with
 its stream function re-solved by device Jacobi sweeps), the explicit
 integrators and ``ThreePointCentralDifferenceMethod``; on polar,
-cylindrical and spherical meshes, the diffusion, wave and (2-D) shallow-water
-equations through the device operators of the differentiator
-(``_solve_curvilinear``).
+cylindrical and spherical meshes and 3-D Cartesian ones, the diffusion,
+wave, shallow-water and Burgers equations through the device operators of
+the differentiator (``_solve_composed``).
 Anything else raises ``NotImplementedError``; there is no host fallback.
 
 Extensions over the reference (keyword-only, defaults reproduce it):
@@ -129,8 +129,11 @@ class FDMOperator(Operator):
 
     def solve(self, ivp: InitialValueProblem, parallel_enabled: bool = True) -> Solution:
         cp = ivp.constrained_problem
-        if cp.mesh is not None and cp.mesh.coordinate_system_type != CoordinateSystem.CARTESIAN:
-            return self._solve_curvilinear(ivp)
+        if cp.mesh is not None and (
+            cp.mesh.coordinate_system_type != CoordinateSystem.CARTESIAN
+            or cp.mesh.dimensions > 2
+        ):
+            return self._solve_composed(ivp)
         problem = self.device_problem(cp)
         integ = dev.integrator_id(self._integrator)
         t = discretize_time_domain(ivp.t_interval, self._d_t)
@@ -177,9 +180,10 @@ class FDMOperator(Operator):
             d_t=self._d_t * every,
         )
 
-    def _solve_curvilinear(self, ivp: InitialValueProblem) -> Solution:
-        """Polar, cylindrical and spherical meshes (fdm_operator.py:48-165 with
-        the curvilinear branches of the differentiator).  The state stays on
+    def _solve_composed(self, ivp: InitialValueProblem) -> Solution:
+        """Polar, cylindrical and spherical meshes and 3-D Cartesian ones
+        (fdm_operator.py:48-165 with the differentiator's curvilinear
+        branches), which the fused kernels do not cover.  The state stays on
         the device in the reference's channel-last layout; each stage's
         right-hand side is composed from the device operators of
         ThreePointCentralDifferenceMethod (the Laplacian of component k with
@@ -188,8 +192,8 @@ class FDMOperator(Operator):
         ``pml_weighted_add`` and the static y constraints through
         ``pml_set_values``, with the stage arithmetic of
         numerical_integrator.py:42-132.  Covers equations whose right-hand
-        side the loop composes (diffusion, wave, shallow water) in FP64 or
-        FP32; other equations raise ``NotImplementedError``.  Called directly, it also
+        side the loop composes (diffusion, wave, shallow water, Burgers) in
+        FP64 or FP32; other equations raise ``NotImplementedError``.  Called directly, it also
         runs Cartesian meshes (the tests compare it with the fused kernels)."""
         import torch
 
@@ -203,10 +207,10 @@ class FDMOperator(Operator):
             raise NotImplementedError("curvilinear meshes need static boundary conditions")
         params = eq.device_parameters()
         family = params[0] if params else None
-        if family not in ("diffusion", "wave", "shallow_water"):
+        if family not in ("diffusion", "wave", "shallow_water", "burgers"):
             raise NotImplementedError(
-                "curvilinear meshes run the diffusion, wave and shallow-water "
-                f"equations; got {type(eq).__name__}"
+                "curvilinear and 3-D meshes run the diffusion, wave, shallow-water "
+                f"and Burgers equations; got {type(eq).__name__}"
             )
         if self._precision not in ("float64", "float32"):
             raise NotImplementedError("curvilinear meshes run in float64 or float32")
@@ -297,9 +301,28 @@ class FDMOperator(Operator):
             c_mh, c_v, c_mg, c_mb, c_f, c_mf = (field_full(x) for x in
                                                 (-h_, v_, -g_, -b_, f_, -f_))
 
+        def rhs_burgers(y):
+            """differential_equation.py:757-765: (1/re) lap(y_i) - y . grad(y_i)."""
+            n = mesh.dimensions
+            ys = [comp(y, k) for k in range(n)]
+            outs = []
+            for i in range(n):
+                terms = [(c_re, diff.laplacian(ys[i], mesh, dbc[:, i : i + 1]), 1)]
+                for j in range(n):
+                    terms.append((ys[j], diff.gradient(ys[i], mesh, j, dbc[:, i : i + 1]), -1))
+                out = torch.empty_like(ys[0])
+                sub_sum(out, terms)
+                outs.append(out)
+            return torch.cat(outs, dim=-1).contiguous()
+
+        if family == "burgers":
+            c_re = field_full(params[1][0])
+
         def rhs(y):
             if family == "shallow_water":
                 return rhs_shallow_water(y)
+            if family == "burgers":
+                return rhs_burgers(y)
             if family == "diffusion":
                 lap = diff.laplacian(y, mesh, dbc)
                 out = torch.empty_like(y)

@@ -250,7 +250,7 @@ def test_curvilinear_stepping_path_matches_fused_kernels_on_cartesian(integ):
         ivp = pml.InitialValueProblem(cp, (0.0, 0.05), ic)
         op = FDMOperator(cls(), DIFF, 0.0025)
         fused = op.solve(ivp).discrete_y()
-        loop = op._solve_curvilinear(ivp).discrete_y()
+        loop = op._solve_composed(ivp).discrete_y()
         assert loop.shape == fused.shape
         _close(loop, fused, 1e-11 if name == "shallow_water" else 1e-12)
 
@@ -311,6 +311,39 @@ def test_polar_shallow_water_example():
     assert float(np.max(np.abs(rk4 - mid))) <= 1e-3 * float(np.max(np.abs(rk4)))
     f32 = FDMOperator(RK4(), DIFF, 0.0025, precision="float32").solve(ivp).discrete_y()
     _close(f32, rk4, 1e-5)
+
+
+def test_spherical_burgers_example_and_3d_cartesian():
+    """The reference's examples/burgers_3d_fdm.py setup (3-D spherical,
+    Neumann edges, u_r = 1/r^2) for 10 RK4 steps of 0.05: finite and FP32
+    follows FP64; on a 3-D Cartesian mesh the composed loop matches the
+    oracle's 3-D diffusion step to 1e-12 relative."""
+    import pararealml_amd as pml
+    from oracle import fdm_numpy as ora
+    from pararealml_amd.operators.fdm import RK4, FDMOperator, ForwardEulerMethod
+
+    print("case: spherical burgers / 3-d cartesian", flush=True)
+    mesh = Mesh([(1.0, 5.0), (0.0, 2.0 * np.pi), (0.25 * np.pi, 0.75 * np.pi)],
+                [0.5, np.pi / 10.0, np.pi / 10.0], CoordinateSystem.SPHERICAL)
+    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.BurgersEquation(3, 100), mesh, [(bc, bc)] * 3)
+    ic = pml.ContinuousInitialCondition(
+        cp, lambda x: np.stack([1.0 / x[:, 0] ** 2, np.zeros_like(x[:, 1]),
+                                np.zeros_like(x[:, 1])], axis=-1))
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.5), ic)
+    y64 = FDMOperator(RK4(), DIFF, 0.05).solve(ivp).discrete_y()
+    assert np.all(np.isfinite(y64))
+    y32 = FDMOperator(RK4(), DIFF, 0.05, precision="float32").solve(ivp).discrete_y()
+    _close(y32, y64, 1e-5)
+
+    mesh = Mesh([(0.0, 1.0)] * 3, [0.125] * 3)
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(3, 0.2), mesh, [(bc, bc)] * 3)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.4, 0.5, 0.6]), 0.02 * np.eye(3))])
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.01), ic)
+    got = FDMOperator(ForwardEulerMethod(), DIFF, 0.001).solve(ivp).discrete_y()
+    _, want = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, 0.01), 0.001, "fe")
+    _close(got, want, 1e-12)
 
 
 def test_curvilinear_stepping_unsupported_equation():
