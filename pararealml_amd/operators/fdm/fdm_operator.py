@@ -132,6 +132,7 @@ class FDMOperator(Operator):
         if cp.mesh is not None and (
             cp.mesh.coordinate_system_type != CoordinateSystem.CARTESIAN
             or cp.mesh.dimensions > 2
+            or not cp.are_all_boundary_conditions_static
         ):
             return self._solve_composed(ivp)
         problem = self.device_problem(cp)
@@ -181,9 +182,10 @@ class FDMOperator(Operator):
         )
 
     def _solve_composed(self, ivp: InitialValueProblem) -> Solution:
-        """Polar, cylindrical and spherical meshes and 3-D Cartesian ones
-        (fdm_operator.py:48-165 with the differentiator's curvilinear
-        branches), which the fused kernels do not cover.  The state stays on
+        """Polar, cylindrical and spherical meshes, 3-D Cartesian ones and
+        time-dependent boundary conditions (fdm_operator.py:48-165, 199-231
+        with the differentiator's curvilinear branches), which the fused
+        kernels do not cover.  The state stays on
         the device in the reference's channel-last layout; each stage's
         right-hand side is composed from the device operators of
         ThreePointCentralDifferenceMethod (the Laplacian of component k with
@@ -203,8 +205,6 @@ class FDMOperator(Operator):
             raise NotImplementedError(
                 "the device path implements ThreePointCentralDifferenceMethod"
             )
-        if not cp.are_all_boundary_conditions_static:
-            raise NotImplementedError("curvilinear meshes need static boundary conditions")
         params = eq.device_parameters()
         family = params[0] if params else None
         if family not in ("diffusion", "wave", "shallow_water", "burgers"):
@@ -236,16 +236,35 @@ class FDMOperator(Operator):
         count = int(np.prod(shape))
         lib = nat.device_lib()
 
-        # full-grid static y constraints, NaN where free
-        full = np.full(shape, np.nan)
-        for f, c in enumerate(cp.static_y_vertex_constraints
-                              if cp.static_y_vertex_constraints is not None else []):
-            if c is not None:
-                full[..., f : f + 1][..., c.mask] = c.values
-        ycons = torch.from_numpy(full.ravel()).to("cuda")
-        dbc = cp.static_boundary_vertex_constraints[1]
-        if dbc is None:
-            dbc = np.empty((mesh.dimensions, shape[-1]), dtype=object)
+        dynamic = not cp.are_all_boundary_conditions_static
+
+        def encode(y_constraints):
+            """Full-grid y constraints as a device FP64 array, NaN where free."""
+            full = np.full(shape, np.nan)
+            for f, c in enumerate(y_constraints if y_constraints is not None else []):
+                if c is not None:
+                    full[..., f : f + 1][..., c.mask] = c.values
+            return torch.from_numpy(full.ravel()).to("cuda")
+
+        def no_dbc(d):
+            return d if d is not None else np.empty((mesh.dimensions, shape[-1]), dtype=object)
+
+        static = None
+        if not dynamic:
+            static = (encode(cp.static_y_vertex_constraints),
+                      no_dbc(cp.static_boundary_vertex_constraints[1]))
+        cache = {}
+
+        def constraints_at(t_c):
+            """(device y constraints, derivative constraints) at time t_c; dynamic
+            conditions are evaluated once per time and step
+            (fdm_operator.py:199-231)."""
+            if not dynamic:
+                return static
+            if t_c not in cache:
+                y_bc, d_bc = cp.create_boundary_constraints(True, t_c)
+                cache[t_c] = (encode(cp.create_y_vertex_constraints(y_bc)), no_dbc(d_bc))
+            return cache[t_c]
 
         def sptr():
             return torch.cuda.current_stream().cuda_stream
@@ -258,7 +277,8 @@ class FDMOperator(Operator):
             nat.check(lib.pml_weighted_add(prec, x.data_ptr(), coef.data_ptr(), out.data_ptr(),
                                            count, mode, sptr()))
 
-        def constrain(y):
+        def constrain(y, t_c):
+            ycons = constraints_at(t_c)[0]
             nat.check(lib.pml_set_values(prec, y.data_ptr(), ycons.data_ptr(), count, sptr()))
             return y
 
@@ -280,7 +300,7 @@ class FDMOperator(Operator):
                 nat.check(lib.pml_weighted_add(prec, x.data_ptr(), c.data_ptr(), out.data_ptr(),
                                                out.numel(), mode, sptr()))
 
-        def rhs_shallow_water(y):
+        def rhs_shallow_water(y, dbc):
             """differential_equation.py:797-819 (h, b, v, f, g)."""
             ys = [comp(y, k) for k in range(3)]
             grad = [[diff.gradient(ys[k], mesh, a, dbc[:, k : k + 1]) for a in range(2)]
@@ -301,7 +321,7 @@ class FDMOperator(Operator):
             c_mh, c_v, c_mg, c_mb, c_f, c_mf = (field_full(x) for x in
                                                 (-h_, v_, -g_, -b_, f_, -f_))
 
-        def rhs_burgers(y):
+        def rhs_burgers(y, dbc):
             """differential_equation.py:757-765: (1/re) lap(y_i) - y . grad(y_i)."""
             n = mesh.dimensions
             ys = [comp(y, k) for k in range(n)]
@@ -318,11 +338,12 @@ class FDMOperator(Operator):
         if family == "burgers":
             c_re = field_full(params[1][0])
 
-        def rhs(y):
+        def rhs(y, t_c):
+            dbc = constraints_at(t_c)[1]
             if family == "shallow_water":
-                return rhs_shallow_water(y)
+                return rhs_shallow_water(y, dbc)
             if family == "burgers":
-                return rhs_burgers(y)
+                return rhs_burgers(y, dbc)
             if family == "diffusion":
                 lap = diff.laplacian(y, mesh, dbc)
                 out = torch.empty_like(y)
@@ -340,10 +361,15 @@ class FDMOperator(Operator):
                                                 (dt, 0.5, 1.0, 2.0, 1.0 / 6.0))
         y = torch.from_numpy(np.ascontiguousarray(ivp.initial_condition.discrete_y_0(True))) \
             .to(device="cuda", dtype=dtype)
+        if dynamic:
+            y = constrain(y, t[0])  # fdm_operator.py:55-63
+        half_dt = dt / 2.0
         snaps = []
         for step in range(n_steps):
+            t_i = t[step]
+            t_h, t_n = t_i + half_dt, t_i + dt
             k1 = torch.empty_like(y)
-            axpy(k1, rhs(y), c_dt, 0)
+            axpy(k1, rhs(y, t_i), c_dt, 0)
             if integ == nat.PML_INT_FE:
                 nxt = y.clone()
                 axpy(nxt, k1, c_one, 1)
@@ -351,29 +377,30 @@ class FDMOperator(Operator):
                 half = y.clone()
                 axpy(half, k1, c_half, 1)
                 k2 = torch.empty_like(y)
-                axpy(k2, rhs(constrain(half)), c_dt, 0)
+                axpy(k2, rhs(constrain(half, t_h), t_h), c_dt, 0)
                 nxt = y.clone()
                 axpy(nxt, k2, c_one, 1)
             else:
                 st = y.clone()
                 axpy(st, k1, c_half, 1)
                 k2 = torch.empty_like(y)
-                axpy(k2, rhs(constrain(st)), c_dt, 0)
+                axpy(k2, rhs(constrain(st, t_h), t_h), c_dt, 0)
                 st = y.clone()
                 axpy(st, k2, c_half, 1)
                 k3 = torch.empty_like(y)
-                axpy(k3, rhs(constrain(st)), c_dt, 0)
+                axpy(k3, rhs(constrain(st, t_h), t_h), c_dt, 0)
                 st = y.clone()
                 axpy(st, k3, c_one, 1)
                 k4 = torch.empty_like(y)
-                axpy(k4, rhs(constrain(st)), c_dt, 0)
+                axpy(k4, rhs(constrain(st, t_n), t_n), c_dt, 0)
                 acc = k1.clone()
                 axpy(acc, k2, c_two, 1)
                 axpy(acc, k3, c_two, 1)
                 axpy(acc, k4, c_one, 1)
                 nxt = y.clone()
                 axpy(nxt, acc, c_sixth, 1)
-            y = constrain(nxt)
+            y = constrain(nxt, t_n)
+            cache.clear()
             if (step + 1) % every == 0:
                 snaps.append(y.clone())
         out = torch.stack(snaps).to(torch.float64).cpu().numpy()
