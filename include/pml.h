@@ -30,6 +30,9 @@
  *   pml_set_values         <- apply_constraints_along_last_axis
  *                             (constraint.py:104-131) on a full channel-last
  *                             state (curvilinear stepping)
+ *   pml_secant_step        <- scipy.optimize.newton (array secant branch) as
+ *                             called by BackwardEulerMethod / CrankNicolsonMethod
+ *                             (operators/fdm/numerical_integrator.py:135-270)
  *   pml_jacobi_sweep       <- ThreePointCentralDifferenceMethod.
  *                             _next_anti_laplacian_estimate (Cartesian) and the
  *                             constraint application of anti_laplacian
@@ -182,6 +185,23 @@ int pml_weighted_add(int precision, const void *x, const void *coef, void *out,
  */
 int pml_set_values(int precision, void *y, const double *values, int64_t count,
                    void *stream);
+
+/*
+ * One iteration of the element-wise secant method of
+ * scipy.optimize.newton on arrays (no derivative), used by the implicit
+ * integrators.  mode 0: where q1 != q0, p_new = p1 - q1 (p1 - p) / (q1 - q0)
+ * and failing[i] = |dp| >= tol; elsewhere p_new = (p1 + p) / 2 for elements
+ * still active (active[i] != 0), then they become inactive, or p_new = p.
+ * counters (2 x uint64, zeroed by the caller) receive the number of
+ * elements with q1 != q0 and of those still failing.  mode 1: p_new =
+ * (p1 + p) / 2 everywhere (q0, q1, active, failing, counters unused).
+ * mode 2: the second start point p_new = p (1 + dx) + (p >= 0 ? dx : -dx)
+ * with dx passed in tol (only p and p_new used).
+ */
+int pml_secant_step(int precision, int mode, const void *p, const void *p1,
+                    const void *q0, const void *q1, void *p_new, void *active,
+                    void *failing, int64_t count, double tol, void *counters,
+                    void *stream);
 
 /*
  * One Jacobi sweep of the Cartesian anti-Laplacian on up to 3 spatial axes

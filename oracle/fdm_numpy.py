@@ -253,7 +253,40 @@ def integrate_rk4(y, t, d_t, f, cf):
     return apply_constraints_last_axis(c_full, y + (k1 + 2.0 * k2 + 2.0 * k3 + k4) / 6.0)
 
 
-INTEGRATORS = {"fe": integrate_fe, "midpoint": integrate_midpoint, "rk4": integrate_rk4}
+def _newton(residual, init, tol=1.48e-8, maxiter=50):
+    """ImplicitMethod._solve (numerical_integrator.py:154-176): SciPy's
+    ``optimize.newton`` without a derivative, i.e. its element-wise secant
+    method (pinned in the reference's requirements as scipy~=1.8.1)."""
+    from scipy.optimize import newton
+
+    return newton(residual, init, tol=tol, maxiter=maxiter)
+
+
+def integrate_backward_euler(y, t, d_t, f, cf):
+    """BackwardEulerMethod.integral (numerical_integrator.py:179-217)."""
+    t_next = t + d_t
+    c = cf(t_next)
+    init = apply_constraints_last_axis(c, y + d_t * f(t, y))
+    return _newton(lambda yn: yn - apply_constraints_last_axis(c, y + d_t * f(t_next, yn)),
+                   init)
+
+
+def integrate_crank_nicolson(y, t, d_t, f, cf, a=0.5):
+    """CrankNicolsonMethod.integral (numerical_integrator.py:220-270)."""
+    t_next = t + d_t
+    forward = d_t * f(t, y)
+    c = cf(t_next)
+    init = apply_constraints_last_axis(c, y + forward)
+    b = 1.0 - a
+    return _newton(
+        lambda yn: yn - apply_constraints_last_axis(c, y + a * d_t * f(t_next, yn) + b * forward),
+        init,
+    )
+
+
+INTEGRATORS = {"fe": integrate_fe, "midpoint": integrate_midpoint, "rk4": integrate_rk4,
+               "backward_euler": integrate_backward_euler,
+               "crank_nicolson": integrate_crank_nicolson}
 
 
 # ----------------------------------------------------------- symbolic RHS eval

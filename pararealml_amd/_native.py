@@ -83,6 +83,11 @@ SIGNATURES = {
         c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p]
     ),
     "pml_set_values": (c_int, [c_int, c_void_p, c_void_p, c_int64, c_void_p]),
+    "pml_secant_step": (
+        c_int,
+        [c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p,
+         c_int64, c_double, c_void_p, c_void_p],
+    ),
     "pml_jacobi_sweep": (
         c_int,
         [c_int, c_void_p, c_void_p, c_void_p, POINTER(c_int64), c_int,

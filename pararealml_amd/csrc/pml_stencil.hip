@@ -218,6 +218,48 @@ __global__ void k_set_values(T *y, const double *v, int64_t count) {
   }
 }
 
+// One iteration of the element-wise secant method that
+// scipy.optimize.newton runs on arrays without a derivative (the implicit
+// integrators, numerical_integrator.py:135-270).  mode 0: where q1 != q0,
+// p_new = p1 - q1 (p1 - p) / (q1 - q0) and fail = |dp| >= tol; where
+// q1 == q0 and the element is still active, p_new = (p1 + p) / 2; elements
+// with q1 == q0 become inactive.  counters[0] += elements with q1 != q0,
+// counters[1] += those still failing.  mode 1: p_new = (p1 + p) / 2
+// everywhere (all residual differences were zero).
+template <typename T>
+__global__ void k_secant(const T *p, const T *p1, const T *q0, const T *q1, T *p_new,
+                         unsigned char *active, unsigned char *fail, int64_t count, double tol,
+                         int mode, unsigned long long *counters) {
+  unsigned long long n_nz = 0, n_fail = 0;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    if (mode == 1) {
+      p_new[t] = (p1[t] + p[t]) / T(2);
+      continue;
+    }
+    if (mode == 2) {  // the second start point; tol carries dx
+      const T dx = static_cast<T>(tol);
+      p_new[t] = p[t] * (T(1) + dx) + (p[t] >= T(0) ? dx : -dx);
+      continue;
+    }
+    if (q1[t] != q0[t]) {
+      const T dp = (q1[t] * (p1[t] - p[t])) / (q1[t] - q0[t]);
+      p_new[t] = p1[t] - dp;
+      const bool f = fabs(static_cast<double>(dp)) >= tol;
+      fail[t] = f;
+      n_nz += 1;
+      n_fail += f;
+    } else {
+      p_new[t] = active[t] ? (p1[t] + p[t]) / T(2) : p[t];
+      active[t] = 0;
+    }
+  }
+  if (mode == 0) {
+    if (n_nz) atomicAdd(&counters[0], n_nz);
+    if (n_fail) atomicAdd(&counters[1], n_fail);
+  }
+}
+
 }  // namespace pml
 
 using namespace pml;
@@ -272,6 +314,32 @@ extern "C" int pml_set_values(int precision, void *y, const double *values, int6
                        count);
   else
     return fail(PML_E_INVALID, "set values takes FP32 or FP64 arrays");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_secant_step(int precision, int mode, const void *p, const void *p1,
+                               const void *q0, const void *q1, void *p_new, void *active,
+                               void *failing, int64_t count, double tol, void *counters,
+                               void *stream) {
+  if (count <= 0 || !p || !p_new || (mode != 2 && !p1) ||
+      (mode == 0 && (!q0 || !q1 || !active || !failing || !counters)) || mode < 0 || mode > 2)
+    return fail(PML_E_INVALID, "bad secant arguments");
+  hipStream_t st = as_stream(stream);
+  const int nb = grid_for(count);
+  auto *act = static_cast<unsigned char *>(active);
+  auto *fl = static_cast<unsigned char *>(failing);
+  auto *cnt = static_cast<unsigned long long *>(counters);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_secant<double>, dim3(nb), dim3(256), 0, st, (const double *)p,
+                       (const double *)p1, (const double *)q0, (const double *)q1,
+                       (double *)p_new, act, fl, count, tol, mode, cnt);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_secant<float>, dim3(nb), dim3(256), 0, st, (const float *)p,
+                       (const float *)p1, (const float *)q0, (const float *)q1,
+                       (float *)p_new, act, fl, count, tol, mode, cnt);
+  else
+    return fail(PML_E_INVALID, "secant step takes FP32 or FP64 arrays");
   PML_LAUNCH_CHECK();
   return PML_OK;
 }

@@ -385,6 +385,35 @@ def test_time_dependent_boundary_conditions_match_oracle(integ):
             assert np.all(np.isfinite(got))
 
 
+@pytest.mark.parametrize("method", ["backward_euler", "crank_nicolson"])
+def test_implicit_integrators_match_oracle(method):
+    """BackwardEulerMethod / CrankNicolsonMethod: the element-wise secant
+    iteration of scipy.optimize.newton runs on the device and matches the
+    oracle (SciPy itself on the oracle's right-hand side) to 1e-6 relative:
+    the iteration stops per element at |dp| < 1.48e-8, so rounding
+    differences between the two right-hand sides can end it one iteration
+    apart.  Diffusion and the wave equation on a Cartesian mesh with
+    Dirichlet and Neumann edges, 8 steps of 0.01 (4x the explicit limit for
+    diffusion); a polar diffusion run stays finite."""
+    import pararealml_amd as pml
+    from oracle import fdm_numpy as ora
+    from pararealml_amd.operators.fdm import BackwardEulerMethod, CrankNicolsonMethod, \
+        FDMOperator
+
+    print(f"case: implicit {method}", flush=True)
+    cls = {"backward_euler": BackwardEulerMethod, "crank_nicolson": CrankNicolsonMethod}[method]
+    probs = _diffusion_wave_problems("CARTESIAN")
+    for name in ("diffusion", "wave"):
+        cp, ic = probs[name]
+        ivp = pml.InitialValueProblem(cp, (0.0, 0.08), ic)
+        got = FDMOperator(cls(), DIFF, 0.01).solve(ivp).discrete_y()
+        _, want = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, 0.08), 0.01, method)
+        _close(got, want, 1e-6)
+    cp, ic = _diffusion_wave_problems("POLAR")["diffusion"]
+    got = FDMOperator(cls(), DIFF, 0.01).solve(pml.InitialValueProblem(cp, (0.0, 0.05), ic))
+    assert np.all(np.isfinite(got.discrete_y()))
+
+
 def test_curvilinear_stepping_unsupported_equation():
     import pararealml_amd as pml
     from pararealml_amd.operators.fdm import RK4, FDMOperator
