@@ -195,8 +195,9 @@ class FDMOperator(Operator):
         ``pml_weighted_add`` and the static y constraints through
         ``pml_set_values``, with the stage arithmetic of
         numerical_integrator.py:42-132.  Covers equations whose right-hand
-        side the loop composes (diffusion, wave, shallow water, Burgers) in
-        FP64 or FP32; other equations raise ``NotImplementedError``.  Called directly, it also
+        side the loop composes (diffusion, convection-diffusion, wave,
+        shallow water, Burgers) in FP64 or FP32; other equations raise
+        ``NotImplementedError``.  Called directly, it also
         runs Cartesian meshes (the tests compare it with the fused kernels)."""
         import torch
 
@@ -208,10 +209,11 @@ class FDMOperator(Operator):
             )
         params = eq.device_parameters()
         family = params[0] if params else None
-        if family not in ("diffusion", "wave", "shallow_water", "burgers"):
+        if family not in ("diffusion", "wave", "shallow_water", "burgers",
+                          "convection_diffusion"):
             raise NotImplementedError(
-                "curvilinear and 3-D meshes run the diffusion, wave, shallow-water "
-                f"and Burgers equations; got {type(eq).__name__}"
+                "curvilinear and 3-D meshes run the diffusion, convection-diffusion, "
+                f"wave, shallow-water and Burgers equations; got {type(eq).__name__}"
             )
         if self._precision not in ("float64", "float32"):
             raise NotImplementedError("curvilinear meshes run in float64 or float32")
@@ -340,12 +342,27 @@ class FDMOperator(Operator):
         if family == "burgers":
             c_re = field_full(params[1][0])
 
+        def rhs_convection_diffusion(y, dbc):
+            """differential_equation.py:633-670: d lap(y) - velocity . grad(y)."""
+            terms = [(c_d, diff.laplacian(y, mesh, dbc), 1)]
+            for j, c_vj in enumerate(c_vel):
+                terms.append((c_vj, diff.gradient(y, mesh, j, dbc), -1))
+            out = torch.empty_like(y)
+            sub_sum(out, terms)
+            return out
+
+        if family == "convection_diffusion":
+            c_d = field_full(params[1][0])
+            c_vel = [field_full(v) for v in eq._velocity]
+
         def rhs(y, t_c):
             dbc = constraints_at(t_c)[1]
             if family == "shallow_water":
                 return rhs_shallow_water(y, dbc)
             if family == "burgers":
                 return rhs_burgers(y, dbc)
+            if family == "convection_diffusion":
+                return rhs_convection_diffusion(y, dbc)
             if family == "diffusion":
                 lap = diff.laplacian(y, mesh, dbc)
                 out = torch.empty_like(y)

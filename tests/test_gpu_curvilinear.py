@@ -213,7 +213,9 @@ def _diffusion_wave_problems(system):
     out = {}
     for name, eq, n_f in (("diffusion", pml.DiffusionEquation(2, 0.3), 1),
                           ("wave", pml.WaveEquation(2, 1.2), 2),
-                          ("shallow_water", pml.ShallowWaterEquation(0.5, v=0.05, f=0.3), 3)):
+                          ("shallow_water", pml.ShallowWaterEquation(0.5, v=0.05, f=0.3), 3),
+                          ("convection_diffusion",
+                           pml.ConvectionDiffusionEquation(2, [0.3, -0.2], 0.05), 1)):
         bcs = [
             (pml.DirichletBoundaryCondition(lambda x, t, n=n_f: np.full((len(x), n), 0.5),
                                             is_static=True),
@@ -237,9 +239,9 @@ def _diffusion_wave_problems(system):
 def test_curvilinear_stepping_path_matches_fused_kernels_on_cartesian(integ):
     """The curvilinear time loop (device operators + weighted sums) run on a
     Cartesian mesh reproduces the fused stepping kernels (oracle-pinned) in
-    FP64 for diffusion, the wave equation (1e-12 relative) and shallow water
-    (1e-11: the fused kernel orders the product terms differently), with
-    Dirichlet and Neumann edges."""
+    FP64 for diffusion, convection-diffusion, the wave equation (1e-12
+    relative) and shallow water (1e-11: the fused kernel orders the product
+    terms differently), with Dirichlet and Neumann edges."""
     import pararealml_amd as pml
     from pararealml_amd.operators.fdm import (
         RK4, ExplicitMidpointMethod, FDMOperator, ForwardEulerMethod)
@@ -409,9 +411,10 @@ def test_implicit_integrators_match_oracle(method):
         got = FDMOperator(cls(), DIFF, 0.01).solve(ivp).discrete_y()
         _, want = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, 0.08), 0.01, method)
         _close(got, want, 1e-6)
-    cp, ic = _diffusion_wave_problems("POLAR")["diffusion"]
-    got = FDMOperator(cls(), DIFF, 0.01).solve(pml.InitialValueProblem(cp, (0.0, 0.05), ic))
-    assert np.all(np.isfinite(got.discrete_y()))
+    for name in ("diffusion", "convection_diffusion"):
+        cp, ic = _diffusion_wave_problems("POLAR")[name]
+        got = FDMOperator(cls(), DIFF, 0.01).solve(pml.InitialValueProblem(cp, (0.0, 0.05), ic))
+        assert np.all(np.isfinite(got.discrete_y()))
 
 
 def test_curvilinear_stepping_unsupported_equation():
