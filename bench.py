@@ -16,7 +16,8 @@ the same batch, and the CPU baseline (the NumPy oracle, FP64, one core, on a
 bounded sample of the same workload).
 
 Other workloads: ``parareal`` (2-D diffusion, learned vs FDM coarse),
-``swe-parareal`` (shallow water on 1025^2, 64 slices), ``surrogate`` (conv
+``swe-parareal`` (shallow water on 1025^2, 64 slices), ``swe-step`` (its
+batched fine stepping throughput), ``surrogate`` (conv
 surrogate training throughput) and ``copy`` (the HBM counter calibration).
 """
 
@@ -47,7 +48,8 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="ch256",
-                    choices=["ch256", "parareal", "swe-parareal", "copy", "surrogate"])
+                    choices=["ch256", "parareal", "swe-parareal", "swe-step", "copy",
+                             "surrogate"])
     ap.add_argument("--sur-batch", type=int, default=32)
     ap.add_argument("--sur-size", type=int, default=129, help="grid vertices per axis")
     # Parareal workload (2-D diffusion, learned vs FDM coarse)
@@ -668,6 +670,64 @@ def surrogate_workload(args):
     }), flush=True)
 
 
+def swe_step_workload(args):
+    """Stepping throughput of the shallow-water fine propagator of the
+    1025^2 Parareal workload: the 64 slices' states (64 grids of 1025^2
+    vertices, three fields, zero-Neumann edges, seeded Gaussian height bumps)
+    advanced together by RK4 in FP32 at d_t = 1e-3, ``--steps`` timed steps
+    after ``--warmup``, graph replays, kernel variant tuned first.  The
+    roofline counts 24 B per cell-step (three FP32 fields read and
+    written)."""
+    import torch
+
+    import pararealml_amd as pml
+    from pararealml_amd import _device as dev
+    from pararealml_amd import _native as nat
+    from pararealml_amd.autotune import tune
+
+    torch.cuda.set_device(0)
+    n, grids = args.swe_n, args.swe_slices
+    eq = pml.ShallowWaterEquation(h=1.0, v=0.01, g=9.81)
+    mesh = pml.Mesh([(-5.0, 5.0), (-5.0, 5.0)], [10.0 / n, 10.0 / n])
+    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    y0s = list(_swe_bumps(cp, grids, np.random.RandomState(SEEDS[2])))
+    problem = dev.device_problem(cp)
+    cells = grids * (n + 1) ** 2
+    stream = dev.solver_stream()
+    stream.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(stream):
+        batch = dev.FDMBatch([problem], [0] * grids, "float32", nat.PML_INT_RK4, stream)
+        batch.load_host(y0s)
+        cfg, timings = tune(batch, args.swe_dt, n_steps=8, repeats=2, store=False)
+        print(f"autotune: {timings} -> config {cfg}", file=sys.stderr)
+        name = batch.lib.pml_fdm_config_name(batch.eq_id, batch.integrator, batch.prec,
+                                             cfg).decode()
+        batch.load_host(y0s)
+        batch.advance(args.swe_dt, args.warmup, tile_config=cfg, use_graph=True)
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        ev0.record(stream)
+        batch.advance(args.swe_dt, args.steps, tile_config=cfg, use_graph=True)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        step_s = ev0.elapsed_time(ev1) / 1e3 / args.steps
+    achieved = cells * 24 / step_s / 1e9
+    print(json.dumps({
+        "workload": "swe-step",
+        "config": {"grids": grids, "vertices": f"{n + 1}^2", "fields": 3, "integrator": "RK4",
+                   "dtype": "f32", "d_t": args.swe_dt, "kernel_variant": name},
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": step_s * 1e3, "wall_s": wall,
+        "Gcell_steps_per_s": cells / step_s / 1e9,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS},
+    }), flush=True)
+
+
 def copy_workload(args):
     """A plain device-to-device copy of 1 GiB, repeated: the calibration
     reference for the FETCH_SIZE / WRITE_SIZE counters (each dispatch moves
@@ -709,6 +769,9 @@ def main():
         return
     if args.workload == "swe-parareal":
         swe_parareal_workload(args)
+        return
+    if args.workload == "swe-step":
+        swe_step_workload(args)
         return
     if args.workload == "copy":
         copy_workload(args)
