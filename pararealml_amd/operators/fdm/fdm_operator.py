@@ -196,8 +196,8 @@ class FDMOperator(Operator):
         ``pml_set_values``, with the stage arithmetic of
         numerical_integrator.py:42-132.  Covers equations whose right-hand
         side the loop composes (diffusion, convection-diffusion, wave,
-        shallow water, Burgers) in FP64 or FP32; other equations raise
-        ``NotImplementedError``.  Called directly, it also
+        shallow water, Burgers, Cahn-Hilliard with its LHS.Y potential) in
+        FP64 or FP32; other equations raise ``NotImplementedError``.  Called directly, it also
         runs Cartesian meshes (the tests compare it with the fused kernels)."""
         import torch
 
@@ -210,10 +210,11 @@ class FDMOperator(Operator):
         params = eq.device_parameters()
         family = params[0] if params else None
         if family not in ("diffusion", "wave", "shallow_water", "burgers",
-                          "convection_diffusion"):
+                          "convection_diffusion", "cahn_hilliard"):
             raise NotImplementedError(
                 "curvilinear and 3-D meshes run the diffusion, convection-diffusion, "
-                f"wave, shallow-water and Burgers equations; got {type(eq).__name__}"
+                "wave, shallow-water, Burgers and Cahn-Hilliard equations; got "
+                f"{type(eq).__name__}"
             )
         if self._precision not in ("float64", "float32"):
             raise NotImplementedError("curvilinear meshes run in float64 or float32")
@@ -355,6 +356,35 @@ class FDMOperator(Operator):
             c_d = field_full(params[1][0])
             c_vel = [field_full(v) for v in eq._velocity]
 
+        def rhs_cahn_hilliard(y, dbc):
+            """differential_equation.py:723-732: c_t = d lap(mu); mu (LHS.Y)
+            has no time derivative inside the step."""
+            lap_mu = diff.laplacian(comp(y, 1), mesh, dbc[:, 1:2])
+            dc = torch.empty_like(lap_mu)
+            sub_sum(dc, [(c_chd, lap_mu, 1)])
+            return torch.cat([dc, torch.zeros_like(dc)], dim=-1).contiguous()
+
+        def with_mu(nxt, y_pre, t_pre, t_next):
+            """fdm_operator.py:127-139: mu_next = C_{t_next}(c^3 - c - gamma
+            lap(c)) from the pre-step c, derivative constraints at t_pre; the
+            other components are left as the integrator produced them."""
+            c = comp(y_pre, 0)
+            lap_c = diff.laplacian(c, mesh, constraints_at(t_pre)[1][:, :1])
+            c2 = torch.empty_like(c)
+            sub_sum(c2, [(c, c, 1)])
+            mu = torch.empty_like(c)
+            sub_sum(mu, [(c2, c, 1), (c_ones_f, c, -1), (c_gamma, lap_c, -1)])
+            tmp = nxt.clone()
+            tmp[..., 1:2].copy_(mu)
+            constrain(tmp, t_next)
+            nxt[..., 1:2].copy_(tmp[..., 1:2])
+            return nxt
+
+        if family == "cahn_hilliard":
+            c_chd = field_full(params[1][0])
+            c_gamma = field_full(params[1][1])
+            c_ones_f = field_full(1.0)
+
         def rhs(y, t_c):
             dbc = constraints_at(t_c)[1]
             if family == "shallow_water":
@@ -363,6 +393,8 @@ class FDMOperator(Operator):
                 return rhs_burgers(y, dbc)
             if family == "convection_diffusion":
                 return rhs_convection_diffusion(y, dbc)
+            if family == "cahn_hilliard":
+                return rhs_cahn_hilliard(y, dbc)
             if family == "diffusion":
                 lap = diff.laplacian(y, mesh, dbc)
                 out = torch.empty_like(y)
@@ -452,6 +484,8 @@ class FDMOperator(Operator):
                     return r
 
                 nxt = secant(init, residual)
+                if family == "cahn_hilliard":
+                    nxt = with_mu(nxt, y, t_i, t_n)
                 y = nxt
                 cache.clear()
                 if (step + 1) % every == 0:
@@ -486,6 +520,8 @@ class FDMOperator(Operator):
                 axpy(acc, k4, c_one, 1)
                 nxt = y.clone()
                 axpy(nxt, acc, c_sixth, 1)
+            if family == "cahn_hilliard":
+                nxt = with_mu(nxt, y, t_i, t_n)
             y = constrain(nxt, t_n)
             cache.clear()
             if (step + 1) % every == 0:

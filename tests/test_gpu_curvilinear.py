@@ -215,7 +215,8 @@ def _diffusion_wave_problems(system):
                           ("wave", pml.WaveEquation(2, 1.2), 2),
                           ("shallow_water", pml.ShallowWaterEquation(0.5, v=0.05, f=0.3), 3),
                           ("convection_diffusion",
-                           pml.ConvectionDiffusionEquation(2, [0.3, -0.2], 0.05), 1)):
+                           pml.ConvectionDiffusionEquation(2, [0.3, -0.2], 0.05), 1),
+                          ("cahn_hilliard", pml.CahnHilliardEquation(2, 0.1, 0.01), 2)):
         bcs = [
             (pml.DirichletBoundaryCondition(lambda x, t, n=n_f: np.full((len(x), n), 0.5),
                                             is_static=True),
@@ -239,9 +240,10 @@ def _diffusion_wave_problems(system):
 def test_curvilinear_stepping_path_matches_fused_kernels_on_cartesian(integ):
     """The curvilinear time loop (device operators + weighted sums) run on a
     Cartesian mesh reproduces the fused stepping kernels (oracle-pinned) in
-    FP64 for diffusion, convection-diffusion, the wave equation (1e-12
-    relative) and shallow water (1e-11: the fused kernel orders the product
-    terms differently), with Dirichlet and Neumann edges."""
+    FP64 for diffusion, convection-diffusion, the wave equation,
+    Cahn-Hilliard with its LHS.Y potential (1e-12 relative) and shallow water
+    (1e-11: the fused kernel orders the product terms differently), with
+    Dirichlet and Neumann edges."""
     import pararealml_amd as pml
     from pararealml_amd.operators.fdm import (
         RK4, ExplicitMidpointMethod, FDMOperator, ForwardEulerMethod)
@@ -422,8 +424,8 @@ def test_curvilinear_stepping_unsupported_equation():
     from pararealml_amd.operators.fdm import RK4, FDMOperator
 
     mesh = Mesh([(1.0, 3.0), (0.0, 1.5)], [0.1, 0.1], CoordinateSystem.POLAR)
-    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 2)), is_static=True)
-    cp = pml.ConstrainedProblem(pml.CahnHilliardEquation(2), mesh, [(bc, bc)] * 2)
+    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 4)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.NavierStokesEquation(100.0), mesh, [(bc, bc)] * 2)
     ic = pml.DiscreteInitialCondition(cp, np.zeros(cp.y_vertices_shape), True)
     with pytest.raises(NotImplementedError):
         FDMOperator(RK4(), DIFF, 0.001).solve(pml.InitialValueProblem(cp, (0.0, 0.01), ic))
