@@ -196,8 +196,9 @@ class FDMOperator(Operator):
         ``pml_set_values``, with the stage arithmetic of
         numerical_integrator.py:42-132.  Covers equations whose right-hand
         side the loop composes (diffusion, convection-diffusion, wave,
-        shallow water, Burgers, Cahn-Hilliard with its LHS.Y potential) in
-        FP64 or FP32; other equations raise ``NotImplementedError``.  Called directly, it also
+        shallow water, Burgers, Cahn-Hilliard with its LHS.Y potential,
+        Navier-Stokes with its Poisson phase) in FP64 or FP32; other
+        equations raise ``NotImplementedError``.  Called directly, it also
         runs Cartesian meshes (the tests compare it with the fused kernels)."""
         import torch
 
@@ -210,11 +211,11 @@ class FDMOperator(Operator):
         params = eq.device_parameters()
         family = params[0] if params else None
         if family not in ("diffusion", "wave", "shallow_water", "burgers",
-                          "convection_diffusion", "cahn_hilliard"):
+                          "convection_diffusion", "cahn_hilliard", "navier_stokes"):
             raise NotImplementedError(
                 "curvilinear and 3-D meshes run the diffusion, convection-diffusion, "
-                "wave, shallow-water, Burgers and Cahn-Hilliard equations; got "
-                f"{type(eq).__name__}"
+                "wave, shallow-water, Burgers, Cahn-Hilliard and Navier-Stokes equations; "
+                f"got {type(eq).__name__}"
             )
         if self._precision not in ("float64", "float32"):
             raise NotImplementedError("curvilinear meshes run in float64 or float32")
@@ -257,18 +258,20 @@ class FDMOperator(Operator):
         static = None
         if not dynamic:
             static = (encode(cp.static_y_vertex_constraints),
-                      no_dbc(cp.static_boundary_vertex_constraints[1]))
+                      no_dbc(cp.static_boundary_vertex_constraints[1]),
+                      cp.static_y_vertex_constraints)
         cache = {}
 
         def constraints_at(t_c):
-            """(device y constraints, derivative constraints) at time t_c; dynamic
-            conditions are evaluated once per time and step
-            (fdm_operator.py:199-231)."""
+            """(device y constraints, derivative constraints, host y
+            constraints) at time t_c; dynamic conditions are evaluated once per
+            time and step (fdm_operator.py:199-231)."""
             if not dynamic:
                 return static
             if t_c not in cache:
                 y_bc, d_bc = cp.create_boundary_constraints(True, t_c)
-                cache[t_c] = (encode(cp.create_y_vertex_constraints(y_bc)), no_dbc(d_bc))
+                host = cp.create_y_vertex_constraints(y_bc)
+                cache[t_c] = (encode(host), no_dbc(d_bc), host)
             return cache[t_c]
 
         def sptr():
@@ -380,6 +383,48 @@ class FDMOperator(Operator):
             nxt[..., 1:2].copy_(tmp[..., 1:2])
             return nxt
 
+        def rhs_navier_stokes(y, dbc):
+            """differential_equation.py:835-850, vorticity row: (1/re) lap(w)
+            - (u, v) . grad(w); psi (LHS.Y_LAPLACIAN), u and v (LHS.Y) have
+            no time derivative inside the step."""
+            w = comp(y, 0)
+            terms = [(c_re_ns, diff.laplacian(w, mesh, dbc[:, 0:1]), 1)]
+            for j in range(2):
+                terms.append((comp(y, 2 + j), diff.gradient(w, mesh, j, dbc[:, 0:1]), -1))
+            dw = torch.empty_like(w)
+            sub_sum(dw, terms)
+            return torch.cat([dw] + [torch.zeros_like(dw)] * 3, dim=-1).contiguous()
+
+        def with_psi_uv(nxt, y_pre, t_pre, t_next):
+            """fdm_operator.py:127-161 for Navier-Stokes: psi solves
+            lap(psi) = -w_pre (Jacobi to the differentiator's tol, constraints
+            at t_next, started from the pre-step psi), u = d psi / d x1 and
+            v = -d psi / d x0 from the pre-step psi (derivative constraints at
+            t_pre), all constrained at t_next."""
+            d_pre = constraints_at(t_pre)[1]
+            d_next = constraints_at(t_next)[1]
+            host_next = constraints_at(t_next)[2]
+            psi = comp(y_pre, 1)
+            u = diff.gradient(psi, mesh, 1, d_pre[:, 1:2])
+            v = torch.empty_like(psi)
+            sub_sum(v, [(c_mone, diff.gradient(psi, mesh, 0, d_pre[:, 1:2]), 1)])
+            src = torch.empty_like(psi)
+            sub_sum(src, [(c_mone, comp(y_pre, 0), 1)])
+            psi_next = diff.anti_laplacian(
+                src, mesh, None if host_next is None else host_next[1:2], d_next[:, 1:2],
+                y_init=psi)
+            tmp = nxt.clone()
+            tmp[..., 1:2].copy_(psi_next)
+            tmp[..., 2:3].copy_(u)
+            tmp[..., 3:4].copy_(v)
+            constrain(tmp, t_next)
+            nxt[..., 1:4].copy_(tmp[..., 1:4])
+            return nxt
+
+        if family == "navier_stokes":
+            c_re_ns = field_full(params[1][0])
+            c_mone = field_full(-1.0)
+
         if family == "cahn_hilliard":
             c_chd = field_full(params[1][0])
             c_gamma = field_full(params[1][1])
@@ -395,6 +440,8 @@ class FDMOperator(Operator):
                 return rhs_convection_diffusion(y, dbc)
             if family == "cahn_hilliard":
                 return rhs_cahn_hilliard(y, dbc)
+            if family == "navier_stokes":
+                return rhs_navier_stokes(y, dbc)
             if family == "diffusion":
                 lap = diff.laplacian(y, mesh, dbc)
                 out = torch.empty_like(y)
@@ -486,6 +533,8 @@ class FDMOperator(Operator):
                 nxt = secant(init, residual)
                 if family == "cahn_hilliard":
                     nxt = with_mu(nxt, y, t_i, t_n)
+                if family == "navier_stokes":
+                    nxt = with_psi_uv(nxt, y, t_i, t_n)
                 y = nxt
                 cache.clear()
                 if (step + 1) % every == 0:
@@ -522,6 +571,8 @@ class FDMOperator(Operator):
                 axpy(nxt, acc, c_sixth, 1)
             if family == "cahn_hilliard":
                 nxt = with_mu(nxt, y, t_i, t_n)
+            if family == "navier_stokes":
+                nxt = with_psi_uv(nxt, y, t_i, t_n)
             y = constrain(nxt, t_n)
             cache.clear()
             if (step + 1) % every == 0:
