@@ -13,10 +13,12 @@ Device coverage: Cartesian 1-D and 2-D problems with static boundary
 conditions, the equations with fused kernels (diffusion, convection-
 diffusion, wave, Cahn-Hilliard, Burgers, shallow water, Navier-Stokes with
 its stream function re-solved by device Jacobi sweeps), the explicit
-integrators and ``ThreePointCentralDifferenceMethod``; on polar,
-cylindrical and spherical meshes and 3-D Cartesian ones, the diffusion,
-wave, shallow-water and Burgers equations through the device operators of
-the differentiator (``_solve_composed``).
+integrators and ``ThreePointCentralDifferenceMethod``.  Polar,
+cylindrical and spherical meshes, 3-D Cartesian ones, time-dependent
+boundary conditions and the implicit integrators run through a composed
+device loop (``_solve_composed``: the differentiator's device operators per
+stage) for the diffusion, convection-diffusion, wave, shallow-water,
+Burgers, Cahn-Hilliard and Navier-Stokes equations.
 Anything else raises ``NotImplementedError``; there is no host fallback.
 
 Extensions over the reference (keyword-only, defaults reproduce it):
