@@ -173,7 +173,7 @@ def cpu_baseline(cps, sizes, y0, budget_s):
 
 def parareal_workload(args):
     """Time-to-target of learned-coarse Parareal, FDM-coarse Parareal and
-    serial fine FDM on one GPU, all judged against the FP64 serial fine
+    serial fine FDM on one GPU, all compared with the FP64 serial fine
     solution at the slice ends.
 
     2-D diffusion (d = 0.5) on [0, 1]^2 with (n + 1)^2 vertices, zero
@@ -376,7 +376,7 @@ def _swe_bumps(cp, n, rng):
 
 
 def swe_parareal_workload(args):
-    """Parareal for the shallow-water equations on one GPU, judged on all
+    """Parareal for the shallow-water equations on one GPU, measured on all
     three fields against the FP64 serial fine solution at the slice ends.
 
     ShallowWaterEquation(h=1, v=0.01, g=9.81) on [-5, 5]^2 with (n + 1)^2

@@ -116,7 +116,7 @@ def test_parareal_single_slice_and_serial_mode():
 
 
 def test_parareal_coarse_grid_fp64_and_compensated_fine():
-    """2x-coarsened coarse operator; compensated-FP32 fine solves judged
+    """2x-coarsened coarse operator; compensated-FP32 fine solves compared
     against the FP64 oracle emulation at a stated 1e-6 relative tolerance."""
     print("case: parareal coarse grid", flush=True)
     cp, ic = diffusion_problem(33)
