@@ -25,16 +25,23 @@ Pinning of each function (independent evidence, beyond this restatement):
   forms of tests/operators/fdm/test_numerical_integrator.py:13-138 and the
   amplification factors ``(1+z)`` / ``1+z+z^2/2`` / ``1+z+...+z^4/24`` on a
   linear ODE.
-* ``fdm_solve``: exponential closed form (test_fdm_operator.py:38-61),
-  discrete sine-mode decay of 2-D diffusion under Dirichlet zero, exact
-  trapezoid-free mass conservation under zero-Neumann reflection, and the
-  Cahn-Hilliard one-step lag (mu from the pre-step c).
-* ``parareal_solve``: serial-equals-fine (test_parareal_operator.py:68-84),
-  one slice equals fine, k = N iterations reproduce the serial fine solution,
-  and G = F converges after one iteration.
+* ``fdm_solve``: the reference's exponential closed form
+  (test_fdm_operator.py:38-61) and RK4's amplification polynomial to the
+  step count on that ODE, the discrete sine-mode amplification of 2-D
+  diffusion under Dirichlet zero (FE, midpoint, RK4), exact
+  trapezoid-weighted mass conservation under zero-Neumann reflection
+  (diffusion and Cahn-Hilliard's c), and the Cahn-Hilliard one-step lag
+  (c' = c + dt d lap(mu), mu' = c^3 - c - gamma lap(c), from the pre-step
+  state) -- tests/test_oracle_pins.py; the Navier-Stokes step relations --
+  tests/test_oracle_navier_stokes.py; the dynamic-boundary path against the
+  static one -- tests/test_oracle_dynamic_bc.py.
+* ``parareal_solve``: one slice equals fine, N iterations reproduce the
+  serial fine solution, G = F stops after one iteration with the serial
+  solution, and the iteration count is the first whose update is below the
+  tolerance -- tests/test_oracle_pins.py.
 * ``restrict_injection`` / ``prolong_bilinear``: these grid transfers have no
   counterpart in the reference; they are pinned by exactness on bilinear
-  fields and restriction(prolongation(u)) == u.
+  fields and restriction(prolongation(u)) == u -- tests/test_oracle_pins.py.
 """
 
 from typing import Callable, List, Optional, Sequence, Tuple

@@ -25,7 +25,10 @@ only by FP32-vs-FP64 accumulation.
 
 Pinning: ``conv3x3`` is checked against a direct sum on tiny inputs and
 against linearity; the backward passes against finite differences of the
-FP64 (unrounded) forward; ``gelu`` against its closed form at known points.
+FP64 (unrounded) forward; ``gelu`` against its closed form at known points
+(tests/test_oracle_surrogate.py); ``adam_step`` against the closed form of
+the first bias-corrected step, lr g / (|g| + eps), and of every step under a
+constant gradient (tests/test_oracle_pins.py).
 """
 
 import numpy as np
