@@ -48,6 +48,9 @@
  *                             164-188)
  *   pml_parareal_correction<- correction = F_end - G_end
  *                             (parareal_operator.py:164)
+ *   pml_graph_cache_clear  <- no counterpart (the reference has no device
+ *                             caches): releases the step-loop graphs and
+ *                             work-unit tables pml_fdm_solve keeps
  */
 #ifndef PML_H
 #define PML_H
@@ -63,6 +66,12 @@ extern "C" {
 #define PML_E_INVALID 1     /* bad argument   -> Python ValueError        */
 #define PML_E_UNSUPPORTED 2 /* not on device  -> Python NotImplementedError */
 #define PML_E_HIP 3         /* runtime error  -> Python RuntimeError       */
+
+/* Blocks of the fixed-order reductions (update and Jacobi norms): their
+ * workspaces hold this many partial sums per reduced quantity. */
+#define PML_REDUCE_SLOTS 2048
+/* Bytes of the state buffer of pml_jacobi_solve / pml_jacobi_solve_weighted. */
+#define PML_JACOBI_STATE_BYTES (32 + 8 * PML_REDUCE_SLOTS)
 
 /* ---- state precision ---------------------------------------------------- */
 #define PML_F32 0  /* FP32 state, FP32 arithmetic                           */
@@ -130,11 +139,22 @@ int pml_device_sync(void *stream);
  *                       captured hipGraph (needs a non-null stream);
  *                       PML_FLAG_CONSTRAIN_INPUT -- impose the Dirichlet
  *                       values on the initial state before the first step
- *                       (what the reference's initial conditions do)
+ *                       (what the reference's initial conditions do);
+ *                       PML_FLAG_PREPARE -- build (and cache) the unit
+ *                       tables and the step-loop graph this call would use,
+ *                       and launch nothing
  * After the call the current state is in state_hi/lo.
+ *
+ * Work-unit tables are built once per distinct grid array and kept (device
+ * memory of the library); with PML_FLAG_GRAPH the step loop is a captured
+ * hipGraph -- the whole loop for up to 64 launches, else a 32-launch graph
+ * replayed -- instantiated once per distinct set of arguments and kept, so
+ * a repeated call costs no host-side preparation.  pml_graph_cache_clear()
+ * synchronises every device and releases the graphs and the tables.
  */
 #define PML_FLAG_GRAPH 1
 #define PML_FLAG_CONSTRAIN_INPUT 2
+#define PML_FLAG_PREPARE 4
 
 int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
                   int integrator_id, int precision, int ndim,
@@ -144,6 +164,8 @@ int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
                   void *snapshots_hi, void *snapshots_lo,
                   int64_t snapshot_every, int tile_config, int flags,
                   void *stream);
+
+int pml_graph_cache_clear(void);
 
 /* Number of kernel variants for (eq, integrator, precision) and a short
  * description of variant i (for the autotuner). */
@@ -209,13 +231,15 @@ int pml_secant_step(int precision, int mode, const void *p, const void *p1,
  * ghosts: per axis a, FP64 lower/upper slab arrays of the derivative
  * constraints (NULL = none), laid out as the array with axis a removed.
  * ycons: full-grid FP64 y constraints (NaN = free) or NULL.
- * sumsq_out: device FP64 scalar receiving sum((y_new-y_old)^2).
+ * sumsq_out: device FP64 scalar receiving sum((y_new-y_old)^2), summed in
+ * a fixed order (bitwise reproducible); work: device workspace of
+ * PML_REDUCE_SLOTS doubles.
  */
 int pml_jacobi_sweep(int precision, const void *y_old, void *y_new,
                      const void *lap, const int64_t *shape, int n_fields,
                      const double *dx, const double *const *ghost_lo,
                      const double *const *ghost_hi, const double *ycons,
-                     double *sumsq_out, void *stream);
+                     double *sumsq_out, double *work, void *stream);
 /*
  * The whole Jacobi loop of anti_laplacian (numerical_differentiator.py:
  * 872-927): sweeps (as pml_jacobi_sweep, y -> work -> y ...) until
@@ -223,7 +247,9 @@ int pml_jacobi_sweep(int precision, const void *y_old, void *y_new,
  * after convergence does nothing) and the host reading the loop state once
  * per `batch` sweeps.  y holds the start (constraints already applied) and
  * receives the result; work is a second array of the same size; state is a
- * device buffer of at least 32 bytes (8-byte aligned).  *sweeps_out = the
+ * device buffer of PML_JACOBI_STATE_BYTES (8-byte aligned).  The update
+ * norm is summed in a fixed order, so the sweep count and the result are
+ * bitwise reproducible.  *sweeps_out = the
  * sweeps taken, or -(sweeps issued) if max_sweeps ran out first; *diff_out
  * (may be NULL) = the last update norm.
  */
@@ -261,7 +287,10 @@ int pml_jacobi_solve_weighted(int precision, void *y, void *work, const void *la
  *   pml_parareal_update: u_next = g_new + corr (one slice, n elements) and
  *     per field (n_fields planes of n/n_fields) the sum of
  *     (u_next - u_old)^2 into sumsq[n_fields] (accumulated: the caller
- *     zeroes it).
+ *     zeroes it).  The sums are formed in a fixed order (per-block partials
+ *     in work, then added in block order), so norms and iteration counts
+ *     are bitwise reproducible; work is a device workspace of
+ *     PML_REDUCE_SLOTS * n_fields doubles.
  */
 int pml_parareal_correction(int f_precision, const void *f_end_hi,
                             const void *f_end_lo, int64_t f_stride,
@@ -269,7 +298,7 @@ int pml_parareal_correction(int f_precision, const void *f_end_hi,
                             int64_t n_slices, void *stream);
 int pml_parareal_update(const double *g_new, const double *corr,
                         double *u_next, int64_t n, int n_fields,
-                        double *sumsq, void *stream);
+                        double *sumsq, double *work, void *stream);
 
 /* Imposes the Dirichlet edge values of one grid's boundary block (pml_grid
  * layout, FP64) on `batch` consecutive FP64 field-major copies of the grid

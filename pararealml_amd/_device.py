@@ -341,12 +341,15 @@ class FDMBatch:
         constrain_input: bool = False,
         first_grid: int = 0,
         n_grids: Optional[int] = None,
+        prepare: bool = False,
     ) -> None:
         """Advances grids ``[first_grid, first_grid + n_grids)`` (default: all)
         by ``n_steps`` steps in place.  ``snapshots`` is an optional
         ``(hi, lo)`` pair of device tensors receiving the batch state after
         every ``snapshot_every``-th step; ``constrain_input`` imposes the
-        Dirichlet values on the state before the first step."""
+        Dirichlet values on the state before the first step.  ``prepare``
+        builds (and caches) the work-unit tables and the step-loop graph of
+        this call without launching anything."""
         snap_hi = snap_lo = None
         if snapshots is not None:
             snap_hi, snap_lo = snapshots
@@ -374,7 +377,8 @@ class FDMBatch:
                 int(snapshot_every),
                 int(tile_config),
                 (nat.PML_FLAG_GRAPH if use_graph else 0)
-                | (nat.PML_FLAG_CONSTRAIN_INPUT if constrain_input else 0),
+                | (nat.PML_FLAG_CONSTRAIN_INPUT if constrain_input else 0)
+                | (nat.PML_FLAG_PREPARE if prepare else 0),
                 self._sptr(),
             )
         )

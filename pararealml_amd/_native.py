@@ -30,8 +30,12 @@ PML_EQ = {
     "navier_stokes": 6,
 }
 
+PML_REDUCE_SLOTS = 2048
+PML_JACOBI_STATE_BYTES = 32 + 8 * PML_REDUCE_SLOTS
+
 PML_FLAG_GRAPH = 1
 PML_FLAG_CONSTRAIN_INPUT = 2
+PML_FLAG_PREPARE = 4
 
 PML_INT_FE = 0
 PML_INT_MIDPOINT = 1
@@ -67,6 +71,7 @@ SIGNATURES = {
             c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p,
         ],
     ),
+    "pml_graph_cache_clear": (c_int, []),
     "pml_fdm_configs": (c_int, [c_int, c_int, c_int]),
     "pml_fdm_config_name": (c_char_p, [c_int, c_int, c_int, c_int]),
     "pml_stencil_first": (
@@ -92,7 +97,7 @@ SIGNATURES = {
         c_int,
         [c_int, c_void_p, c_void_p, c_void_p, POINTER(c_int64), c_int,
          POINTER(c_double), POINTER(c_void_p), POINTER(c_void_p), c_void_p,
-         c_void_p, c_void_p],
+         c_void_p, c_void_p, c_void_p],
     ),
     "pml_jacobi_solve": (
         c_int,
@@ -114,7 +119,7 @@ SIGNATURES = {
          c_void_p],
     ),
     "pml_parareal_update": (
-        c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p]
+        c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_void_p, c_void_p, c_void_p]
     ),
     "pml_from_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "pml_to_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),

@@ -51,15 +51,28 @@ def _compile(src: Path) -> Path:
     return obj
 
 
-def build(verbose: bool = True) -> Path:
-    """Compiles every ``csrc/*.hip`` and links ``libpml.so``; always rebuilds."""
+def _stale(src: Path, headers) -> bool:
+    obj = BUILD_DIR / (src.stem + ".o")
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    return src.stat().st_mtime > t or any(h.stat().st_mtime > t for h in headers)
+
+
+def build(verbose: bool = True, incremental: bool = False) -> Path:
+    """Compiles every ``csrc/*.hip`` and links ``libpml.so``.  Always
+    rebuilds every object unless ``incremental`` (then only objects older
+    than their source or any header)."""
     if not Path(HIPCC).exists() and shutil.which(HIPCC) is None:
         raise RuntimeError(f"hipcc not found at {HIPCC}")
     BUILD_DIR.mkdir(parents=True, exist_ok=True)
     sources = sorted(CSRC.glob("*.hip"))
-    workers = max(1, min(len(sources), os.cpu_count() or 1, 8))
+    headers = sorted(CSRC.glob("*.h")) + sorted((PKG.parent / "include").glob("*.h"))
+    todo = [s for s in sources if not incremental or _stale(s, headers)]
+    workers = max(1, min(max(len(todo), 1), os.cpu_count() or 1, 8))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
-        objs = list(ex.map(_compile, sources))
+        list(ex.map(_compile, todo))
+    objs = [BUILD_DIR / (s.stem + ".o") for s in sources]
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", str(tmp), *map(str, objs)]
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -72,4 +85,4 @@ def build(verbose: bool = True) -> Path:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv[1:])

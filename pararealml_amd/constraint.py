@@ -4,7 +4,7 @@ Semantics follow pararealml/constraint.py:7-131: a constraint is a read-only
 pair of a boolean mask and the values the masked elements take; applying it
 writes the values in place.  The device kernels never see masks: the boundary
 constraints of a problem are flattened into per-edge value arrays where NaN
-marks an unconstrained element (see ``pararealml_amd/_edges.py``).
+marks an unconstrained element (``pararealml_amd._device.device_problem``).
 """
 
 from typing import Optional, Sequence, Union

@@ -75,10 +75,12 @@ __global__ void k_correction(const void *f_hi, const void *f_lo, int64_t f_strid
 }
 
 // u_next = g_new + corr; per field sum of squared updates against the
-// previous u_next (parareal_operator.py:53-100, 183-185).  One block-level
-// tree reduction per field, then one atomic per block.
+// previous u_next (parareal_operator.py:53-100, 183-185).  Each block
+// reduces its share (grid-stride order, then a fixed tree) and writes one
+// partial per field to work[f * gridDim.x + block]; k_sum_partials adds the
+// partials in block order, so the norm does not depend on scheduling.
 __global__ void k_update(const double *g, const double *corr, double *u, int64_t n,
-                         int n_fields, double *sumsq) {
+                         int n_fields, double *work) {
   __shared__ double red[256];
   const int64_t per_field = n / n_fields;
   for (int f = 0; f < n_fields; ++f) {
@@ -97,9 +99,25 @@ __global__ void k_update(const double *g, const double *corr, double *u, int64_t
       if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
-    if (threadIdx.x == 0) atomicAdd(&sumsq[f], red[0]);
+    if (threadIdx.x == 0) work[static_cast<int64_t>(f) * gridDim.x + blockIdx.x] = red[0];
     __syncthreads();
   }
+}
+
+// out[f] += sum over b < n_parts of parts[f * n_parts + b], one block per f,
+// in a fixed order (strided per-thread sums, then a fixed tree).
+__global__ void k_sum_partials(const double *parts, int n_parts, double *out) {
+  __shared__ double red[256];
+  const double *p = parts + static_cast<int64_t>(blockIdx.x) * n_parts;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < n_parts; b += blockDim.x) acc += p[b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] += red[0];
 }
 
 // ---- 2x grid transfer ---------------------------------------------------------
@@ -276,14 +294,18 @@ extern "C" int pml_parareal_correction(int f_precision, const void *f_end_hi,
 }
 
 extern "C" int pml_parareal_update(const double *g_new, const double *corr, double *u_next,
-                                   int64_t n, int n_fields, double *sumsq, void *stream) {
-  if (n <= 0 || n_fields <= 0 || n % n_fields != 0 || !g_new || !corr || !u_next || !sumsq)
+                                   int64_t n, int n_fields, double *sumsq, double *work,
+                                   void *stream) {
+  if (n <= 0 || n_fields <= 0 || n % n_fields != 0 || !g_new || !corr || !u_next || !sumsq ||
+      !work)
     return fail(PML_E_INVALID, "bad update arguments");
   hipStream_t st = as_stream(stream);
   int nb = blocks_for(n / n_fields, 256);
-  if (nb > 1024) nb = 1024;
+  if (nb > PML_REDUCE_SLOTS) nb = PML_REDUCE_SLOTS;
   hipLaunchKernelGGL(k_update, dim3(nb), dim3(256), 0, st, g_new, corr, u_next, n, n_fields,
-                     sumsq);
+                     work);
+  PML_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_partials, dim3(n_fields), dim3(256), 0, st, work, nb, sumsq);
   PML_LAUNCH_CHECK();
   return PML_OK;
 }

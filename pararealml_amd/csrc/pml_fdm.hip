@@ -4,7 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
-#include <map>
+#include <unordered_map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -29,10 +29,10 @@ static const std::vector<Variant> *find_variants(int eq, int integ, int prec, in
   return v;
 }
 
-// Device copies of work-unit and grid tables, keyed by their contents and
-// kept for the life of the process (they are small), so that repeated calls
-// with the same batch layout upload nothing and calls in flight never see a
-// table overwritten.
+// Device copies of work-unit and grid tables.  They are derived from the
+// caller's grid array and the variant's unit shape alone, so the cache key
+// is that array (plus unit shape, dimensionality and device) and the unit
+// list is only built on a miss.  Tables live until pml_graph_cache_clear().
 struct TableSet {
   void *units = nullptr;
   void *grids = nullptr;
@@ -40,30 +40,125 @@ struct TableSet {
 };
 
 static std::mutex g_mu;
-static std::map<std::string, TableSet> g_tables;
+static std::unordered_map<std::string, TableSet> g_tables;
 
-static int device_tables(const std::vector<TileDev> &units, const std::vector<GridDev> &grids,
-                         hipStream_t st, TableSet *out) {
-  std::string key(reinterpret_cast<const char *>(units.data()), units.size() * sizeof(TileDev));
-  key.append(reinterpret_cast<const char *>(grids.data()), grids.size() * sizeof(GridDev));
+// Instantiated step-loop graphs.  A graph records its kernels' arguments by
+// value (buffer pointers, dt, coefficients, tables), so the key holds all of
+// them; a hit replays the graph without capturing anything.  At most
+// kMaxGraphs are kept; the oldest is destroyed after a device synchronise
+// (a replay may still be in flight on any stream).
+struct GraphEntry {
+  hipGraphExec_t exec = nullptr;
+  hipGraph_t graph = nullptr;
+  int device = 0;
+  uint64_t stamp = 0;
+};
+static std::unordered_map<std::string, GraphEntry> g_graphs;
+static uint64_t g_stamp = 0;
+static constexpr size_t kMaxGraphs = 64;
+
+static std::string grid_key(const pml_grid *grids, int n_grids, const Variant &V, int ndim,
+                            int dev) {
+  std::string key(reinterpret_cast<const char *>(grids), sizeof(pml_grid) * n_grids);
+  const int extra[4] = {V.ox, V.oy, ndim, dev};
+  key.append(reinterpret_cast<const char *>(extra), sizeof(extra));
+  return key;
+}
+
+static void destroy_entry(GraphEntry &e) {
+  if (e.exec) hipGraphExecDestroy(e.exec);
+  if (e.graph) hipGraphDestroy(e.graph);
+  e.exec = nullptr;
+  e.graph = nullptr;
+}
+
+// Caller holds g_mu.
+static int evict_oldest_graph() {
+  auto oldest = g_graphs.end();
+  for (auto it = g_graphs.begin(); it != g_graphs.end(); ++it)
+    if (oldest == g_graphs.end() || it->second.stamp < oldest->second.stamp) oldest = it;
+  if (oldest == g_graphs.end()) return PML_OK;
+  int cur = 0;
+  PML_HIP_CHECK(hipGetDevice(&cur));
+  PML_HIP_CHECK(hipSetDevice(oldest->second.device));
+  hipError_t e = hipDeviceSynchronize();
+  hipSetDevice(cur);
+  if (e != hipSuccess)
+    return fail(PML_E_HIP, std::string("hipDeviceSynchronize: ") + hipGetErrorString(e));
+  destroy_entry(oldest->second);
+  g_graphs.erase(oldest);
+  return PML_OK;
+}
+
+}  // namespace pml
+
+using namespace pml;
+
+extern "C" int pml_graph_cache_clear(void) {
+  std::lock_guard<std::mutex> lock(g_mu);
+  int cur = 0;
+  PML_HIP_CHECK(hipGetDevice(&cur));
+  int count = 0;
+  PML_HIP_CHECK(hipGetDeviceCount(&count));
+  // Everything in flight on any device may still read the tables or run a
+  // graph: wait for all of it before releasing them.
+  for (int d = 0; d < count; ++d) {
+    PML_HIP_CHECK(hipSetDevice(d));
+    PML_HIP_CHECK(hipDeviceSynchronize());
+  }
+  PML_HIP_CHECK(hipSetDevice(cur));
+  for (auto &kv : g_graphs) destroy_entry(kv.second);
+  g_graphs.clear();
+  for (auto &kv : g_tables) {
+    if (kv.second.units) hipFree(kv.second.units);
+    if (kv.second.grids) hipFree(kv.second.grids);
+  }
+  g_tables.clear();
+  return PML_OK;
+}
+
+namespace pml {
+
+static int device_tables(const pml_grid *grids, int n_grids, const Variant &V, int ndim,
+                         TableSet *out) {
   int dev = 0;
   PML_HIP_CHECK(hipGetDevice(&dev));
-  key.append(reinterpret_cast<const char *>(&dev), sizeof(dev));
+  const std::string key = grid_key(grids, n_grids, V, ndim, dev);
   std::lock_guard<std::mutex> lock(g_mu);
   auto it = g_tables.find(key);
   if (it != g_tables.end()) {
     *out = it->second;
     return PML_OK;
   }
+  std::vector<GridDev> gd(n_grids);
+  std::vector<TileDev> units;
+  for (int g = 0; g < n_grids; ++g) {
+    const pml_grid &s = grids[g];
+    GridDev &d = gd[g];
+    d.n0 = s.n0;
+    d.n1 = s.n1;
+    d.off = s.state_offset;
+    d.bc = s.bc_offset;
+    d.inv_dx0sq = 1.0 / (s.dx0 * s.dx0);
+    d.inv_dx1sq = 1.0 / (s.dx1 * s.dx1);
+    d.inv_2dx0 = 1.0 / (2.0 * s.dx0);
+    d.inv_2dx1 = 1.0 / (2.0 * s.dx1);
+    d.m2dx0 = -2.0 * s.dx0;
+    d.m2dx1 = -2.0 * s.dx1;
+    d.p2dx0 = 2.0 * s.dx0;
+    d.p2dx1 = 2.0 * s.dx1;
+    for (int i0 = 0; i0 < s.n0; i0 += V.oy)
+      for (int j0 = 0; j0 < s.n1; j0 += V.ox) units.push_back(TileDev{g, i0, j0, 0});
+  }
+  if (units.size() > (1u << 30)) return fail(PML_E_INVALID, "batch too large");
   TableSet t;
   PML_HIP_CHECK(hipMalloc(&t.units, std::max<size_t>(units.size() * sizeof(TileDev), 16)));
-  PML_HIP_CHECK(hipMalloc(&t.grids, std::max<size_t>(grids.size() * sizeof(GridDev), 16)));
+  PML_HIP_CHECK(hipMalloc(&t.grids, std::max<size_t>(gd.size() * sizeof(GridDev), 16)));
   // Synchronous copies: the host vectors are temporaries.
   PML_HIP_CHECK(hipMemcpy(t.units, units.data(), units.size() * sizeof(TileDev),
                           hipMemcpyHostToDevice));
-  PML_HIP_CHECK(hipMemcpy(t.grids, grids.data(), grids.size() * sizeof(GridDev),
+  PML_HIP_CHECK(hipMemcpy(t.grids, gd.data(), gd.size() * sizeof(GridDev),
                           hipMemcpyHostToDevice));
-  (void)st;
   t.n_units = static_cast<int>(units.size());
   g_tables.emplace(key, t);
   *out = t;
@@ -113,6 +208,7 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
     return fail(PML_E_INVALID, "compensated snapshots need a lo plane");
   const bool use_graph = flags & PML_FLAG_GRAPH;
   const bool constrain = flags & PML_FLAG_CONSTRAIN_INPUT;
+  const bool prepare = flags & PML_FLAG_PREPARE;
 
   // Two-column stream variants move two values per lane instruction: every
   // grid's row length and state offset must be even and the planes aligned
@@ -137,9 +233,6 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
   const bool can_fuse = V.steps > 1 && (!snapshots_hi || snapshot_every % V.steps == 0);
   const int steps_per_launch = can_fuse ? V.steps : 1;
 
-  // Grid and work-unit tables.
-  std::vector<GridDev> gd(n_grids);
-  std::vector<TileDev> units;
   for (int g = 0; g < n_grids; ++g) {
     const pml_grid &s = grids[g];
     if (s.n1 < 3 || (ndim == 2 && s.n0 < 3) || (ndim == 1 && s.n0 != 1))
@@ -151,26 +244,10 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
     if (s.bc_offset < 0) return fail(PML_E_INVALID, "negative bc_offset");
     if (static_cast<int64_t>(n_fields) * s.n0 * s.n1 >= (int64_t(1) << 31))
       return fail(PML_E_INVALID, "a single grid must hold fewer than 2^31 values");
-    GridDev &d = gd[g];
-    d.n0 = s.n0;
-    d.n1 = s.n1;
-    d.off = s.state_offset;
-    d.bc = s.bc_offset;
-    d.inv_dx0sq = 1.0 / (s.dx0 * s.dx0);
-    d.inv_dx1sq = 1.0 / (s.dx1 * s.dx1);
-    d.inv_2dx0 = 1.0 / (2.0 * s.dx0);
-    d.inv_2dx1 = 1.0 / (2.0 * s.dx1);
-    d.m2dx0 = -2.0 * s.dx0;
-    d.m2dx1 = -2.0 * s.dx1;
-    d.p2dx0 = 2.0 * s.dx0;
-    d.p2dx1 = 2.0 * s.dx1;
-    for (int i0 = 0; i0 < s.n0; i0 += V.oy)
-      for (int j0 = 0; j0 < s.n1; j0 += V.ox) units.push_back(TileDev{g, i0, j0, 0});
   }
-  if (units.size() > (1u << 30)) return fail(PML_E_INVALID, "batch too large");
   hipStream_t st = as_stream(stream);
   TableSet tab;
-  int rc = device_tables(units, gd, st, &tab);
+  int rc = device_tables(grids, n_grids, V, ndim, &tab);
   if (rc) return rc;
 
   FdmArgs a{};
@@ -211,9 +288,14 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
     return PML_OK;
   };
 
+  const bool graphed = use_graph && st != nullptr && !snapshots_hi;
+  KernelFn main_fn = can_fuse ? V.fn : V.fn_single;
+  const int64_t fused_launches = n_steps / steps_per_launch;
+  if (prepare && !(graphed && fused_launches >= 2)) return PML_OK;  // tables only
+
   // Dirichlet values imposed on the input state in place (it is overwritten
   // by the result anyway).
-  if (constrain && n_steps > 0) {
+  if (constrain && n_steps > 0 && !prepare) {
     const dim3 cgrid(n_grids, n_fields);
     if (precision == PML_F64)
       hipLaunchKernelGGL(fdm_constrain<PrecF64>, cgrid, dim3(256), 0, st, a.grids, bc_values,
@@ -227,50 +309,110 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
     PML_LAUNCH_CHECK();
   }
 
-  const int64_t fused_launches = n_steps / steps_per_launch;
   int64_t li = 0;
-  // The first launch runs eagerly (it also warms the unit tables).
-  if (fused_launches > 0) {
-    rc = launch(can_fuse ? V.fn : V.fn_single, steps_per_launch);
-    if (rc) return rc;
-    li = 1;
-  }
-  // Graph replay: a fixed block of launches (even, so the ping-pong parity
-  // is the same at every replay) captured once; no snapshot may fall inside
-  // a replayed block and stream capture needs a created (non-null) stream.
-  const int64_t block = 32;
-  if (use_graph && st != nullptr && !snapshots_hi && fused_launches - li >= 2 * block) {
-    hipGraph_t graph = nullptr;
+
+  // Graph replay (a created stream and no snapshots): a run of up to
+  // kWhole launches is one graph replayed once; a longer run replays a
+  // graph of kBlock launches (even, so the ping-pong parity is the same at
+  // every replay) and launches the remainder directly.  Instantiated graphs
+  // are cached with every argument they recorded, so repeated calls replay
+  // without capturing.
+  constexpr int64_t kWhole = 64, kBlock = 32;
+  if (graphed && fused_launches >= 2) {
+    const int64_t glen = fused_launches <= kWhole ? fused_launches : kBlock;
+    const int64_t reps = fused_launches <= kWhole ? 1 : fused_launches / kBlock;
+    int dev = 0;
+    PML_HIP_CHECK(hipGetDevice(&dev));
+    struct {
+      const void *fn, *tiles, *grids, *bc, *p0, *p1, *p2, *p3;
+      double dt, p[6];
+      int64_t glen, n_tiles;
+      int ndim, dev, blocks, nt;
+    } gk{};
+    gk.fn = reinterpret_cast<const void *>(main_fn);
+    gk.tiles = a.tiles;
+    gk.grids = a.grids;
+    gk.bc = bc_values;
+    gk.p0 = state_hi;
+    gk.p1 = state_lo;
+    gk.p2 = work_hi;
+    gk.p3 = work_lo;
+    gk.dt = dt;
+    for (int i = 0; i < 6; ++i) gk.p[i] = a.p[i];
+    gk.glen = glen;
+    gk.n_tiles = a.n_tiles;
+    gk.ndim = ndim;
+    gk.dev = dev;
+    gk.blocks = blocks;
+    gk.nt = V.nt;
+    const std::string key(reinterpret_cast<const char *>(&gk), sizeof(gk));
     hipGraphExec_t exec = nullptr;
-    const int64_t done_before = done;
-    PML_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-    for (int64_t i = 0; i < block; ++i) {
-      rc = launch(can_fuse ? V.fn : V.fn_single, steps_per_launch);
-      if (rc) {
-        hipStreamEndCapture(st, &graph);
-        if (graph) hipGraphDestroy(graph);
-        return rc;
+    {
+      std::lock_guard<std::mutex> lock(g_mu);
+      auto it = g_graphs.find(key);
+      if (it != g_graphs.end()) {
+        it->second.stamp = ++g_stamp;
+        exec = it->second.exec;
       }
     }
-    PML_HIP_CHECK(hipStreamEndCapture(st, &graph));
-    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-      hipGraphDestroy(graph);
-      return fail(PML_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    if (!exec) {
+      hipGraph_t graph = nullptr;
+      PML_HIP_CHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      for (int64_t i = 0; i < glen; ++i) {
+        rc = launch(main_fn, steps_per_launch);
+        if (rc) {
+          hipStreamEndCapture(st, &graph);
+          if (graph) hipGraphDestroy(graph);
+          return rc;
+        }
+      }
+      PML_HIP_CHECK(hipStreamEndCapture(st, &graph));
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        return fail(PML_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+      }
+      std::lock_guard<std::mutex> lock(g_mu);
+      if (g_graphs.size() >= kMaxGraphs) {
+        rc = evict_oldest_graph();
+        if (rc) {
+          hipGraphExecDestroy(exec);
+          hipGraphDestroy(graph);
+          return rc;
+        }
+      }
+      GraphEntry ent;
+      ent.exec = exec;
+      ent.graph = graph;
+      ent.device = dev;
+      ent.stamp = ++g_stamp;
+      g_graphs.emplace(key, ent);
+      // Capture recorded the launches without running them: rewind.
+      cur_hi = state_hi;
+      cur_lo = state_lo;
+      nxt_hi = work_hi;
+      nxt_lo = work_lo;
+      done = 0;
     }
-    const int64_t reps = (fused_launches - li) / block;
-    for (int64_t r = 0; r < reps && e == hipSuccess; ++r) e = hipGraphLaunch(exec, st);
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
-    if (e != hipSuccess)
-      return fail(PML_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
-    // Capture recorded (not ran) the block; the buffers are back in place
-    // after every even-length replay.
-    done = done_before + reps * block * steps_per_launch;
-    li += reps * block;
+    if (prepare) return PML_OK;
+    for (int64_t r = 0; r < reps; ++r) {
+      hipError_t e = hipGraphLaunch(exec, st);
+      if (e != hipSuccess)
+        return fail(PML_E_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+    }
+    // After the replays the state is in state_hi (even graph length or an
+    // even number of replays) or in work_hi.
+    if ((glen * reps) & 1) {
+      cur_hi = work_hi;
+      cur_lo = work_lo;
+      nxt_hi = state_hi;
+      nxt_lo = state_lo;
+    }
+    done = glen * reps * steps_per_launch;
+    li = glen * reps;
   }
   for (; li < fused_launches; ++li) {
-    rc = launch(can_fuse ? V.fn : V.fn_single, steps_per_launch);
+    rc = launch(main_fn, steps_per_launch);
     if (rc) return rc;
   }
   while (done < n_steps) {

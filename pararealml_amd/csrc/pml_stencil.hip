@@ -4,6 +4,8 @@
 // pararealml/operators/fdm/numerical_differentiator.py:1012-1242.
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "pml_common.h"
 
 namespace pml {
@@ -105,11 +107,16 @@ struct JacobiState {
   long long sweeps;
   int done;
   int pad;
+  double partials[PML_REDUCE_SLOTS];  // one per sweep block, summed in order
 };
+static_assert(sizeof(JacobiState) == PML_JACOBI_STATE_BYTES, "JacobiState layout");
 
+// Each block writes its sum of squared updates to partials[blockIdx.x]
+// (grid-stride order, then a fixed tree); k_jacobi_check adds them in block
+// order, so the norm and the sweep count do not depend on scheduling.
 template <typename T>
 __global__ void k_jacobi(const T *y, T *yn, const T *lap, int64_t total, int nf,
-                         JacobiGeom g, const double *ycons, double *sumsq,
+                         JacobiGeom g, const double *ycons, double *partials,
                          const int *done) {
   // A loop that has converged turns its remaining sweeps into no-ops.
   if (done && *done) return;
@@ -170,7 +177,25 @@ __global__ void k_jacobi(const T *y, T *yn, const T *lap, int64_t total, int nf,
     if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomicAdd(sumsq, red[0]);
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+// sum of parts[0 .. n) in a fixed order into *out (one block).
+__device__ void sum_in_order(const double *parts, int n, double *out) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < n; b += blockDim.x) acc += parts[b];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+__global__ void k_jacobi_sum(const double *parts, int n, double *out) {
+  sum_in_order(parts, n, out);
 }
 
 __global__ void k_jacobi_init(JacobiState *st) {
@@ -183,16 +208,17 @@ __global__ void k_jacobi_init(JacobiState *st) {
   }
 }
 
-// After a sweep: count it, stop once ||y_new - y_old||_2 <= tol (the
-// reference's loop condition), clear the accumulator for the next sweep.
-__global__ void k_jacobi_check(JacobiState *st, double tol) {
+// After a sweep of n_parts blocks: add its partials in order, count it,
+// stop once ||y_new - y_old||_2 <= tol (the reference's loop condition).
+// A loop that is already done leaves the state as it is.
+__global__ void k_jacobi_check(JacobiState *st, int n_parts, double tol) {
+  if (st->done) return;
+  sum_in_order(st->partials, n_parts, &st->sumsq);
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (!st->done) {
-      st->sweeps += 1;
-      st->last_diff = sqrt(st->sumsq);
-      if (st->last_diff <= tol) st->done = 1;
-    }
-    st->sumsq = 0.0;
+    st->sweeps += 1;
+    st->last_diff = sqrt(st->sumsq);
+    if (st->last_diff <= tol) st->done = 1;
   }
 }
 
@@ -399,8 +425,9 @@ static int jacobi_geom(const int64_t *shape, int n_fields, const double *dx,
 extern "C" int pml_jacobi_sweep(int precision, const void *y_old, void *y_new, const void *lap,
                                 const int64_t *shape, int n_fields, const double *dx,
                                 const double *const *ghost_lo, const double *const *ghost_hi,
-                                const double *ycons, double *sumsq_out, void *stream) {
-  if (!y_old || !y_new || !lap || !shape || !dx || !sumsq_out || n_fields <= 0)
+                                const double *ycons, double *sumsq_out, double *work,
+                                void *stream) {
+  if (!y_old || !y_new || !lap || !shape || !dx || !sumsq_out || !work || n_fields <= 0)
     return fail(PML_E_INVALID, "bad Jacobi arguments");
   JacobiGeom g{};
   int64_t total = 0;
@@ -408,17 +435,19 @@ extern "C" int pml_jacobi_sweep(int precision, const void *y_old, void *y_new, c
     return rc;
   hipStream_t st = as_stream(stream);
   int nb = grid_for(total);
-  if (nb > 2048) nb = 2048;
+  if (nb > PML_REDUCE_SLOTS) nb = PML_REDUCE_SLOTS;
   if (precision == PML_F64)
     hipLaunchKernelGGL(k_jacobi<double>, dim3(nb), dim3(256), 0, st, (const double *)y_old,
                        (double *)y_new, (const double *)lap, total, n_fields, g, ycons,
-                       sumsq_out, nullptr);
+                       work, nullptr);
   else if (precision == PML_F32)
     hipLaunchKernelGGL(k_jacobi<float>, dim3(nb), dim3(256), 0, st, (const float *)y_old,
                        (float *)y_new, (const float *)lap, total, n_fields, g, ycons,
-                       sumsq_out, nullptr);
+                       work, nullptr);
   else
     return fail(PML_E_INVALID, "Jacobi sweep takes FP32 or FP64 arrays");
+  PML_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_jacobi_sum, dim3(1), dim3(256), 0, st, work, nb, sumsq_out);
   PML_LAUNCH_CHECK();
   return PML_OK;
 }
@@ -452,9 +481,12 @@ static int jacobi_solve(int precision, void *y, void *work, const void *lap,
   hipStream_t st = as_stream(stream);
   JacobiState *js = static_cast<JacobiState *>(state);
   int nb = grid_for(total);
-  if (nb > 2048) nb = 2048;
+  if (nb > PML_REDUCE_SLOTS) nb = PML_REDUCE_SLOTS;
   hipLaunchKernelGGL(k_jacobi_init, dim3(1), dim3(64), 0, st, js);
-  JacobiState host{};
+  JacobiState host;
+  host.done = 0;
+  host.sweeps = 0;
+  host.last_diff = 0.0;
   int64_t issued = 0;
   while (issued < max_sweeps) {
     const int64_t n = std::min<int64_t>(batch, max_sweeps - issued);
@@ -467,17 +499,18 @@ static int jacobi_solve(int precision, void *y, void *work, const void *lap,
         hipLaunchKernelGGL(k_jacobi<double>, dim3(nb), dim3(256), 0, st,
                            static_cast<const double *>(src), static_cast<double *>(dst),
                            static_cast<const double *>(lap), total, n_fields, g, ycons,
-                           &js->sumsq, &js->done);
+                           js->partials, &js->done);
       else
         hipLaunchKernelGGL(k_jacobi<float>, dim3(nb), dim3(256), 0, st,
                            static_cast<const float *>(src), static_cast<float *>(dst),
                            static_cast<const float *>(lap), total, n_fields, g, ycons,
-                           &js->sumsq, &js->done);
-      hipLaunchKernelGGL(k_jacobi_check, dim3(1), dim3(64), 0, st, js, tol);
+                           js->partials, &js->done);
+      hipLaunchKernelGGL(k_jacobi_check, dim3(1), dim3(256), 0, st, js, nb, tol);
     }
     issued += n;
     PML_LAUNCH_CHECK();
-    PML_HIP_CHECK(hipMemcpyAsync(&host, js, sizeof(host), hipMemcpyDeviceToHost, st));
+    PML_HIP_CHECK(hipMemcpyAsync(&host, js, offsetof(JacobiState, partials),
+                                 hipMemcpyDeviceToHost, st));
     PML_HIP_CHECK(hipStreamSynchronize(st));
     if (host.done) break;
   }

@@ -642,7 +642,8 @@ class ThreePointCentralDifferenceMethod(NumericalDifferentiator):
             mask = ~torch.isnan(ycons)
             cur.view(-1)[mask] = ycons[mask].to(cur.dtype)
         work = torch.empty_like(cur)
-        state = torch.zeros(4, dtype=torch.float64, device=cur.device)
+        state = torch.zeros(nat.PML_JACOBI_STATE_BYTES // 8, dtype=torch.float64,
+                            device=cur.device)
         lib = nat.device_lib()
         c_shape = (ctypes.c_int64 * 3)(*spatial)
         c_dx = (ctypes.c_double * 3)(*dxs)

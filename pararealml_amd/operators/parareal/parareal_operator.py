@@ -144,6 +144,7 @@ class PararealOperator(Operator):
             f_end = torch.empty((n, elems), **f64)
             corr = torch.empty((n, elems), **f64)
             sumsq = torch.zeros((n, n_fields), **f64)
+            reduce_work = torch.empty(nat.PML_REDUCE_SLOTS * n_fields, **f64)
 
             def sync_time(key, fn):
                 stream.synchronize()
@@ -187,7 +188,7 @@ class PararealOperator(Operator):
                             lib.pml_parareal_update(
                                 g_end[j].data_ptr(), corr[j].data_ptr(),
                                 u[j + 1].data_ptr(), elems, n_fields,
-                                sumsq[j].data_ptr(), sp,
+                                sumsq[j].data_ptr(), reduce_work.data_ptr(), sp,
                             )
                         )
 

@@ -35,13 +35,44 @@ def test_keys_follow_the_batch_size_class(fresh_cache):
     assert autotune.key_for(p, 0, "float64", cells=(1 << 14) + 1).endswith("/s")
 
 
+NAMES = ["stream-a", "stream-b", "tile-c", "tile-d"]
+
+
 def test_lookup_defaults_to_variant_zero_and_reads_the_cache_file(fresh_cache):
     p = _P()
-    assert autotune.lookup(p, 2, "float32c") == 0
-    fresh_cache.write_text(json.dumps({autotune.key_for(p, 2, "float32c"): 3}))
+    assert autotune.lookup(p, 2, "float32c", names=NAMES) == 0
+    fresh_cache.write_text(json.dumps({autotune.key_for(p, 2, "float32c"): "tile-d"}))
     autotune._cache = None
-    assert autotune.lookup(p, 2, "float32c") == 3
-    assert autotune.lookup(p, 2, "float32c", cells=p.plane * 128) == 0
+    assert autotune.lookup(p, 2, "float32c", names=NAMES) == 3
+    assert autotune.lookup(p, 2, "float32c", cells=p.plane * 128, names=NAMES) == 0
+
+
+def test_stale_cache_entries_fall_back_to_the_default_variant(fresh_cache):
+    p = _P()
+    key = autotune.key_for(p, 2, "float32c")
+    # a variant another build offered, and a bare index from an old cache
+    fresh_cache.write_text(json.dumps({key: "pruned-variant"}))
+    autotune._cache = None
+    assert autotune.lookup(p, 2, "float32c", names=NAMES) == 0
+    fresh_cache.write_text(json.dumps({key: 99}))
+    autotune._cache = None
+    assert autotune.lookup(p, 2, "float32c", names=NAMES) == 0
+    # reordered variants: the name, not the position, selects the kernel
+    fresh_cache.write_text(json.dumps({key: "tile-c"}))
+    autotune._cache = None
+    assert autotune.lookup(p, 2, "float32c", names=list(reversed(NAMES))) == 1
+
+
+def test_lookup_resolves_names_against_the_loaded_library(fresh_cache):
+    from pararealml_amd import _native as nat
+
+    p = _P()
+    names = autotune.variant_names(nat.PML_EQ["diffusion"], nat.PML_INT_RK4, nat.PML_F32C)
+    assert len(names) >= 2 and len(set(names)) == len(names)
+    key = autotune.key_for(p, nat.PML_INT_RK4, "float32c")
+    fresh_cache.write_text(json.dumps({key: names[-1]}))
+    autotune._cache = None
+    assert autotune.lookup(p, nat.PML_INT_RK4, "float32c") == len(names) - 1
 
 
 @pytest.mark.gpu

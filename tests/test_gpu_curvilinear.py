@@ -397,8 +397,13 @@ def test_implicit_integrators_match_oracle(method):
     the iteration stops per element at |dp| < 1.48e-8, so rounding
     differences between the two right-hand sides can end it one iteration
     apart.  Diffusion and the wave equation on a Cartesian mesh with
-    Dirichlet and Neumann edges, 8 steps of 0.01 (4x the explicit limit for
-    diffusion); a polar diffusion run stays finite."""
+    Dirichlet and Neumann edges, 8 steps at a size where SciPy's iteration
+    converges on every element (backward Euler 0.005, 2x the explicit limit
+    for diffusion; Crank-Nicolson 0.01); a polar diffusion run stays
+    finite.  At 0.01 backward Euler leaves some elements unconverged after
+    50 iterations: SciPy warns, and so does the device path."""
+    import warnings
+
     import pararealml_amd as pml
     from oracle import fdm_numpy as ora
     from pararealml_amd.operators.fdm import BackwardEulerMethod, CrankNicolsonMethod, \
@@ -406,17 +411,35 @@ def test_implicit_integrators_match_oracle(method):
 
     print(f"case: implicit {method}", flush=True)
     cls = {"backward_euler": BackwardEulerMethod, "crank_nicolson": CrankNicolsonMethod}[method]
+    d_t = 0.005 if method == "backward_euler" else 0.01
     probs = _diffusion_wave_problems("CARTESIAN")
     for name in ("diffusion", "wave"):
         cp, ic = probs[name]
-        ivp = pml.InitialValueProblem(cp, (0.0, 0.08), ic)
-        got = FDMOperator(cls(), DIFF, 0.01).solve(ivp).discrete_y()
-        _, want = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, 0.08), 0.01, method)
+        ivp = pml.InitialValueProblem(cp, (0.0, 8 * d_t), ic)
+        with warnings.catch_warnings():
+            warnings.simplefilter("error", RuntimeWarning)
+            got = FDMOperator(cls(), DIFF, d_t).solve(ivp).discrete_y()
+            _, want = ora.fdm_solve(cp, ic.discrete_y_0(True), (0.0, 8 * d_t), d_t, method)
         _close(got, want, 1e-6)
+    if method == "backward_euler":
+        cp, ic = probs["diffusion"]
+        ivp = pml.InitialValueProblem(cp, (0.0, 0.02), ic)
+        with pytest.warns(RuntimeWarning, match="some failed to converge after 50"):
+            FDMOperator(cls(), DIFF, 0.01).solve(ivp)
     for name in ("diffusion", "convection_diffusion"):
         cp, ic = _diffusion_wave_problems("POLAR")[name]
         got = FDMOperator(cls(), DIFF, 0.01).solve(pml.InitialValueProblem(cp, (0.0, 0.05), ic))
         assert np.all(np.isfinite(got.discrete_y()))
+
+
+def test_implicit_integrators_refuse_fp32():
+    import pararealml_amd as pml
+    from pararealml_amd.operators.fdm import BackwardEulerMethod, FDMOperator
+
+    cp, ic = _diffusion_wave_problems("CARTESIAN")["diffusion"]
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.02), ic)
+    with pytest.raises(NotImplementedError, match="float64"):
+        FDMOperator(BackwardEulerMethod(), DIFF, 0.01, precision="float32").solve(ivp)
 
 
 @pytest.mark.parametrize("integ", ["fe", "rk4"])
@@ -449,6 +472,92 @@ def test_navier_stokes_composed_matches_fused_poisson_path(integ):
     ivp = pml.InitialValueProblem(cp, (0.0, 5e-3), pml.DiscreteInitialCondition(cp, y0, True))
     y = FDMOperator(cls(), DIFF, 1e-3).solve(ivp).discrete_y()
     assert np.all(np.isfinite(y)) and float(np.max(np.abs(y[..., 1]))) > 0.0
+
+
+def test_polar_navier_stokes_step_relations():
+    """Two forward-Euler Navier-Stokes steps on a polar annulus, checked
+    against the relations the reference's step fixes (fdm_operator.py:
+    114-161), with the device operators pinned by the polar known answers:
+    psi_n solves lap(psi_n) = -w_{n-1} (interior, to the Jacobi tolerance),
+    u_n = d psi_{n-1} / d x1 and v_n = -d psi_{n-1} / d x0 from the pre-step
+    psi, and w_n = C(w_{n-1} + dt ((1/re) lap(w_{n-1}) - u_{n-1} dw/dx0
+    - v_{n-1} dw/dx1)) from the pre-step state."""
+    import pararealml_amd as pml
+    from pararealml_amd.constraint import apply_constraints_along_last_axis
+    from pararealml_amd.operators.fdm import FDMOperator, ForwardEulerMethod
+
+    print("case: polar navier-stokes relations", flush=True)
+    re_, d_t = 100.0, 1e-3
+    diff = ThreePointCentralDifferenceMethod(tol=1e-12)
+    mesh = Mesh([(1.0, 2.0), (0.0, 1.5)], [0.0625, 0.0625], CoordinateSystem.POLAR)
+    wall = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 4)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.NavierStokesEquation(re_), mesh, [(wall, wall)] * 2)
+    r, th = mesh.vertex_coordinate_grids
+    y0 = np.zeros(mesh.vertices_shape + (4,))
+    y0[..., 0] = np.exp(-((r - 1.5) ** 2 + (th - 0.75) ** 2) / 0.02)
+    y0[..., 1] = 0.05 * np.sin(np.pi * (r - 1.0)) * np.sin(np.pi * th / 1.5)
+    ic = pml.DiscreteInitialCondition(cp, y0, True)
+    y = FDMOperator(ForwardEulerMethod(), diff, d_t).solve(
+        pml.InitialValueProblem(cp, (0.0, 2 * d_t), ic)).discrete_y()
+    states = [ic.discrete_y_0(True), y[0], y[1]]
+    dbc = cp.static_boundary_vertex_constraints[1]
+    ycons = cp.static_y_vertex_constraints
+    inner = (slice(1, -1), slice(1, -1))
+    for n in (1, 2):
+        pre, post = states[n - 1], states[n]
+        lap_psi = diff.laplacian(post[..., 1:2], mesh, dbc[:, 1:2])[..., 0]
+        _close(lap_psi[inner], -pre[..., 0][inner], 1e-7)
+        g1 = diff.gradient(pre[..., 1:2], mesh, 1, dbc[:, 1:2])[..., 0]
+        g0 = diff.gradient(pre[..., 1:2], mesh, 0, dbc[:, 1:2])[..., 0]
+        _close(post[..., 2][inner], g1[inner], 1e-12)
+        _close(post[..., 3][inner], -g0[inner], 1e-12)
+        w = pre[..., 0:1]
+        rhs = (diff.laplacian(w, mesh, dbc[:, 0:1]) / re_
+               - pre[..., 2:3] * diff.gradient(w, mesh, 0, dbc[:, 0:1])
+               - pre[..., 3:4] * diff.gradient(w, mesh, 1, dbc[:, 0:1]))
+        want = apply_constraints_along_last_axis(ycons[0:1], w + d_t * rhs)
+        _close(post[..., 0:1], want, 1e-12)
+
+
+def test_polar_time_dependent_boundaries_one_step():
+    """A forward-Euler step on a polar mesh with a time-dependent Dirichlet
+    edge (sin 5t) and Neumann edge (0.3 t) equals the host composition
+    C_{t1}(y' + dt d lap(y', dbc(t0))) with y' = C_{t0}(y0) (the reference
+    constrains the initial state at t0, fdm_operator.py:55-63, and
+    evaluates the stage's derivative constraints at its own time and the y
+    constraints at the target time, 199-231), using the device Laplacian
+    pinned by the polar known answers."""
+    import pararealml_amd as pml
+    from pararealml_amd.constraint import apply_constraints_along_last_axis
+    from pararealml_amd.operators.fdm import FDMOperator, ForwardEulerMethod
+
+    print("case: polar dynamic boundaries one step", flush=True)
+    mesh = Mesh([(1.0, 2.0), (0.0, 1.2)], [0.1, 0.1], CoordinateSystem.POLAR)
+    bcs = [
+        (pml.DirichletBoundaryCondition(lambda x, t: np.full((len(x), 1), np.sin(5.0 * t)),
+                                        is_static=False),
+         pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)),
+        (pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True),
+         pml.NeumannBoundaryCondition(lambda x, t: np.full((len(x), 1), 0.3 * t),
+                                      is_static=False)),
+    ]
+    d, d_t, t0 = 0.4, 0.002, 0.1
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(2, d), mesh, bcs)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([1.5, 0.6]), 0.05 * np.eye(2))])
+    ivp = pml.InitialValueProblem(cp, (t0, t0 + 2 * d_t), ic)
+    got = FDMOperator(ForwardEulerMethod(), DIFF, d_t).solve(ivp).discrete_y()
+    y = ic.discrete_y_0(True).copy()
+    for k in range(2):
+        t_a, t_b = t0 + k * d_t, t0 + (k + 1) * d_t
+        y_bc_a, d_bc_a = cp.create_boundary_constraints(True, t_a)
+        if k == 0:
+            apply_constraints_along_last_axis(cp.create_y_vertex_constraints(y_bc_a), y)
+        y_bc_b, _ = cp.create_boundary_constraints(True, t_b)
+        nxt = y + d_t * (d * DIFF.laplacian(y, mesh, d_bc_a))
+        y = apply_constraints_along_last_axis(cp.create_y_vertex_constraints(y_bc_b), nxt)
+        _close(got[k], y, 1e-13)
+    # the derivative edge is not trivially zero at these times
+    assert abs(0.3 * (t0 + d_t)) > 0.01
 
 
 def test_curvilinear_stepping_unsupported_precision():
