@@ -48,6 +48,13 @@
  *                             164-188)
  *   pml_parareal_correction<- correction = F_end - G_end
  *                             (parareal_operator.py:164)
+ *   pml_program_create / pml_expr_eval
+ *                          <- SymbolMapper / FDMSymbolMapper: the lambdified
+ *                             right-hand sides (operators/symbol_mapper.py:
+ *                             160-271, fdm/fdm_symbol_mapper.py:45-158)
+ *   pml_ode_solve          <- FDMOperator.solve on a system without a mesh
+ *                             (fdm_operator.py:48-77, 190) with the explicit
+ *                             integrators (numerical_integrator.py:42-132)
  *   pml_graph_cache_clear  <- no counterpart (the reference has no device
  *                             caches): releases the step-loop graphs and
  *                             work-unit tables pml_fdm_solve keeps
@@ -322,6 +329,50 @@ int pml_restrict2x(int precision, const void *fine, void *coarse,
                    int64_t batch, int64_t m0, int64_t m1, void *stream);
 int pml_prolong2x(int precision, const void *coarse, void *fine,
                   int64_t batch, int64_t m0, int64_t m1, void *stream);
+
+/*
+ * Symbolic right-hand sides on the device.  The host compiles a system's
+ * right-hand sides (SymPy expressions over the reference's symbols: y, t,
+ * x, y-gradient, y-hessian, y-laplacian, y-divergence, y-curl,
+ * y-vector-laplacian) into a register-machine program: n_instr
+ * instructions of four int32 {op, dst, a, b} (ops in pml_expr.hip:
+ * constants, input loads, t, + - * /, pow, integer powers, negation,
+ * elementary functions, min/max/sign, output stores), n_regs registers
+ * (at most PML_EXPR_MAX_REGS), n_inputs per-element inputs and n_outputs
+ * outputs.  pml_program_create validates every instruction's operands
+ * against those counts and uploads the program to the current device.
+ *   pml_expr_eval: evaluates the program at `count` elements; input k of
+ *     element e is inputs[k][e * in_strides[k]] and output k goes to
+ *     outputs[k][e * out_strides[k]] (element strides; FP32 or FP64 by
+ *     precision, consts always FP64); t is the time symbol.  Replaces the
+ *     lambdified evaluation of SymbolMapper.map (symbol_mapper.py:222-271)
+ *     once the derivative symbols have been computed into arrays.
+ *   pml_ode_solve: advances `batch` independent states of an ODE system
+ *     (n = n_outputs unknowns; input k is y[k]) by n_steps explicit steps
+ *     (PML_INT_FE / MIDPOINT / RK4) of size dt, one thread per state, the
+ *     whole time loop in one launch; y is (batch, n) in/out; the time of
+ *     step s of state b is s * t_step[b] + t0[b] (the reference's
+ *     linspace grid; t0, t_step device FP64 arrays); snapshots (may be
+ *     NULL) receive every snapshot_every-th state at
+ *     snapshots[(k * batch + b) * n].  Replaces FDMOperator.solve for
+ *     systems without a mesh (fdm_operator.py:48-77 with
+ *     numerical_integrator.py:42-132).
+ */
+#define PML_EXPR_MAX_REGS 96
+#define PML_EXPR_MAX_IO 48
+typedef struct pml_program_s *pml_program;
+int pml_program_create(const int32_t *code, int n_instr, const double *consts,
+                       int n_consts, int n_regs, int n_inputs, int n_outputs,
+                       pml_program *out);
+int pml_program_destroy(pml_program program);
+int pml_expr_eval(int precision, pml_program program, const void *const *inputs,
+                  const int64_t *in_strides, void *const *outputs,
+                  const int64_t *out_strides, double t, int64_t count,
+                  void *stream);
+int pml_ode_solve(int precision, pml_program program, int integrator_id,
+                  int64_t batch, const double *t0, const double *t_step,
+                  double dt, int64_t n_steps, void *y, void *snapshots,
+                  int64_t snapshot_every, void *stream);
 
 /*
  * Convolutional surrogate (no counterpart in the reference; it replaces the

@@ -72,6 +72,20 @@ SIGNATURES = {
         ],
     ),
     "pml_graph_cache_clear": (c_int, []),
+    "pml_program_create": (
+        c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int, POINTER(c_void_p)]
+    ),
+    "pml_program_destroy": (c_int, [c_void_p]),
+    "pml_expr_eval": (
+        c_int,
+        [c_int, c_void_p, POINTER(c_void_p), POINTER(c_int64), POINTER(c_void_p),
+         POINTER(c_int64), c_double, c_int64, c_void_p],
+    ),
+    "pml_ode_solve": (
+        c_int,
+        [c_int, c_void_p, c_int, c_int64, c_void_p, c_void_p, c_double, c_int64, c_void_p,
+         c_void_p, c_int64, c_void_p],
+    ),
     "pml_fdm_configs": (c_int, [c_int, c_int, c_int]),
     "pml_fdm_config_name": (c_char_p, [c_int, c_int, c_int, c_int]),
     "pml_stencil_first": (

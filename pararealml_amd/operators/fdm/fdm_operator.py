@@ -21,10 +21,20 @@ stage) for the diffusion, convection-diffusion, wave, shallow-water,
 Burgers, Cahn-Hilliard and Navier-Stokes equations.
 Anything else raises ``NotImplementedError``; there is no host fallback.
 
+Any other equation -- every ODE system and any user-defined
+``DifferentialEquation`` -- runs its symbolic right-hand sides through a
+device program (``pararealml_amd.operators.device_program``): ODEs in one
+launch of the batched ODE kernel for the whole time loop
+(``pml_ode_solve``), PDEs through the composed loop with the derivative
+symbols computed by the differentiator's device operators and the
+expressions by ``pml_expr_eval``.
+
 Extensions over the reference (keyword-only, defaults reproduce it):
 ``precision`` ("float64", "float32" or "float32c" = compensated FP32),
 ``snapshot_every`` (keep every k-th step only), ``tile_config`` (kernel
-variant; None = the tuned or default one, see ``tune``) and ``use_graph``.
+variant; None = the tuned or default one, see ``tune``), ``use_graph`` and
+``symbolic_rhs`` (evaluate the symbolic right-hand side through the device
+program even when a fused kernel exists).
 """
 
 import warnings
@@ -45,6 +55,105 @@ from pararealml_amd.operators.fdm.numerical_integrator import ImplicitMethod, Nu
 from pararealml_amd.solution import Solution
 
 
+class _SymbolicDeviceRHS:
+    """Device evaluation of some of a system's right-hand sides on a
+    channel-last state: the FDMSymbolMapper's symbol map
+    (fdm_symbol_mapper.py:45-158) with every derivative symbol computed by
+    the differentiator's device operators (constraints of the symbol's own
+    components at the evaluation time) and the expressions by one
+    ``pml_expr_eval`` launch of their compiled program
+    (symbol_mapper.py:222-271)."""
+
+    def __init__(self, cp, diff, indices, dtype):
+        import sympy as sp
+        import torch
+
+        from pararealml_amd.operators.device_program import compile_expressions
+
+        rhs = cp.differential_equation.symbolic_equation_system.rhs
+        exprs = [sp.sympify(rhs[i]) for i in indices]
+        syms = sorted(set().union(*[e.free_symbols for e in exprs]), key=lambda s: s.name)
+        self.inputs = [s for s in syms if s.name != "t"]
+        index = {s: k for k, s in enumerate(self.inputs)}
+        self.program = compile_expressions(exprs, index.__getitem__, self.inputs)
+        self.specs = []
+        for sym in self.inputs:
+            tokens = sym.name.split("_")
+            self.specs.append((tokens[0], [int(v) for v in tokens[1:]]))
+        self.mesh = cp.mesh
+        self.diff = diff
+        self.dtype = dtype
+        self.prec = nat.PML_F64 if dtype == torch.float64 else nat.PML_F32
+        self.x = {}
+        for prefix, idx in self.specs:
+            if prefix == "x" and idx[0] not in self.x:
+                grid = np.ascontiguousarray(cp.mesh.vertex_coordinate_grids[idx[0]])
+                self.x[idx[0]] = torch.from_numpy(grid).to(device="cuda", dtype=dtype)
+
+    def _derivative(self, prefix, idx, y, dbc):
+        diff, mesh = self.diff, self.mesh
+        if prefix == "y-gradient":
+            i, axis = idx
+            return diff.gradient(y[..., i:i + 1].contiguous(), mesh, axis, dbc[:, i:i + 1])
+        if prefix == "y-hessian":
+            i, a1, a2 = idx
+            return diff.hessian(y[..., i:i + 1].contiguous(), mesh, a1, a2, dbc[:, i:i + 1])
+        if prefix == "y-laplacian":
+            i = idx[0]
+            return diff.laplacian(y[..., i:i + 1].contiguous(), mesh, dbc[:, i:i + 1])
+        if prefix == "y-divergence":
+            return diff.divergence(y[..., idx].contiguous(), mesh, dbc[:, idx])
+        if prefix == "y-curl":
+            if mesh.dimensions == 2:
+                return diff.curl(y[..., idx].contiguous(), mesh, 0, dbc[:, idx])
+            comps = idx[:-1]
+            return diff.curl(y[..., comps].contiguous(), mesh, idx[-1], dbc[:, comps])
+        if prefix == "y-vector-laplacian":
+            comps = idx[:-1]
+            return diff.vector_laplacian(y[..., comps].contiguous(), mesh, idx[-1],
+                                         dbc[:, comps])
+        raise ValueError(f"unknown symbol prefix {prefix}")
+
+    def __call__(self, y, t, dbc, out, cols) -> None:
+        """Writes right-hand side k into column cols[k] of the channel-last
+        tensor ``out`` (same vertices as ``y``)."""
+        import ctypes
+
+        import torch
+
+        y_dim = y.shape[-1]
+        esz = y.element_size()
+        ptrs, strides, keep = [], [], []
+        for prefix, idx in self.specs:
+            if prefix == "y":
+                ptrs.append(y.data_ptr() + idx[0] * esz)
+                strides.append(y_dim)
+            elif prefix == "x":
+                ptrs.append(self.x[idx[0]].data_ptr())
+                strides.append(1)
+            else:
+                arr = self._derivative(prefix, idx, y, dbc).contiguous()
+                keep.append(arr)
+                ptrs.append(arr.data_ptr())
+                strides.append(1)
+        n_out = out.shape[-1]
+        outs = [out.data_ptr() + int(c) * out.element_size() for c in cols]
+        n_in, n_o = len(ptrs), len(outs)
+        nat.check(nat.device_lib().pml_expr_eval(
+            self.prec, self.program.handle(), (ctypes.c_void_p * max(n_in, 1))(*ptrs),
+            (ctypes.c_int64 * max(n_in, 1))(*strides), (ctypes.c_void_p * n_o)(*outs),
+            (ctypes.c_int64 * n_o)(*([n_out] * n_o)), float(t), y.numel() // y_dim,
+            torch.cuda.current_stream().cuda_stream))
+        del keep
+
+
+def ode_time_step(t0: float, n_steps: int, d_t: float) -> float:
+    """The spacing of the reference's time grid ``linspace(t0, t0 + n d_t,
+    n + 1)`` (operator.py:60-74), as NumPy computes it: ``(stop - start) /
+    n``; step s starts at ``s * spacing + t0``."""
+    return ((t0 + n_steps * d_t) - t0) / n_steps
+
+
 class FDMOperator(Operator):
     """A finite difference method based differential equation solver."""
 
@@ -58,6 +167,7 @@ class FDMOperator(Operator):
         snapshot_every: int = 1,
         tile_config: Optional[int] = None,
         use_graph: bool = True,
+        symbolic_rhs: bool = False,
     ):
         super().__init__(d_t, True)
         dev.precision_id(precision)
@@ -69,6 +179,7 @@ class FDMOperator(Operator):
         self._snapshot_every = int(snapshot_every)
         self._tile_config = tile_config
         self._use_graph = use_graph
+        self._symbolic_rhs = bool(symbolic_rhs)
 
     @property
     def integrator(self) -> NumericalIntegrator:
@@ -123,20 +234,33 @@ class FDMOperator(Operator):
         torch.cuda.current_stream().wait_stream(stream)
         return best
 
-    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False):
+    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False,
+                         borders=None):
         """Device propagator over Parareal time slices (see
-        ``pararealml_amd.operators.parareal.propagators``)."""
-        from pararealml_amd.operators.parareal.propagators import FDMSlicePropagator
+        ``pararealml_amd.operators.parareal.propagators``); ``borders`` are
+        the slice boundary times (needed by systems whose right-hand side
+        depends on t)."""
+        from pararealml_amd.operators.parareal.propagators import (
+            FDMSlicePropagator,
+            ODESlicePropagator,
+        )
 
+        if not cp.differential_equation.x_dimension:
+            return ODESlicePropagator(self, cp, n_slices, slice_steps, stream,
+                                      record_trajectory, borders)
         return FDMSlicePropagator(self, cp, n_slices, slice_steps, stream, record_trajectory)
 
     def solve(self, ivp: InitialValueProblem, parallel_enabled: bool = True) -> Solution:
         cp = ivp.constrained_problem
-        if cp.mesh is not None and (
+        if not cp.differential_equation.x_dimension:
+            return self._solve_ode(ivp)
+        if (
             cp.mesh.coordinate_system_type != CoordinateSystem.CARTESIAN
             or cp.mesh.dimensions > 2
             or not cp.are_all_boundary_conditions_static
             or isinstance(self._integrator, ImplicitMethod)
+            or self._symbolic_rhs
+            or cp.differential_equation.device_parameters() is None
         ):
             return self._solve_composed(ivp)
         problem = self.device_problem(cp)
@@ -185,6 +309,69 @@ class FDMOperator(Operator):
             d_t=self._d_t * every,
         )
 
+    def _ode_precision(self) -> int:
+        if self._precision not in ("float64", "float32"):
+            raise NotImplementedError("ODE systems run in float64 or float32")
+        return nat.PML_F64 if self._precision == "float64" else nat.PML_F32
+
+    def _ode_program(self, diff_eq):
+        """The compiled right-hand side of an ODE system (cached per
+        equation object)."""
+        from pararealml_amd.operators.device_program import ode_program
+
+        key = id(diff_eq)
+        cache = self.__dict__.setdefault("_programs", {})
+        if key not in cache or cache[key][0] is not diff_eq:
+            cache[key] = (diff_eq, ode_program(diff_eq))
+        return cache[key][1]
+
+    def _solve_ode(self, ivp: InitialValueProblem) -> Solution:
+        """FDMOperator.solve on a system without a mesh (fdm_operator.py:
+        48-77; the constraint functions return None, 167-172): the compiled
+        right-hand side and the explicit integrator run the whole time loop
+        in one launch of the ODE kernel, writing every snapshot_every-th
+        state."""
+        import torch
+
+        if isinstance(self._integrator, ImplicitMethod):
+            raise NotImplementedError(
+                "ODE systems run the explicit integrators (ForwardEulerMethod, "
+                "ExplicitMidpointMethod, RK4) on the device"
+            )
+        integ = dev.integrator_id(self._integrator)
+        prec = self._ode_precision()
+        cp = ivp.constrained_problem
+        n = cp.differential_equation.y_dimension
+        t = discretize_time_domain(ivp.t_interval, self._d_t)
+        n_steps = len(t) - 1
+        every = self._snapshot_every
+        if n_steps % every:
+            raise ValueError(
+                f"snapshot_every ({every}) must divide the number of steps ({n_steps})"
+            )
+        n_snap = n_steps // every
+        if n_snap == 0:
+            return Solution(ivp, t[1:], np.empty((0, n)), vertex_oriented=True, d_t=self._d_t)
+        program = self._ode_program(cp.differential_equation)
+        dtype = torch.float64 if prec == nat.PML_F64 else torch.float32
+        y0 = np.asarray(ivp.initial_condition.discrete_y_0(True), dtype=np.float64)
+        stream = dev.solver_stream()
+        stream.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(stream):
+            y = torch.from_numpy(y0.reshape(1, n)).to(device="cuda", dtype=dtype)
+            snaps = torch.empty((n_snap, 1, n), dtype=dtype, device="cuda")
+            t0 = torch.tensor([t[0]], dtype=torch.float64, device="cuda")
+            step = torch.tensor([ode_time_step(t[0], n_steps, self._d_t)],
+                                dtype=torch.float64, device="cuda")
+            lib = nat.device_lib()
+            nat.check(lib.pml_ode_solve(prec, program.handle(), integ, 1, t0.data_ptr(),
+                                        step.data_ptr(), float(self._d_t), int(n_steps),
+                                        y.data_ptr(), snaps.data_ptr(), int(every),
+                                        stream.cuda_stream))
+            out = snaps.view(n_snap, n).to(torch.float64).cpu().numpy()
+        torch.cuda.current_stream().wait_stream(stream)
+        return Solution(ivp, t[every::every], out, vertex_oriented=True, d_t=self._d_t * every)
+
     def _solve_composed(self, ivp: InitialValueProblem) -> Solution:
         """Polar, cylindrical and spherical meshes, 3-D Cartesian ones and
         time-dependent boundary conditions (fdm_operator.py:48-165, 199-231
@@ -213,7 +400,9 @@ class FDMOperator(Operator):
             )
         params = eq.device_parameters()
         family = params[0] if params else None
-        if family not in ("diffusion", "wave", "shallow_water", "burgers",
+        if self._symbolic_rhs or family is None:
+            family = "symbolic"
+        if family not in ("diffusion", "wave", "shallow_water", "burgers", "symbolic",
                           "convection_diffusion", "cahn_hilliard", "navier_stokes"):
             raise NotImplementedError(
                 "curvilinear and 3-D meshes run the diffusion, convection-diffusion, "
@@ -297,8 +486,10 @@ class FDMOperator(Operator):
             nat.check(lib.pml_set_values(prec, y.data_ptr(), ycons.data_ptr(), count, sptr()))
             return y
 
-        coef_p = full_of(params[1][0])
-        coef_f = torch.full(shape[:-1] + (1,), float(params[1][0]), dtype=dtype, device="cuda")
+        if family != "symbolic":
+            coef_p = full_of(params[1][0])
+            coef_f = torch.full(shape[:-1] + (1,), float(params[1][0]), dtype=dtype,
+                                device="cuda")
 
         def comp(y, k):
             return y[..., k : k + 1].contiguous()
@@ -437,8 +628,52 @@ class FDMOperator(Operator):
             c_gamma = field_full(params[1][1])
             c_ones_f = field_full(1.0)
 
+        if family == "symbolic":
+            from pararealml_amd.differential_equation import LHS
+
+            system = eq.symbolic_equation_system
+            d_idx = list(system.equation_indices_by_type(LHS.D_Y_OVER_D_T))
+            y_idx = list(system.equation_indices_by_type(LHS.Y))
+            lap_idx = list(system.equation_indices_by_type(LHS.Y_LAPLACIAN))
+            sym_d = _SymbolicDeviceRHS(cp, diff, d_idx, dtype)
+            sym_y = _SymbolicDeviceRHS(cp, diff, y_idx, dtype) if y_idx else None
+            sym_l = _SymbolicDeviceRHS(cp, diff, lap_idx, dtype) if lap_idx else None
+
+        def rhs_symbolic(y, t_c, dbc):
+            """d y / d t: zero except at the D_Y_OVER_D_T components
+            (fdm_operator.py:114-120)."""
+            out = torch.zeros_like(y)
+            sym_d(y, t_c, dbc, out, d_idx)
+            return out
+
+        def with_symbolic_lhs(nxt, y_pre, t_pre, t_next):
+            """fdm_operator.py:127-161: LHS.Y components = C_{t_next}(their
+            right-hand sides at the pre-step state and time); LHS.Y_LAPLACIAN
+            components solve lap(y_k) = rhs_k (pre-step state) by the device
+            Jacobi iteration with the constraints at t_next, started from
+            the pre-step values."""
+            dbc_pre = constraints_at(t_pre)[1]
+            if sym_y is not None:
+                tmp = nxt.clone()
+                sym_y(y_pre, t_pre, dbc_pre, tmp, y_idx)
+                constrain(tmp, t_next)
+                nxt[..., y_idx] = tmp[..., y_idx]
+            if sym_l is not None:
+                lap = torch.empty(shape[:-1] + (len(lap_idx),), dtype=dtype, device="cuda")
+                sym_l(y_pre, t_pre, dbc_pre, lap, range(len(lap_idx)))
+                d_next = constraints_at(t_next)[1]
+                host_next = constraints_at(t_next)[2]
+                ycons = None if host_next is None else \
+                    np.asarray(host_next, dtype=object)[lap_idx]
+                nxt[..., lap_idx] = diff.anti_laplacian(
+                    lap, mesh, ycons, d_next[:, lap_idx],
+                    y_init=y_pre[..., lap_idx].contiguous())
+            return nxt
+
         def rhs(y, t_c):
             dbc = constraints_at(t_c)[1]
+            if family == "symbolic":
+                return rhs_symbolic(y, t_c, dbc)
             if family == "shallow_water":
                 return rhs_shallow_water(y, dbc)
             if family == "burgers":
@@ -548,6 +783,8 @@ class FDMOperator(Operator):
                     nxt = with_mu(nxt, y, t_i, t_n)
                 if family == "navier_stokes":
                     nxt = with_psi_uv(nxt, y, t_i, t_n)
+                if family == "symbolic":
+                    nxt = with_symbolic_lhs(nxt, y, t_i, t_n)
                 y = nxt
                 cache.clear()
                 if (step + 1) % every == 0:
@@ -586,6 +823,8 @@ class FDMOperator(Operator):
                 nxt = with_mu(nxt, y, t_i, t_n)
             if family == "navier_stokes":
                 nxt = with_psi_uv(nxt, y, t_i, t_n)
+            if family == "symbolic":
+                nxt = with_symbolic_lhs(nxt, y, t_i, t_n)
             y = constrain(nxt, t_n)
             cache.clear()
             if (step + 1) % every == 0:

@@ -62,7 +62,8 @@ class LearnedCoarseOperator:
     def vertex_oriented(self) -> Optional[bool]:
         return True
 
-    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False):
+    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False,
+                         borders=None):
         fb = None
         if self.fallback is not None:
             fb = self.fallback.slice_propagator(
@@ -189,7 +190,8 @@ class CorrectedCoarseOperator:
     def vertex_oriented(self) -> Optional[bool]:
         return True
 
-    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False):
+    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False,
+                         borders=None):
         fb = None
         if self.fallback is not None:
             fb = self.fallback.slice_propagator(

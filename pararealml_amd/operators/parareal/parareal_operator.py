@@ -31,6 +31,7 @@ border points ``u`` (device tensor, (N + 1, elems), field-major), its time is
 excluded from the timings, and a True return ends the iterations.
 """
 
+import ctypes
 import sys
 import time
 from typing import Callable, Optional, Sequence, Union
@@ -126,15 +127,17 @@ class PararealOperator(Operator):
         timings = {"fine_s": 0.0, "coarse_s": 0.0, "correction_s": 0.0}
         with torch.cuda.stream(stream):
             fine = f.slice_propagator(
-                cp, n, f_steps, stream, record_trajectory=self._output == "trajectory"
+                cp, n, f_steps, stream, record_trajectory=self._output == "trajectory",
+                borders=borders,
             )
-            coarse = g.slice_propagator(cp, n, g_steps, stream)
+            coarse = g.slice_propagator(cp, n, g_steps, stream, borders=borders)
             problem = fine.problem
             elems = problem.elems
             plane = problem.plane
             n_fields = problem.n_fields
-            lib = fine.batch.lib
-            sp = fine._sptr()
+            lib = nat.device_lib()
+            sp = ctypes.c_void_p(stream.cuda_stream)
+            spatial = tuple(cp.y_vertices_shape[:-1])
             f64 = dict(dtype=torch.float64, device="cuda")
 
             y0 = ivp.initial_condition.discrete_y_0(True)
@@ -199,7 +202,7 @@ class PararealOperator(Operator):
                 if callable(tc):
                     layout = lambda t: problem.from_device_layout(  # noqa: E731
                         t.cpu().numpy().reshape(n, n_fields, problem.n0, problem.n1),
-                        cp.mesh.vertices_shape,
+                        spatial,
                     )
                     if tc(layout(old), layout(u[1:])):
                         break
@@ -215,7 +218,7 @@ class PararealOperator(Operator):
                 shifted = (f_end + (u[1:] - f_end)).cpu().numpy()
                 y = problem.from_device_layout(
                     shifted.reshape(n, n_fields, problem.n0, problem.n1),
-                    cp.mesh.vertices_shape,
+                    spatial,
                 )
                 t = borders[1:]
                 d_t_out = delta
@@ -225,7 +228,7 @@ class PararealOperator(Operator):
                 traj = (traj + shift).permute(1, 0, 2).reshape(n * f_steps, elems)
                 y = problem.from_device_layout(
                     traj.cpu().numpy().reshape(n * f_steps, n_fields, problem.n0, problem.n1),
-                    cp.mesh.vertices_shape,
+                    spatial,
                 )
                 t = discretize_time_domain(ivp.t_interval, f.d_t)[1:]
                 d_t_out = f.d_t

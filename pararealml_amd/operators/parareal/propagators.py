@@ -15,6 +15,11 @@ slices at once:
   on the coarse points, so a chain of per-slice coarse solves equals one
   continuous coarse solve, as the reference's initial coarse sweep assumes.
 
+* ``ODESlicePropagator`` -- an ``FDMOperator`` on a system without a mesh:
+  all requested slices advance together in one launch of the ODE kernel
+  (``pml_ode_solve``, one thread per slice, the whole slice in the launch),
+  each from its own start time.
+
 Every slice's initial value has the problem's static Dirichlet values
 imposed before its first step, as ``DiscreteInitialCondition`` does for
 each sub-problem of the reference's Parareal (parareal_operator.py:153-163).
@@ -91,6 +96,96 @@ class FDMSlicePropagator:
         return flat.view(self.steps, self.n_slices, self.elems)
 
 
+class ODEProblem:
+    """The device-layout description of an ODE state (the interface
+    ``PararealOperator`` uses for grids): ``n`` fields of one point."""
+
+    def __init__(self, n: int):
+        self.n_fields = n
+        self.n0 = self.n1 = 1
+
+    @property
+    def plane(self) -> int:
+        return 1
+
+    @property
+    def elems(self) -> int:
+        return self.n_fields
+
+    def to_device_layout(self, y) -> np.ndarray:
+        return np.asarray(y, dtype=np.float64).reshape(self.n_fields, 1, 1)
+
+    def from_device_layout(self, y, spatial_shape) -> np.ndarray:
+        lead = y.shape[:-3]
+        return np.ascontiguousarray(y.reshape(lead + (self.n_fields,)))
+
+
+class ODESlicePropagator:
+    """Slices of an ODE system, all advanced by one kernel launch.  Slice j
+    starts at ``borders[j]`` and takes ``slice_steps`` steps on the
+    reference's time grid of the sub-problem, linspace(T_j, T_j + n d_t,
+    n + 1) (the time of sub-problem solves in parareal_operator.py:
+    153-163)."""
+
+    def __init__(self, op, cp, n_slices: int, slice_steps: int, stream,
+                 record_trajectory: bool = False, borders=None):
+        import torch
+
+        from pararealml_amd.operators.fdm.fdm_operator import ode_time_step
+
+        if borders is None or len(borders) != n_slices + 1:
+            raise ValueError("an ODE slice propagator needs the n_slices + 1 slice borders")
+        self.op = op
+        self.prec = op._ode_precision()
+        self.integ = dev.integrator_id(op.integrator)
+        self.program = op._ode_program(cp.differential_equation)
+        n = cp.differential_equation.y_dimension
+        self.problem = ODEProblem(n)
+        self.elems = n
+        self.n_slices = n_slices
+        self.steps = slice_steps
+        self.stream = stream
+        self.dtype = torch.float64 if self.prec == nat.PML_F64 else torch.float32
+        f64 = dict(dtype=torch.float64, device="cuda")
+        self.t0 = torch.tensor([float(b) for b in borders[:-1]], **f64)
+        self.t_step = torch.tensor(
+            [ode_time_step(float(b), slice_steps, op.d_t) for b in borders[:-1]], **f64)
+        self.y = torch.empty((n_slices, n), dtype=self.dtype, device="cuda")
+        self.snaps = None
+        if record_trajectory:
+            self.snaps = torch.empty((slice_steps, n_slices, n), dtype=self.dtype, device="cuda")
+
+    def run(self, u_f64, first: int, last: int, out_f64):
+        """Advances slices [first, last) from ``u_f64`` (FP64, (count, n))
+        into ``out_f64``."""
+        import torch
+
+        count = last - first
+        if count <= 0:
+            return
+        n = self.elems
+        y = self.y[first:last]
+        y.copy_(u_f64.reshape(count, n))
+        tmp = None
+        if self.snaps is not None:
+            tmp = torch.empty((self.steps, count, n), dtype=self.dtype, device="cuda")
+        lib = nat.device_lib()
+        nat.check(lib.pml_ode_solve(
+            self.prec, self.program.handle(), self.integ, count,
+            self.t0.data_ptr() + 8 * first, self.t_step.data_ptr() + 8 * first,
+            float(self.op.d_t), int(self.steps), y.data_ptr(),
+            None if tmp is None else tmp.data_ptr(), 1, self.stream.cuda_stream))
+        if tmp is not None:
+            self.snaps[:, first:last].copy_(tmp)
+        out_f64.reshape(count, n).copy_(y)
+
+    def trajectory_f64(self):
+        """Recorded states, FP64, shape (steps, n_slices, n)."""
+        if self.snaps is None:
+            raise RuntimeError("trajectory was not recorded")
+        return self.snaps.double()
+
+
 class CoarseGridPropagator:
     """An FDM operator on the 2x-coarsened vertex grid of the problem."""
 
@@ -148,5 +243,6 @@ class CoarseGridFDMOperator:
     def vertex_oriented(self) -> Optional[bool]:
         return True
 
-    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False):
+    def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False,
+                         borders=None):
         return CoarseGridPropagator(self.fdm, cp, n_slices, slice_steps, stream)

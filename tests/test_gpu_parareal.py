@@ -182,3 +182,33 @@ def test_parareal_validation_errors():
     f = FDMOperator(RK4(), DIFF, 1e-4)
     with pytest.raises(ValueError):
         PararealOperator(f, g, [1e-6, 1e-6], n_slices=2).solve(ivp)
+
+
+def test_same_inputs_replay_bit_identically():
+    """Deterministic replay: the Parareal update norms and the Jacobi
+    iteration's norms are summed in a fixed order, so the same inputs give
+    bit-identical norms, iteration counts, sweep counts and results."""
+    print("case: deterministic replay", flush=True)
+    cp, ic = diffusion_problem(65)
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.04), ic)
+    runs = []
+    for _ in range(2):
+        f = FDMOperator(RK4(), DIFF, 1e-4, precision="float32")
+        g = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), DIFF, 5e-4))
+        p = PararealOperator(f, g, 1e-7, n_slices=8)
+        sol = p.solve(ivp)
+        runs.append((p.last_iterations, p.last_stats["update_norms"], sol.discrete_y()))
+    assert runs[0][0] == runs[1][0]
+    assert runs[0][1] == runs[1][1]  # lists of floats, compared exactly
+    np.testing.assert_array_equal(runs[0][2], runs[1][2])
+
+    lap = np.random.RandomState(5).standard_normal(cp.y_vertices_shape)
+    y_cons = cp.static_y_vertex_constraints
+    outs = []
+    for _ in range(2):
+        np.random.seed(11)  # the reference's random start (numerical_differentiator.py:908)
+        diff = ThreePointCentralDifferenceMethod(tol=1e-9)
+        res = diff.anti_laplacian(lap, cp.mesh, y_cons)
+        outs.append((diff.last_jacobi_sweeps, res))
+    assert outs[0][0] == outs[1][0]
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
