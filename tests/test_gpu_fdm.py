@@ -397,10 +397,89 @@ def test_ragged_batch_matches_per_grid():
 
 def test_unsupported_cases_raise():
     from pararealml_amd.constrained_problem import ConstrainedProblem
+    from pararealml_amd.operators.fdm import BackwardEulerMethod
 
-    # ODEs and equations without fused kernels are refused, not run on the host
+    # what the device path does not run is refused, never run on the host:
+    # ODEs with an implicit integrator or in compensated FP32
     cp = ConstrainedProblem(pml.LorenzEquation())
     ic = pml.ContinuousInitialCondition(cp, lambda _: np.ones(3))
     ivp = pml.InitialValueProblem(cp, (0.0, 1.0), ic)
     with pytest.raises(NotImplementedError):
-        FDMOperator(RK4(), ThreePointCentralDifferenceMethod(), 0.1).solve(ivp)
+        FDMOperator(BackwardEulerMethod(), ThreePointCentralDifferenceMethod(), 0.1).solve(ivp)
+    with pytest.raises(NotImplementedError):
+        FDMOperator(RK4(), ThreePointCentralDifferenceMethod(), 0.1,
+                    precision="float32c").solve(ivp)
+
+
+def test_c2_full_ragged_batch_fp32_against_fp64():
+    """The batched Cahn-Hilliard workload at full size: 256 grids drawn from
+    {64, 96, 128, 192, 256}^2, RK4, 2000 steps of 1e-4, the tuned FP32
+    streaming kernel (two steps per launch, graph replays) against the FP64
+    kernel (itself pinned to the oracle per grid): per grid, max relative
+    error <= 2e-5, and the trapezoid mass of c (conserved exactly by the
+    zero-Neumann scheme) drifts by less than 1e-5 of the field's scale."""
+    import bench
+    from pararealml_amd import _device as dev
+    from pararealml_amd import _native as nat
+
+    print("case: c2 full ragged batch fp32 vs fp64, 2000 steps", flush=True)
+    sizes, cps, c0 = bench.ch_workload()
+    y0 = bench.mu0_on_device(cps, sizes, c0)
+    size_list = sorted(cps)
+    problems = [dev.device_problem(cps[n]) for n in size_list]
+    grid_problem = [size_list.index(n) for n in sizes]
+    out = {}
+    for prec in ("float32", "float64"):
+        b = dev.FDMBatch(problems, grid_problem, prec, nat.PML_INT_RK4, dev.solver_stream())
+        b.load_host(y0)
+        names = [b.lib.pml_fdm_config_name(b.eq_id, b.integrator, b.prec, i).decode()
+                 for i in range(b.lib.pml_fdm_configs(b.eq_id, b.integrator, b.prec))]
+        cfg = names.index("stream2-rw8-pd2-s2") if prec == "float32" else 0
+        b.advance(1e-4, 2000, tile_config=cfg, use_graph=True)
+        out[prec] = (b, b.state_f64().cpu().numpy())
+    worst = 0.0
+    for k, n in enumerate(sizes):
+        a = out["float32"][0].grid_view(out["float32"][1], k)
+        r = out["float64"][0].grid_view(out["float64"][1], k)
+        worst = max(worst, float(np.max(np.abs(a - r)) / np.max(np.abs(r))))
+        w = np.ones(n)
+        w[0] = w[-1] = 0.5
+        w2 = np.outer(w, w)
+        m0 = float(np.sum(w2 * c0[k][..., 0]))
+        m1 = float(np.sum(w2 * a[0]))
+        assert abs(m1 - m0) <= 1e-5 * 0.05 * n * n, (k, m0, m1)
+    assert worst <= 2e-5, worst
+
+
+def test_shallow_water_fp32_against_fp64():
+    """Shallow water (h=1, v=0.01, g=9.81, reflective edges), four 257^2
+    grids with seeded bumps, 200 RK4 steps of 1e-3 in FP32 against FP64 on
+    all three fields: per field, max error <= 2e-5 of the field's largest
+    magnitude (the velocity fields start at zero, so the scale is the
+    field's own)."""
+    from pararealml_amd import _device as dev
+    from pararealml_amd import _native as nat
+
+    print("case: shallow water fp32 vs fp64", flush=True)
+    eq = pml.ShallowWaterEquation(h=1.0, v=0.01, g=9.81)
+    mesh = pml.Mesh([(-5.0, 5.0), (-5.0, 5.0)], [10.0 / 256] * 2)
+    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    rng = np.random.RandomState(1462316)
+    y0s = []
+    for _ in range(4):
+        mean = rng.uniform(-2.5, 2.5, 2)
+        ic = pml.GaussianInitialCondition(cp, [(mean, 0.25 * np.eye(2))] * 3, [0.1, 0.0, 0.0])
+        y0s.append(ic.discrete_y_0(True))
+    problem = dev.device_problem(cp)
+    res = {}
+    for prec in ("float32", "float64"):
+        b = dev.FDMBatch([problem], [0] * 4, prec, nat.PML_INT_RK4, dev.solver_stream())
+        b.load_host(y0s)
+        b.advance(1e-3, 200, use_graph=True)
+        res[prec] = b.state_f64().cpu().numpy().reshape(4, 3, 257, 257)
+    for q in range(3):
+        scale = float(np.max(np.abs(res["float64"][:, q])))
+        assert scale > 0
+        err = float(np.max(np.abs(res["float32"][:, q] - res["float64"][:, q]))) / scale
+        assert err <= 2e-5, (q, err)
