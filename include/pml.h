@@ -174,6 +174,29 @@ int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
 
 int pml_graph_cache_clear(void);
 
+/*
+ * pml_fdm_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under hipGraph
+ * capture, serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as pml_fdm_solve; the work-unit and grid tables
+ * go to a file-scope device array of PML_FDM_SCRATCH_BYTES (one per GPU,
+ * held until the process ends) instead of cached library allocations, no
+ * step-loop graph is used (PML_FLAG_GRAPH is ignored; PML_FLAG_PREPARE is
+ * an error), and a batch whose tables exceed the array is refused with
+ * PML_E_INVALID.  The array is rewritten by every call: calls that overlap
+ * on one GPU (different streams or threads) race.
+ */
+#define PML_FDM_SCRATCH_BYTES (4 << 20)
+int pml_fdm_solve_noalloc(int eq_id, const double *eq_params, int n_params,
+                          int integrator_id, int precision, int ndim,
+                          const pml_grid *grids, int n_grids,
+                          const double *bc_values, void *state_hi,
+                          void *state_lo, void *work_hi, void *work_lo,
+                          int64_t total_elems, double dt, int64_t n_steps,
+                          void *snapshots_hi, void *snapshots_lo,
+                          int64_t snapshot_every, int tile_config, int flags,
+                          void *stream);
+
 /* Number of kernel variants for (eq, integrator, precision) and a short
  * description of variant i (for the autotuner). */
 int pml_fdm_configs(int eq_id, int integrator_id, int precision);

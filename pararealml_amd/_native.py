@@ -31,6 +31,7 @@ PML_EQ = {
 }
 
 PML_REDUCE_SLOTS = 2048
+PML_FDM_SCRATCH_BYTES = 4 << 20
 PML_JACOBI_STATE_BYTES = 32 + 8 * PML_REDUCE_SLOTS
 
 PML_FLAG_GRAPH = 1
@@ -72,6 +73,16 @@ SIGNATURES = {
         ],
     ),
     "pml_graph_cache_clear": (c_int, []),
+    "pml_fdm_solve_noalloc": (
+        c_int,
+        [
+            c_int, POINTER(c_double), c_int, c_int, c_int, c_int,
+            POINTER(PmlGrid), c_int, c_void_p,
+            c_void_p, c_void_p, c_void_p, c_void_p,
+            c_int64, c_double, c_int64,
+            c_void_p, c_void_p, c_int64, c_int, c_int, c_void_p,
+        ],
+    ),
     "pml_program_create": (
         c_int, [c_void_p, c_int, c_void_p, c_int, c_int, c_int, c_int, POINTER(c_void_p)]
     ),

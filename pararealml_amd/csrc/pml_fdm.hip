@@ -183,13 +183,90 @@ extern "C" const char *pml_fdm_config_name(int eq_id, int integrator_id, int pre
   return (*v)[config].name;
 }
 
-extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
-                             int integrator_id, int precision, int ndim, const pml_grid *grids,
-                             int n_grids, const double *bc_values, void *state_hi,
-                             void *state_lo, void *work_hi, void *work_lo, int64_t total_elems,
-                             double dt, int64_t n_steps, void *snapshots_hi,
-                             void *snapshots_lo, int64_t snapshot_every, int tile_config,
-                             int flags, void *stream) {
+// The work-unit and grid tables of the allocation-free entry point live in
+// this file-scope device array (one copy per GPU, held for the life of the
+// process).  Host code never passes the array's name to a kernel: it uses
+// the device address from hipGetSymbolAddress, looked up once per device.
+static __device__ __align__(256) unsigned char scratch[PML_FDM_SCRATCH_BYTES];
+
+namespace pml {
+
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[64] = {};
+
+// The device address of `scratch` on the current device (looked up on the
+// first call made with that device current).
+static int scratch_address(void **out) {
+  int dev = 0;
+  PML_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return fail(PML_E_INVALID, "device index beyond 63");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    PML_HIP_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(PML_E_HIP, "hipGetSymbolAddress returned no address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PML_OK;
+}
+
+// Builds the tables on the host (reusing this thread's buffer) and copies
+// them into `scratch` on the stream: grid table first, then the units at
+// the next 256-byte boundary.  Nothing is allocated on the device.
+static int scratch_tables(const pml_grid *grids, int n_grids, const Variant &V,
+                          hipStream_t st, TableSet *out) {
+  thread_local std::vector<unsigned char> host;
+  const size_t grid_bytes = sizeof(GridDev) * static_cast<size_t>(n_grids);
+  const size_t unit_off = (grid_bytes + 255) / 256 * 256;
+  size_t n_units = 0;
+  for (int g = 0; g < n_grids; ++g)
+    n_units += static_cast<size_t>((grids[g].n0 + V.oy - 1) / V.oy) *
+               static_cast<size_t>((grids[g].n1 + V.ox - 1) / V.ox);
+  const size_t need = unit_off + sizeof(TileDev) * n_units;
+  if (need > PML_FDM_SCRATCH_BYTES)
+    return fail(PML_E_INVALID, "the batch needs " + std::to_string(need) +
+                                   " bytes of work-unit tables; pml_fdm_solve_noalloc holds at "
+                                   "most " + std::to_string(PML_FDM_SCRATCH_BYTES));
+  void *base = nullptr;
+  if (int rc = scratch_address(&base); rc != PML_OK) return rc;
+  if (host.size() < need) host.resize(need);
+  auto *gd = reinterpret_cast<GridDev *>(host.data());
+  auto *units = reinterpret_cast<TileDev *>(host.data() + unit_off);
+  size_t u = 0;
+  for (int g = 0; g < n_grids; ++g) {
+    const pml_grid &s = grids[g];
+    GridDev &d = gd[g];
+    d.n0 = s.n0;
+    d.n1 = s.n1;
+    d.off = s.state_offset;
+    d.bc = s.bc_offset;
+    d.inv_dx0sq = 1.0 / (s.dx0 * s.dx0);
+    d.inv_dx1sq = 1.0 / (s.dx1 * s.dx1);
+    d.inv_2dx0 = 1.0 / (2.0 * s.dx0);
+    d.inv_2dx1 = 1.0 / (2.0 * s.dx1);
+    d.m2dx0 = -2.0 * s.dx0;
+    d.m2dx1 = -2.0 * s.dx1;
+    d.p2dx0 = 2.0 * s.dx0;
+    d.p2dx1 = 2.0 * s.dx1;
+    for (int i0 = 0; i0 < s.n0; i0 += V.oy)
+      for (int j0 = 0; j0 < s.n1; j0 += V.ox) units[u++] = TileDev{g, i0, j0, 0};
+  }
+  // a pageable-memory copy: the host buffer may be reused once it returns
+  PML_HIP_CHECK(hipMemcpyAsync(base, host.data(), need, hipMemcpyHostToDevice, st));
+  out->grids = base;
+  out->units = static_cast<unsigned char *>(base) + unit_off;
+  out->n_units = static_cast<int>(n_units);
+  return PML_OK;
+}
+
+static int fdm_solve_impl(int eq_id, const double *eq_params, int n_params,
+                          int integrator_id, int precision, int ndim, const pml_grid *grids,
+                          int n_grids, const double *bc_values, void *state_hi,
+                          void *state_lo, void *work_hi, void *work_lo, int64_t total_elems,
+                          double dt, int64_t n_steps, void *snapshots_hi,
+                          void *snapshots_lo, int64_t snapshot_every, int tile_config,
+                          int flags, void *stream, bool no_alloc) {
   int n_fields = 0;
   if (ndim != 1 && ndim != 2) return fail(PML_E_INVALID, "ndim must be 1 or 2");
   const auto *vars = find_variants(eq_id, integrator_id, precision, ndim, &n_fields);
@@ -247,7 +324,10 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
   }
   hipStream_t st = as_stream(stream);
   TableSet tab;
-  int rc = device_tables(grids, n_grids, V, ndim, &tab);
+  if (no_alloc && (flags & PML_FLAG_PREPARE))
+    return fail(PML_E_INVALID, "the allocation-free entry point has nothing to prepare");
+  int rc = no_alloc ? scratch_tables(grids, n_grids, V, st, &tab)
+                    : device_tables(grids, n_grids, V, ndim, &tab);
   if (rc) return rc;
 
   FdmArgs a{};
@@ -288,7 +368,8 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
     return PML_OK;
   };
 
-  const bool graphed = use_graph && st != nullptr && !snapshots_hi;
+  // (no graphs without allocation: instantiating one allocates)
+  const bool graphed = !no_alloc && use_graph && st != nullptr && !snapshots_hi;
   KernelFn main_fn = can_fuse ? V.fn : V.fn_single;
   const int64_t fused_launches = n_steps / steps_per_launch;
   if (prepare && !(graphed && fused_launches >= 2)) return PML_OK;  // tables only
@@ -426,4 +507,33 @@ extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
           hipMemcpyAsync(state_lo, cur_lo, total_elems * esz, hipMemcpyDeviceToDevice, st));
   }
   return PML_OK;
+}
+
+}  // namespace pml
+
+extern "C" int pml_fdm_solve(int eq_id, const double *eq_params, int n_params,
+                             int integrator_id, int precision, int ndim, const pml_grid *grids,
+                             int n_grids, const double *bc_values, void *state_hi,
+                             void *state_lo, void *work_hi, void *work_lo, int64_t total_elems,
+                             double dt, int64_t n_steps, void *snapshots_hi,
+                             void *snapshots_lo, int64_t snapshot_every, int tile_config,
+                             int flags, void *stream) {
+  return fdm_solve_impl(eq_id, eq_params, n_params, integrator_id, precision, ndim, grids,
+                        n_grids, bc_values, state_hi, state_lo, work_hi, work_lo, total_elems,
+                        dt, n_steps, snapshots_hi, snapshots_lo, snapshot_every, tile_config,
+                        flags, stream, false);
+}
+
+extern "C" int pml_fdm_solve_noalloc(int eq_id, const double *eq_params, int n_params,
+                                     int integrator_id, int precision, int ndim,
+                                     const pml_grid *grids, int n_grids,
+                                     const double *bc_values, void *state_hi, void *state_lo,
+                                     void *work_hi, void *work_lo, int64_t total_elems,
+                                     double dt, int64_t n_steps, void *snapshots_hi,
+                                     void *snapshots_lo, int64_t snapshot_every,
+                                     int tile_config, int flags, void *stream) {
+  return fdm_solve_impl(eq_id, eq_params, n_params, integrator_id, precision, ndim, grids,
+                        n_grids, bc_values, state_hi, state_lo, work_hi, work_lo, total_elems,
+                        dt, n_steps, snapshots_hi, snapshots_lo, snapshot_every, tile_config,
+                        flags, stream, true);
 }
