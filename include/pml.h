@@ -352,6 +352,14 @@ int pml_restrict2x(int precision, const void *fine, void *coarse,
                    int64_t batch, int64_t m0, int64_t m1, void *stream);
 int pml_prolong2x(int precision, const void *coarse, void *fine,
                   int64_t batch, int64_t m0, int64_t m1, void *stream);
+/* Cubic 2x prolongation (at least 4 coarse points per axis): even fine
+ * points copy the coarse ones, odd ones take the 4-point Lagrange midpoint
+ * value ((-c[k-1] + 9 c[k]) + 9 c[k+1] - c[k+2]) / 16, one-sided
+ * (((5 c[0] + 15 c[1]) - 5 c[2]) + c[3]) / 16 at the two ends; axis 0
+ * first, then axis 1.  Exact on cubics in each axis.  No counterpart in
+ * the reference. */
+int pml_prolong2x_cubic(int precision, const void *coarse, void *fine,
+                        int64_t batch, int64_t m0, int64_t m1, void *stream);
 
 /*
  * Symbolic right-hand sides on the device.  The host compiles a system's

@@ -39,9 +39,11 @@ Pinning of each function (independent evidence, beyond this restatement):
   serial fine solution, G = F stops after one iteration with the serial
   solution, and the iteration count is the first whose update is below the
   tolerance -- tests/test_oracle_pins.py.
-* ``restrict_injection`` / ``prolong_bilinear``: these grid transfers have no
-  counterpart in the reference; they are pinned by exactness on bilinear
-  fields and restriction(prolongation(u)) == u -- tests/test_oracle_pins.py.
+* ``restrict_injection`` / ``prolong_bilinear`` / ``prolong_cubic``: these
+  grid transfers have no counterpart in the reference; they are pinned by
+  exactness on bilinear (cubic: tensor-product cubic) fields, restriction
+  after prolongation = identity, and the 4-point midpoint weights --
+  tests/test_oracle_pins.py.
 """
 
 from typing import Callable, List, Optional, Sequence, Tuple
@@ -491,6 +493,32 @@ def prolong_bilinear(u: np.ndarray) -> np.ndarray:
     out[::2, 1::2] = 0.5 * (u[:, :-1] + u[:, 1:])
     out[1::2, 1::2] = 0.25 * (u[:-1, :-1] + u[1:, :-1] + u[:-1, 1:] + u[1:, 1:])
     return out
+
+
+def _cubic_midpoints_axis(c: np.ndarray, axis: int) -> np.ndarray:
+    """2x refinement along one axis: even points copy, odd points take the
+    4-point Lagrange midpoint ((-c[k-1] + 9 c[k]) + 9 c[k+1] - c[k+2]) / 16,
+    one-sided (((5 c[0] + 15 c[1]) - 5 c[2]) + c[3]) / 16 at the ends."""
+    c = np.moveaxis(c, axis, 0)
+    m = c.shape[0]
+    if m < 4:
+        raise ValueError("cubic prolongation needs at least 4 coarse points per axis")
+    mid = np.empty((m - 1,) + c.shape[1:], dtype=c.dtype)
+    mid[1 : m - 2] = (((-c[0 : m - 3] + 9.0 * c[1 : m - 2]) + 9.0 * c[2 : m - 1])
+                      - c[3:m]) / 16.0
+    mid[0] = (((5.0 * c[0] + 15.0 * c[1]) - 5.0 * c[2]) + c[3]) / 16.0
+    mid[m - 2] = (((5.0 * c[m - 1] + 15.0 * c[m - 2]) - 5.0 * c[m - 3]) + c[m - 4]) / 16.0
+    out = np.empty((2 * m - 1,) + c.shape[1:], dtype=c.dtype)
+    out[::2] = c
+    out[1::2] = mid
+    return np.moveaxis(out, 0, axis)
+
+
+def prolong_cubic(u: np.ndarray) -> np.ndarray:
+    """Coarse (m points per axis, m >= 4) -> fine (2m-1 points) by the
+    tensor-product 4-point cubic interpolation, axis 0 then axis 1.  New
+    capability (no reference counterpart)."""
+    return _cubic_midpoints_axis(_cubic_midpoints_axis(u, 0), 1)
 
 
 # ------------------------------------------------------------------- Parareal

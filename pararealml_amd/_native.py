@@ -154,6 +154,9 @@ SIGNATURES = {
     "pml_prolong2x": (
         c_int, [c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p]
     ),
+    "pml_prolong2x_cubic": (
+        c_int, [c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p]
+    ),
     "pml_apply_dirichlet": (
         c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int64, c_void_p]
     ),

@@ -155,6 +155,67 @@ __global__ void k_prolong(const T *coarse, T *fine, int64_t batch, int64_t m0, i
   }
 }
 
+// Cubic 2x prolongation: fine point 2k along an axis copies coarse point
+// k; fine point 2k + 1 is the 4-point Lagrange midpoint value
+// ((-c[k-1] + 9 c[k]) + 9 c[k+1] - c[k+2]) / 16, one-sided at the two ends
+// (((5 c[0] + 15 c[1]) - 5 c[2]) + c[3]) / 16 and its mirror image, exact
+// for cubics; axis 0 first, then axis 1 (a tensor-product stencil of up to
+// 4 x 4 coarse points).  Needs at least 4 coarse points per axis.
+template <typename T>
+__device__ __forceinline__ int cubic_taps(int i, int m, int *idx, T *w) {
+  if (!(i & 1)) {
+    idx[0] = i >> 1;
+    w[0] = T(1);
+    return 1;
+  }
+  const int k = i >> 1;  // midpoint between coarse k and k + 1
+  if (k == 0) {
+    idx[0] = 0; idx[1] = 1; idx[2] = 2; idx[3] = 3;
+    w[0] = T(5); w[1] = T(15); w[2] = T(-5); w[3] = T(1);
+  } else if (k + 1 == m - 1) {
+    idx[0] = m - 1; idx[1] = m - 2; idx[2] = m - 3; idx[3] = m - 4;
+    w[0] = T(5); w[1] = T(15); w[2] = T(-5); w[3] = T(1);
+  } else {
+    idx[0] = k - 1; idx[1] = k; idx[2] = k + 1; idx[3] = k + 2;
+    w[0] = T(-1); w[1] = T(9); w[2] = T(9); w[3] = T(-1);
+  }
+  return 4;
+}
+
+template <typename T>
+__device__ __forceinline__ T cubic_combine(int nt, const T *w, const T *v) {
+  if (nt == 1) return v[0];
+  // left to right, as the written formulas: ((w0 v0 + w1 v1) + w2 v2) + w3 v3
+  T acc = w[0] * v[0] + w[1] * v[1];
+  acc = acc + w[2] * v[2];
+  acc = acc + w[3] * v[3];
+  return acc / T(16);
+}
+
+template <typename T>
+__global__ void k_prolong_cubic(const T *coarse, T *fine, int64_t batch, int64_t m0,
+                                int64_t m1) {
+  const int64_t f0 = 2 * m0 - 1, f1 = 2 * m1 - 1;
+  const int64_t total = batch * f0 * f1;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / (f0 * f1), r = t - b * f0 * f1;
+    const int i = static_cast<int>(r / f1), j = static_cast<int>(r - (r / f1) * f1);
+    const T *c = coarse + b * m0 * m1;
+    int ri[4], cj[4];
+    T wi[4], wj[4];
+    const int ni = cubic_taps<T>(i, static_cast<int>(m0), ri, wi);
+    const int nj = cubic_taps<T>(j, static_cast<int>(m1), cj, wj);
+    T col[4];
+    for (int q = 0; q < nj; ++q) {
+      T v[4];
+      for (int p = 0; p < ni; ++p) v[p] = c[static_cast<int64_t>(ri[p]) * m1 + cj[q]];
+      col[q] = cubic_combine<T>(ni, wi, v);
+    }
+    fine[t] = cubic_combine<T>(nj, wj, col);
+  }
+}
+
 // Dirichlet values of a 2-D grid's edge arrays (the pml_grid boundary block
 // layout) imposed on FP64 states of `batch` copies of the grid.
 __global__ void k_apply_dirichlet(double *u, const double *bc, int n_fields, int n0, int n1,
@@ -340,6 +401,25 @@ extern "C" int pml_prolong2x(int precision, const void *coarse, void *fine, int6
   else if (precision == PML_F32)
     hipLaunchKernelGGL(k_prolong<float>, dim3(nb), dim3(256), 0, st, (const float *)coarse,
                        (float *)fine, batch, m0, m1);
+  else
+    return fail(PML_E_INVALID, "prolongation takes FP32 or FP64 planes");
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_prolong2x_cubic(int precision, const void *coarse, void *fine,
+                                   int64_t batch, int64_t m0, int64_t m1, void *stream) {
+  if (batch <= 0 || m0 < 4 || m1 < 4 || !fine || !coarse)
+    return fail(PML_E_INVALID, "bad cubic prolongation arguments (at least 4 coarse points "
+                               "per axis)");
+  hipStream_t st = as_stream(stream);
+  const int nb = blocks_for(batch * (2 * m0 - 1) * (2 * m1 - 1), 256);
+  if (precision == PML_F64)
+    hipLaunchKernelGGL(k_prolong_cubic<double>, dim3(nb), dim3(256), 0, st,
+                       (const double *)coarse, (double *)fine, batch, m0, m1);
+  else if (precision == PML_F32)
+    hipLaunchKernelGGL(k_prolong_cubic<float>, dim3(nb), dim3(256), 0, st,
+                       (const float *)coarse, (float *)fine, batch, m0, m1);
   else
     return fail(PML_E_INVALID, "prolongation takes FP32 or FP64 planes");
   PML_LAUNCH_CHECK();

@@ -8,12 +8,16 @@ slices at once:
   all requested slices form one batch advanced by a single
   ``pml_fdm_solve`` call (the fine solves of every slice in one launch per
   step).  Optionally records every step's state for the trajectory output.
-* ``CoarseGridPropagator`` -- an ``FDMOperator`` on the 2x-coarsened grid:
-  injection to the coarse grid, stepping there, bilinear prolongation back
-  (``pml_restrict2x`` / ``pml_prolong2x``).  The grid transfer has no
+* ``CoarseGridPropagator`` -- an ``FDMOperator`` on a 2x-coarsened grid
+  (one or more levels): injection to the coarse grid, stepping there,
+  cubic (or bilinear) prolongation back (``pml_restrict2x`` /
+  ``pml_prolong2x_cubic`` / ``pml_prolong2x``).  The grid transfer has no
   counterpart in the reference; injection followed by prolongation is exact
   on the coarse points, so a chain of per-slice coarse solves equals one
   continuous coarse solve, as the reference's initial coarse sweep assumes.
+  The cubic prolongation matters for Parareal: the bilinear one's
+  interpolation error is the largest part of the coarse operator's error
+  on smooth states and costs iterations.
 
 * ``ODESlicePropagator`` -- an ``FDMOperator`` on a system without a mesh:
   all requested slices advance together in one launch of the ODE kernel
@@ -186,54 +190,128 @@ class ODESlicePropagator:
         return self.snaps.double()
 
 
-class CoarseGridPropagator:
-    """An FDM operator on the 2x-coarsened vertex grid of the problem."""
+def coarse_problem(cp, levels: int = 1):
+    """The constrained problem on the vertex grid coarsened ``levels`` times
+    by 2 (every other vertex kept, so each axis needs 2m - 1 vertices at
+    every level)."""
+    from pararealml_amd.constrained_problem import ConstrainedProblem
+    from pararealml_amd.mesh import Mesh
 
-    def __init__(self, op, cp, n_slices: int, slice_steps: int, stream):
-        import torch
-
-        from pararealml_amd.constrained_problem import ConstrainedProblem
-        from pararealml_amd.mesh import Mesh
-
+    for _ in range(levels):
         mesh = cp.mesh
         if mesh.dimensions != 2:
             raise NotImplementedError("grid coarsening covers 2-D problems")
-        if any((n - 1) % 2 for n in mesh.vertices_shape):
+        if any((n - 1) % 2 or n < 5 for n in mesh.vertices_shape):
             raise ValueError(
-                "coarsening needs an odd number of vertices (2m - 1) along every axis"
+                "coarsening needs an odd number of vertices (2m - 1, m >= 3) along every axis"
             )
-        coarse_mesh = Mesh(mesh.x_intervals, [2.0 * d for d in mesh.d_x],
-                           mesh.coordinate_system_type)
-        self.coarse_cp = ConstrainedProblem(
-            cp.differential_equation, coarse_mesh, cp.boundary_conditions
-        )
-        self.fine_shape = mesh.vertices_shape
-        self.m0, self.m1 = coarse_mesh.vertices_shape
+        cmesh = Mesh(mesh.x_intervals, [2.0 * d for d in mesh.d_x], mesh.coordinate_system_type)
+        cp = ConstrainedProblem(cp.differential_equation, cmesh, cp.boundary_conditions)
+    return cp
+
+
+class GridLevels:
+    """A problem's vertex grid (level 0) and its 2x coarsenings down to
+    ``levels``, with the transfers between them on FP64 device states:
+    injection downwards, bilinear or cubic (``pml_prolong2x_cubic``, where
+    the coarser grid has at least 4 vertices per axis) prolongation upwards,
+    the problem's Dirichlet values imposed on every grid a transfer lands
+    on.  Holds one FP64 buffer per intermediate level for ``batch``
+    states."""
+
+    def __init__(self, cp, levels: int, prolongation: str = "cubic", batch: int = 1):
+        import torch
+
+        if prolongation not in ("cubic", "bilinear"):
+            raise ValueError("prolongation is 'cubic' or 'bilinear'")
+        self.prolongation = prolongation
+        self.levels = levels
+        self.batch = batch
+        self.cps = [cp]
+        for _ in range(levels):
+            self.cps.append(coarse_problem(self.cps[-1]))
+        self.probs = [dev.device_problem(c) for c in self.cps]
+        self.bcs = [torch.from_numpy(p.bc).to("cuda") for p in self.probs]
+        self.bufs = [torch.empty(batch * p.elems, dtype=torch.float64, device="cuda")
+                     for p in self.probs]
+        self.lib = nat.device_lib()
+
+    def _dirichlet(self, t_ptr, level, count, sp):
+        p = self.probs[level]
+        nat.check(self.lib.pml_apply_dirichlet(t_ptr, self.bcs[level].data_ptr(), p.n_fields,
+                                               p.n0, p.n1, p.ndim, count, sp))
+
+    def down(self, src_ptr, dst_ptr, count: int, lo: int, hi: int, sp) -> None:
+        """Injection of ``count`` states from level ``lo`` to level ``hi``
+        (> lo)."""
+        if count > self.batch:
+            raise ValueError("more states than the transfer buffers hold")
+        cur = src_ptr
+        for lv in range(lo + 1, hi + 1):
+            p = self.probs[lv]
+            dst = dst_ptr if lv == hi else self.bufs[lv].data_ptr()
+            nat.check(self.lib.pml_restrict2x(nat.PML_F64, cur, dst, count * p.n_fields, p.n0,
+                                              p.n1, sp))
+            self._dirichlet(dst, lv, count, sp)
+            cur = dst
+
+    def up(self, src_ptr, dst_ptr, count: int, lo: int, hi: int, sp) -> None:
+        """Prolongation of ``count`` states from level ``hi`` to level ``lo``
+        (< hi)."""
+        if count > self.batch:
+            raise ValueError("more states than the transfer buffers hold")
+        cur = src_ptr
+        for lv in range(hi, lo, -1):
+            p = self.probs[lv]
+            dst = dst_ptr if lv - 1 == lo else self.bufs[lv - 1].data_ptr()
+            cubic = self.prolongation == "cubic" and p.n0 >= 4 and p.n1 >= 4
+            fn = self.lib.pml_prolong2x_cubic if cubic else self.lib.pml_prolong2x
+            nat.check(fn(nat.PML_F64, cur, dst, count * p.n_fields, p.n0, p.n1, sp))
+            self._dirichlet(dst, lv - 1, count, sp)
+            cur = dst
+
+
+class CoarseGridPropagator:
+    """An FDM operator on the ``levels`` times 2x-coarsened vertex grid of
+    the problem: each slice's initial value is injected down, stepped there
+    and prolonged back (cubic by default)."""
+
+    def __init__(self, op, cp, n_slices: int, slice_steps: int, stream, levels: int = 1,
+                 prolongation: str = "cubic"):
+        import torch
+
+        self.grids = GridLevels(cp, levels, prolongation)
+        self.levels = levels
+        self.coarse_cp = self.grids.cps[-1]
         self.inner = FDMSlicePropagator(op, self.coarse_cp, 1, slice_steps, stream)
-        self.n_fields = cp.differential_equation.y_dimension
         self.stream = stream
         self.c_in = torch.empty(self.inner.elems, dtype=torch.float64, device="cuda")
         self.c_out = torch.empty(self.inner.elems, dtype=torch.float64, device="cuda")
-        self.elems = self.n_fields * int(np.prod(self.fine_shape))
+        self.elems = self.grids.probs[0].elems
 
     def run(self, u_f64, first: int, last: int, out_f64):
-        lib = self.inner.batch.lib
         sp = ctypes.c_void_p(self.stream.cuda_stream)
         for k in range(last - first):
-            nat.check(lib.pml_restrict2x(nat.PML_F64, u_f64[k].data_ptr(), self.c_in.data_ptr(),
-                                         self.n_fields, self.m0, self.m1, sp))
+            self.grids.down(u_f64[k].data_ptr(), self.c_in.data_ptr(), 1, 0, self.levels, sp)
             self.inner.run(self.c_in.view(1, -1), 0, 1, self.c_out.view(1, -1))
-            nat.check(lib.pml_prolong2x(nat.PML_F64, self.c_out.data_ptr(), out_f64[k].data_ptr(),
-                                        self.n_fields, self.m0, self.m1, sp))
+            self.grids.up(self.c_out.data_ptr(), out_f64[k].data_ptr(), 1, 0, self.levels, sp)
 
 
 class CoarseGridFDMOperator:
-    """Marks an ``FDMOperator`` to be run on the 2x-coarsened grid when used
-    as a Parareal coarse propagator (a capability the reference does not
-    have: its coarse operators share the fine grid)."""
+    """Marks an ``FDMOperator`` to be run on a coarsened grid when used as a
+    Parareal coarse propagator (a capability the reference does not have:
+    its coarse operators share the fine grid): ``levels`` 2x coarsenings
+    (injection) and back by ``prolongation`` ("cubic", the default, or
+    "bilinear")."""
 
-    def __init__(self, fdm_operator):
+    def __init__(self, fdm_operator, levels: int = 1, prolongation: str = "cubic"):
+        if levels < 1:
+            raise ValueError("levels must be at least 1")
+        if prolongation not in ("cubic", "bilinear"):
+            raise ValueError("prolongation is 'cubic' or 'bilinear'")
         self.fdm = fdm_operator
+        self.levels = levels
+        self.prolongation = prolongation
 
     @property
     def d_t(self) -> float:
@@ -245,4 +323,5 @@ class CoarseGridFDMOperator:
 
     def slice_propagator(self, cp, n_slices, slice_steps, stream, record_trajectory=False,
                          borders=None):
-        return CoarseGridPropagator(self.fdm, cp, n_slices, slice_steps, stream)
+        return CoarseGridPropagator(self.fdm, cp, n_slices, slice_steps, stream, self.levels,
+                                    self.prolongation)

@@ -10,7 +10,8 @@ Parareal (parareal_operator.py:102-197) driven by the oracle network
 * ``LearnedCoarseOperator`` (the network as the whole coarse propagator,
   two applications per slice on the 2x grid);
 * ``CorrectedCoarseOperator`` on the shallow-water equations (three fields,
-  RK4 base).
+  RK4 base);
+* cubic prolongation (the default) throughout, and the bilinear one once.
 
 The device network accumulates in FP32 and the oracle in FP64 with BF16
 rounding at the same points, so the coarse predictions agree to about 1e-3
@@ -48,7 +49,7 @@ def _coarse_cp(cp):
                                   cp.boundary_conditions)
 
 
-def _oracle_g(kind, cp, params, base_dt, base_int, delta, t0, k_net):
+def _oracle_g(kind, cp, params, base_dt, base_int, delta, t0, k_net, prolong):
     """The coarse propagators restated with the oracle: per slice
     restrict -> (FDM base | k network steps) -> prolong (-> network
     correction, live or frozen per slice), Dirichlet values imposed on
@@ -69,9 +70,9 @@ def _oracle_g(kind, cp, params, base_dt, base_int, delta, t0, k_net):
         if kind == "learned":
             for _ in range(k_net):
                 yc = cc(so.predict(params, yc[np.newaxis])[0])
-            return cf(ora.prolong_bilinear(yc))
+            return cf(prolong(yc))
         yc = ora.fdm_solve(ccp, yc, (a, a + delta), base_dt, base_int, keep="last")[1][0]
-        yf = cf(ora.prolong_bilinear(cc(yc)))
+        yf = cf(prolong(cc(yc)))
         if kind == "live":
             return cf(so.predict(params, yf[np.newaxis])[0])
         if j not in frozen:
@@ -100,9 +101,11 @@ def _setup(problem):
     return cp, ic, dict(t1=0.16, n=4, f_dt=1e-3, base_dt=4e-3, base="rk4", c=3)
 
 
-@pytest.mark.parametrize("kind,problem", [("live", "diffusion"), ("frozen", "diffusion"),
-                                          ("learned", "diffusion"), ("live", "swe")])
-def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem):
+@pytest.mark.parametrize("kind,problem,prolongation",
+                         [("live", "diffusion", "cubic"), ("frozen", "diffusion", "cubic"),
+                          ("learned", "diffusion", "cubic"), ("live", "swe", "cubic"),
+                          ("live", "diffusion", "bilinear")])
+def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongation):
     import torch
 
     from pararealml_amd.operators.fdm import (
@@ -118,7 +121,8 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem):
     )
     from pararealml_amd.operators.parareal import PararealOperator
 
-    print(f"case: learned parareal {kind} {problem}", flush=True)
+    print(f"case: learned parareal {kind} {problem} {prolongation}", flush=True)
+    prolong = ora.prolong_cubic if prolongation == "cubic" else ora.prolong_bilinear
     cp, ic, cfg = _setup(problem)
     params = _params(cfg["c"], seed=5)
     delta = cfg["t1"] / cfg["n"]
@@ -128,7 +132,7 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem):
     ints = {"fe": ForwardEulerMethod, "rk4": RK4}
 
     # the oracle: every iterate, then a tolerance between two update norms
-    g_ora = _oracle_g(kind, cp, params, cfg["base_dt"], cfg["base"], delta, 0.0, k_net)
+    g_ora = _oracle_g(kind, cp, params, cfg["base_dt"], cfg["base"], delta, 0.0, k_net, prolong)
 
     def f_ora(y, a, b):
         return ora.fdm_solve(cp, y, (a, b), cfg["f_dt"], "rk4")[1]
@@ -151,11 +155,11 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem):
     net = ConvSurrogate(cfg["c"], seed=0)
     net.set_params(params)
     if kind == "learned":
-        g = LearnedCoarseOperator(net, delta / k_net, coarsen=True)
+        g = LearnedCoarseOperator(net, delta / k_net, coarsen=True, prolongation=prolongation)
     else:
         base = FDMOperator(ints[cfg["base"]](), diff, cfg["base_dt"])
         g = CorrectedCoarseOperator(base, net, base_levels=1, correct_on="fine",
-                                    freeze=kind == "frozen")
+                                    freeze=kind == "frozen", prolongation=prolongation)
     f = FDMOperator(RK4(), diff, cfg["f_dt"])
     ivp = pml.InitialValueProblem(cp, (0.0, cfg["t1"]), ic)
     got = []
@@ -183,7 +187,7 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem):
         tol = float(np.sqrt(norms[k] * min(norms[:k])))
         _, _, it_ora, _ = ora.parareal_solve(y0, (0.0, cfg["t1"]), cfg["n"], f_ora,
                                              _oracle_g(kind, cp, params, cfg["base_dt"],
-                                                       cfg["base"], delta, 0.0, k_net),
+                                                       cfg["base"], delta, 0.0, k_net, prolong),
                                              cfg["f_dt"], delta, cons, tol)
         # (a frozen operator's corrections belong to one solve: the
         # propagator is created anew by every solve)

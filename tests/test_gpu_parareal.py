@@ -50,10 +50,12 @@ def oracle_parareal(cp, ic, t_interval, n_slices, f_dt, g_dt, f_int, g_int, tol,
         ccp = pml.ConstrainedProblem(cp.differential_equation, cmesh, cp.boundary_conditions)
 
         def g_solve(y, a, b):
+            # injection, the coarse solve, cubic prolongation; Dirichlet
+            # values on both grids (CoarseGridFDMOperator's transfers)
             yc = ora.apply_constraints_last_axis(ccp.static_y_vertex_constraints,
                                                  ora.restrict_injection(y).copy())
             traj = ora.fdm_solve(ccp, yc, (a, b), g_dt, g_int)[1]
-            return np.stack([ora.prolong_bilinear(s) for s in traj])
+            return np.stack([constrain(ora.prolong_cubic(s)) for s in traj])
     else:
         def g_solve(y, a, b):
             return ora.fdm_solve(cp, y, (a, b), g_dt, g_int)[1]
@@ -212,3 +214,47 @@ def test_same_inputs_replay_bit_identically():
         outs.append((diff.last_jacobi_sweeps, res))
     assert outs[0][0] == outs[1][0]
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+@pytest.mark.parametrize("prec", ["float64", "float32"])
+def test_grid_transfers_match_oracle(prec):
+    """pml_restrict2x / pml_prolong2x / pml_prolong2x_cubic against the
+    oracle's injection and bilinear / cubic prolongation (themselves pinned
+    by exactness on bilinear / cubic fields) on a batch of random fields:
+    bitwise in FP64 (same operation order), to FP32 rounding in FP32; the
+    cubic refuses grids with fewer than 4 points per axis."""
+    import ctypes
+
+    import torch
+
+    from pararealml_amd import _native as nat
+
+    print(f"case: grid transfers {prec}", flush=True)
+    lib = nat.device_lib()
+    dtype = torch.float64 if prec == "float64" else torch.float32
+    pid = nat.PML_F64 if prec == "float64" else nat.PML_F32
+    rng = np.random.RandomState(3)
+    m0, m1, batch = 9, 14, 3
+    coarse = rng.standard_normal((batch, m0, m1))
+    fine = rng.standard_normal((batch, 2 * m0 - 1, 2 * m1 - 1))
+    sp = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    c_dev = torch.tensor(coarse, dtype=dtype, device="cuda")
+    f_dev = torch.tensor(fine, dtype=dtype, device="cuda")
+    out_f = torch.empty_like(f_dev)
+    out_c = torch.empty_like(c_dev)
+    nat.check(lib.pml_restrict2x(pid, f_dev.data_ptr(), out_c.data_ptr(), batch, m0, m1, sp))
+    got_r = out_c.cpu().numpy()
+    tol = 0.0 if prec == "float64" else 1e-6
+    for name, fn, ref in (("bilinear", lib.pml_prolong2x, ora.prolong_bilinear),
+                          ("cubic", lib.pml_prolong2x_cubic, ora.prolong_cubic)):
+        nat.check(fn(pid, c_dev.data_ptr(), out_f.data_ptr(), batch, m0, m1, sp))
+        got = out_f.cpu().numpy()
+        src = c_dev.cpu().numpy().astype(np.float64)
+        for b in range(batch):
+            want = ref(src[b][..., None])[..., 0]
+            np.testing.assert_allclose(got[b], want, rtol=0, atol=tol * 8, err_msg=name)
+    for b in range(batch):
+        np.testing.assert_array_equal(got_r[b], f_dev.cpu().numpy()[b][::2, ::2])
+    with pytest.raises(ValueError):
+        nat.check(lib.pml_prolong2x_cubic(pid, c_dev.data_ptr(), out_f.data_ptr(), batch, 3,
+                                          m1, sp))

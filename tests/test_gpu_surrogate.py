@@ -290,7 +290,9 @@ def test_corrected_coarse_with_identity_network_is_the_coarse_grid_fdm():
     assert p_c.last_iterations == p_f.last_iterations
     np.testing.assert_array_equal(y_c, y_f)
     with pytest.raises(ValueError):
-        CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=3)
+        CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=5)
+    with pytest.raises(ValueError):
+        CorrectedCoarseOperator(base, ConvSurrogate(1), base_levels=1, correct_on=2)
 
 
 @pytest.mark.parametrize("correct_on", ["coarse", "fine"])
@@ -339,9 +341,13 @@ def test_corrected_propagator_applies_the_network_in_stream_order(correct_on):
     torch.cuda.synchronize()
     y_net = out_net.cpu().numpy().reshape(65, 65)
     y_id = out_id.cpu().numpy().reshape(65, 65)
-    # the maximum of the 2x-grid state survives the prolongation
-    s = float(np.float32(np.abs(y_id).max()))
-    np.testing.assert_allclose(y_net[2:-2, 2:-2] - y_id[2:-2, 2:-2], s * beta, rtol=0, atol=1e-12)
+    # the network's scale: the maximum of the state it corrects (the 2x
+    # grid's points are the even points of the prolonged output)
+    s = float(np.float32(np.abs(y_id[::2, ::2] if correct_on == "coarse" else y_id).max()))
+    # (a 2x-grid shift reaches the fine grid through the cubic prolongation,
+    # whose 4-point stencils touch the zero Dirichlet edge up to fine index 3)
+    r = slice(4, -4) if correct_on == "coarse" else slice(2, -2)
+    np.testing.assert_allclose(y_net[r, r] - y_id[r, r], s * beta, rtol=0, atol=1e-12)
     # The correction was frozen at the first call: a perturbed input moves
     # the output exactly as it moves the base solve's.
     du = 1e-3 * torch.roll(u, 7, dims=1)

@@ -154,6 +154,29 @@ def test_grid_transfers_exact_on_bilinear_fields():
                                rtol=0, atol=1e-15)
 
 
+def test_cubic_prolongation_exact_on_cubics():
+    m0, m1 = 7, 9
+    rng = np.random.RandomState(1)
+    coef = rng.standard_normal((4, 4))
+    g0, g1 = np.linspace(-1, 2, 2 * m0 - 1), np.linspace(0, 1, 2 * m1 - 1)
+    X, Y = np.meshgrid(g0, g1, indexing="ij")
+    fine = sum(coef[a, b] * X**a * Y**b for a in range(4) for b in range(4))[..., None]
+    np.testing.assert_allclose(ora.prolong_cubic(fine[::2, ::2]), fine, rtol=0, atol=1e-12)
+    # the 4-point midpoint weights, one-sided at the ends (a non-polynomial field)
+    v = rng.standard_normal((m0, m1, 1))
+    p = ora.prolong_cubic(v)
+    np.testing.assert_array_equal(ora.restrict_injection(p), v)
+    np.testing.assert_allclose(p[3, 0], (-v[0, 0] + 9 * v[1, 0] + 9 * v[2, 0] - v[3, 0]) / 16,
+                               rtol=0, atol=1e-15)
+    np.testing.assert_allclose(p[1, 0], (5 * v[0, 0] + 15 * v[1, 0] - 5 * v[2, 0] + v[3, 0]) / 16,
+                               rtol=0, atol=1e-15)
+    np.testing.assert_allclose(p[-2, 0],
+                               (5 * v[-1, 0] + 15 * v[-2, 0] - 5 * v[-3, 0] + v[-4, 0]) / 16,
+                               rtol=0, atol=1e-15)
+    with pytest.raises(ValueError):
+        ora.prolong_cubic(np.zeros((3, 5, 1)))
+
+
 def _lorenz_setup():
     cp = pml.ConstrainedProblem(pml.LorenzEquation())
     y0 = np.array([1.0, 1.0, 1.0])
