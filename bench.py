@@ -79,10 +79,16 @@ def parse_args():
     ap.add_argument("--pr-traj", type=int, default=C3["traj"], help="training trajectories")
     ap.add_argument("--pr-train-steps", type=int, default=C3["train_steps"])
     ap.add_argument("--pr-train-batch", type=int, default=C3["train_batch"])
+    ap.add_argument("--pr-mix", type=float, default=C3["mix"],
+                    help="fraction of each training minibatch made of linear combinations of "
+                         "pairs (the diffusion propagators are linear)")
     ap.add_argument("--pr-correct-on", default=C3["correct_on"],
                     choices=["fine", "coarse", "base"],
                     help="grid the learned correction acts on (the base grid, the 2x grid or "
                          "the fine grid)")
+    ap.add_argument("--pr-no-increments", action="store_true",
+                    help="apply the learned coarse operator to the iterates instead of to their "
+                         "changes in Parareal's correction")
     ap.add_argument("--pr-frozen", action="store_true",
                     help="freeze each slice's correction after the initial sweep instead of "
                          "applying the network in every coarse evaluation")
@@ -198,8 +204,8 @@ def cpu_baseline(cps, sizes, y0, budget_s):
 # prolongation to the fine grid.
 C3 = dict(n=512, slices=64, fine_steps=78125, coarse_steps=2560, same_grid_steps=10240,
           base_steps=160, base_levels=3, correct_on="base", live=True, traj=32,
-          train_steps=10000, train_batch=16, target=1e-6, fine_precision="float32c",
-          serial_sample=8, learned=True)
+          train_steps=10000, train_batch=16, mix=0.5, increments=True, target=1e-6,
+          fine_precision="float32c", serial_sample=8, learned=True)
 
 
 def parareal_rows(cfg, log):
@@ -294,7 +300,7 @@ def parareal_rows(cfg, log):
     err_serial = float(np.max(np.linalg.norm(y_serial.reshape(sample, -1) - ref_flat[:sample],
                                              axis=1) / np.linalg.norm(ref_flat[:sample], axis=1)))
 
-    def run_parareal(g, name):
+    def run_parareal(g, name, increments=False):
         history = []
         t_start = time.perf_counter()
 
@@ -306,7 +312,8 @@ def parareal_rows(cfg, log):
             return e <= target or time.perf_counter() - t_start > t_serial
 
         f = FDMOperator(RK4(), diff, dt_f, precision=fp)
-        p = PararealOperator(f, g, 0.0, n_slices=S, output="endpoints", monitor=monitor)
+        p = PararealOperator(f, g, 0.0, n_slices=S, output="endpoints", monitor=monitor,
+                             coarse_increments=increments)
         log(f"parareal: {name}")
         p.solve(ivp)
         st = p.last_stats
@@ -390,7 +397,7 @@ def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
     log(f"parareal: training on {inputs.shape[0]} pairs")
     t0 = time.perf_counter()
     losses = net.fit(inputs, targets, steps=cfg["train_steps"], batch=cfg["train_batch"],
-                     lr=1e-3, log_every=max(1, cfg["train_steps"] // 5))
+                     lr=1e-3, log_every=max(1, cfg["train_steps"] // 5), mix=cfg["mix"])
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t0
     n_pairs = int(inputs.shape[0])
@@ -398,8 +405,10 @@ def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
     torch.cuda.empty_cache()
     g_learned = CorrectedCoarseOperator(base, net, fallback=g_fdm, base_levels=lv,
                                         correct_on=on, freeze=not cfg["live"])
-    learned_res = run_parareal(g_learned, "learned coarse")
-    same_base = lv == 1 and cfg["base_steps"] == cfg["coarse_steps"] and level == 1
+    inc = cfg["increments"] and cfg["live"]
+    learned_res = run_parareal(g_learned, "learned coarse", increments=inc)
+    same_base = (lv == 1 and cfg["base_steps"] == cfg["coarse_steps"] and level == 1
+                 and not inc)
     if same_base:
         # without the network the operator is the FDM coarse operator (an
         # untrained surrogate is the identity; tested bitwise)
@@ -407,7 +416,8 @@ def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
     else:
         g_plain = CorrectedCoarseOperator(base, ConvSurrogate(1, seed=0), fallback=g_fdm,
                                           base_levels=lv, correct_on=on)
-        plain_res = run_parareal(g_plain, "coarse pipeline without the network")
+        plain_res = run_parareal(g_plain, "coarse pipeline without the network",
+                                 increments=inc)
     return {
         "learned_coarse": dict(
             learned_res,
@@ -416,7 +426,8 @@ def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
                     + {0: "fine grid", 1: "2x grid"}.get(level, f"{2 ** level}x grid")
                     + (", applied in every coarse evaluation" if cfg["live"]
                        else ", frozen after the initial sweep")
-                    + ", cubic prolongation to the fine grid")),
+                    + ", cubic prolongation to the fine grid"
+                    + ("; Parareal correction in increment form" if inc else ""))),
         "learned_fallbacks": g_learned.fallbacks,
         "cheap_coarse_without_network": plain_res,
         "training": {
@@ -425,6 +436,7 @@ def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
             "data_generation_s": t_data,
             "train_steps": cfg["train_steps"],
             "train_batch": cfg["train_batch"],
+            "mix": cfg["mix"],
             "train_s": t_train,
             "loss_first": float(np.mean(losses[:10])),
             "loss_last": float(np.mean(losses[-10:])),
@@ -438,6 +450,7 @@ def parareal_config(args) -> dict:
                 base_steps=args.pr_base_steps, base_levels=args.pr_base_levels,
                 correct_on=args.pr_correct_on, live=not args.pr_frozen, traj=args.pr_traj,
                 train_steps=args.pr_train_steps, train_batch=args.pr_train_batch,
+                mix=args.pr_mix, increments=not args.pr_no_increments,
                 target=args.pr_target, fine_precision=args.pr_fine_precision,
                 serial_sample=args.pr_serial_sample, learned=not args.pr_no_learned)
 

@@ -551,9 +551,13 @@ def parareal_solve(
     constrain: Callable[[np.ndarray], np.ndarray],
     tol,
     max_iterations: int = 10**9,
+    coarse_increments: bool = False,
 ):
     """Serial emulation of PararealOperator.solve with ``n_slices`` ranks
-    (parareal_operator.py:102-197).
+    (parareal_operator.py:102-197).  With ``coarse_increments`` the sweep
+    sets U_new[j+1] = F(U_old[j]) + G(U_new[j] - U_old[j]) (G(0) = 0 for
+    j = i), which equals the reference's update for a linear G with
+    homogeneous constraints.
 
     ``f_solve(y0, t0, t1)`` / ``g_solve`` return full trajectories excluding
     t0; ``constrain`` applies the static Dirichlet constraints (what
@@ -580,7 +584,13 @@ def parareal_solve(
             fine[r] = f_solve(constrain(np.copy(u[r])), borders[r], borders[r + 1])
             corr[r] = fine[r][-1] - g_end[r]
         old = np.copy(u[1:])
+        prev = np.copy(u)
         for j in range(i, n_slices):
+            if coarse_increments:
+                g = (g_solve(constrain(u[j] - prev[j]), borders[j], borders[j + 1])[-1]
+                     if j > i else np.zeros_like(u[j]))
+                u[j + 1] = g + fine[j][-1]
+                continue
             if j > i:
                 g_end[j] = g_solve(constrain(np.copy(u[j])), borders[j], borders[j + 1])[-1]
             u[j + 1] = g_end[j] + corr[j]

@@ -316,12 +316,20 @@ class ConvSurrogate:
                 None, None, None, 0, 0, sp))
 
     def fit(self, inputs, targets, steps: int, batch: int = 32, lr: float = 1e-3,
-            seed: int = 0, log_every: int = 0, scale_output: bool = True) -> List[float]:
+            seed: int = 0, log_every: int = 0, scale_output: bool = True,
+            mix: float = 0.0) -> List[float]:
         """Adam on random minibatches of (inputs, targets) (device tensors
         (N, C, H, W)), cosine-decayed learning rate; returns the losses (in
         units of the output scale).  With ``scale_output``, an untrained
         network first sets ``out_scale`` to the RMS of the normalised
-        increments (t - u) / max|u| over up to 256 pairs."""
+        increments (t - u) / max|u| over up to 256 pairs.
+
+        ``mix`` (for maps that are linear in the state, such as the
+        propagators of a linear equation): that fraction of every minibatch
+        is replaced by linear combinations ``u_a + beta u_b`` of two pairs
+        (targets combined alike, beta uniform in [-1.25, 1.25]), so the
+        network also learns how the map responds to differences of states,
+        which is what Parareal's correction G(U_{k+1}) - G(U_k) evaluates."""
         t = self.torch
         gen = t.Generator(device="cpu")
         gen.manual_seed(seed)
@@ -333,9 +341,16 @@ class ConvSurrogate:
             rms = float(((tg - u) / sc).pow(2).mean().sqrt())
             self.out_scale = rms if rms > 0.0 and math.isfinite(rms) else 1.0
         losses = []
+        n_mix = int(round(batch * mix))
         for k in range(steps):
             idx = t.randint(0, n, (batch,), generator=gen).to(inputs.device)
-            loss = self.loss_and_grads(inputs[idx], targets[idx])
+            x, y = inputs[idx], targets[idx]
+            if n_mix:
+                jdx = t.randint(0, n, (n_mix,), generator=gen).to(inputs.device)
+                beta = (2.5 * t.rand((n_mix, 1, 1, 1), generator=gen) - 1.25).to(x)
+                x = t.cat([x[n_mix:], x[:n_mix] + beta * inputs[jdx]])
+                y = t.cat([y[n_mix:], y[:n_mix] + beta * targets[jdx]])
+            loss = self.loss_and_grads(x.contiguous(), y.contiguous())
             cur_lr = lr * 0.5 * (1.0 + math.cos(math.pi * k / max(steps, 1)))
             self.adam_step(cur_lr)
             losses.append(loss)

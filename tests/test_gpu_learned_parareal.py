@@ -11,7 +11,9 @@ Parareal (parareal_operator.py:102-197) driven by the oracle network
   two applications per slice on the 2x grid);
 * ``CorrectedCoarseOperator`` on the shallow-water equations (three fields,
   RK4 base);
-* cubic prolongation (the default) throughout, and the bilinear one once.
+* cubic prolongation (the default) throughout, and the bilinear one once;
+* the increment form of the correction (``coarse_increments``, the network
+  evaluated on U_new[j] - U_old[j]) with the live corrected operator.
 
 The device network accumulates in FP32 and the oracle in FP64 with BF16
 rounding at the same points, so the coarse predictions agree to about 1e-3
@@ -101,11 +103,14 @@ def _setup(problem):
     return cp, ic, dict(t1=0.16, n=4, f_dt=1e-3, base_dt=4e-3, base="rk4", c=3)
 
 
-@pytest.mark.parametrize("kind,problem,prolongation",
-                         [("live", "diffusion", "cubic"), ("frozen", "diffusion", "cubic"),
-                          ("learned", "diffusion", "cubic"), ("live", "swe", "cubic"),
-                          ("live", "diffusion", "bilinear")])
-def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongation):
+@pytest.mark.parametrize("kind,problem,prolongation,increments",
+                         [("live", "diffusion", "cubic", False),
+                          ("frozen", "diffusion", "cubic", False),
+                          ("learned", "diffusion", "cubic", False),
+                          ("live", "swe", "cubic", False),
+                          ("live", "diffusion", "bilinear", False),
+                          ("live", "diffusion", "cubic", True)])
+def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongation, increments):
     import torch
 
     from pararealml_amd.operators.fdm import (
@@ -121,7 +126,8 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongatio
     )
     from pararealml_amd.operators.parareal import PararealOperator
 
-    print(f"case: learned parareal {kind} {problem} {prolongation}", flush=True)
+    print(f"case: learned parareal {kind} {problem} {prolongation} increments={increments}",
+          flush=True)
     prolong = ora.prolong_cubic if prolongation == "cubic" else ora.prolong_bilinear
     cp, ic, cfg = _setup(problem)
     params = _params(cfg["c"], seed=5)
@@ -150,7 +156,7 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongatio
         return False
 
     ora.parareal_solve(y0, (0.0, cfg["t1"]), cfg["n"], f_ora, g_ora, cfg["f_dt"], delta, cons,
-                       record)
+                       record, coarse_increments=increments)
 
     net = ConvSurrogate(cfg["c"], seed=0)
     net.set_params(params)
@@ -168,7 +174,8 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongatio
         got.append(u[1:].cpu().numpy())
         return False
 
-    p = PararealOperator(f, g, 0.0, n_slices=cfg["n"], output="endpoints", monitor=monitor)
+    p = PararealOperator(f, g, 0.0, n_slices=cfg["n"], output="endpoints", monitor=monitor,
+                         coarse_increments=increments)
     p.solve(ivp)
     torch.cuda.synchronize()
     assert len(got) == len(iterates) == cfg["n"]
@@ -188,9 +195,11 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongatio
         _, _, it_ora, _ = ora.parareal_solve(y0, (0.0, cfg["t1"]), cfg["n"], f_ora,
                                              _oracle_g(kind, cp, params, cfg["base_dt"],
                                                        cfg["base"], delta, 0.0, k_net, prolong),
-                                             cfg["f_dt"], delta, cons, tol)
+                                             cfg["f_dt"], delta, cons, tol,
+                                             coarse_increments=increments)
         # (a frozen operator's corrections belong to one solve: the
         # propagator is created anew by every solve)
-        p = PararealOperator(f, g, tol, n_slices=cfg["n"], output="endpoints")
+        p = PararealOperator(f, g, tol, n_slices=cfg["n"], output="endpoints",
+                             coarse_increments=increments)
         p.solve(ivp)
         assert p.last_iterations == it_ora == k + 1

@@ -216,6 +216,47 @@ def test_same_inputs_replay_bit_identically():
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+def test_increment_form_equals_the_reference_update_for_fdm_coarse():
+    """coarse_increments=True (U_new[j+1] = F(U_old[j]) + G(U_new[j] -
+    U_old[j])) with a linear FDM coarse operator and zero Dirichlet edges
+    gives the reference update's iterates (to rounding) and iteration count;
+    it is refused for a non-linear equation, non-zero boundary values and a
+    frozen learned correction."""
+    from pararealml_amd.operators.ml import ConvSurrogate, CorrectedCoarseOperator
+
+    print("case: parareal increment form", flush=True)
+    cp, ic = diffusion_problem(33)
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.04), ic)
+    f = FDMOperator(RK4(), DIFF, 1e-4)
+    g = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), DIFF, 5e-4))
+    its = []
+    for inc in (False, True):
+        seen = []
+        p = PararealOperator(f, g, 1e-8, n_slices=4, output="endpoints", coarse_increments=inc,
+                             monitor=lambda i, u, seen=seen: seen.append(u.cpu().numpy()) and False)
+        p.solve(ivp)
+        its.append((p.last_iterations, seen))
+    assert its[0][0] == its[1][0] >= 2
+    for a, b in zip(its[0][1], its[1][1]):
+        assert rel_err(b, a) <= 1e-12
+    cp_ch, ic_ch = cahn_hilliard(17)
+    with pytest.raises(ValueError):
+        PararealOperator(f, g, 1e-6, n_slices=2, coarse_increments=True).solve(
+            pml.InitialValueProblem(cp_ch, (0.0, 0.02), ic_ch))
+    eq = pml.DiffusionEquation(2, 0.5)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / 16] * 2)
+    one = pml.DirichletBoundaryCondition(lambda x, t: np.ones((len(x), 1)), is_static=True)
+    cp1 = pml.ConstrainedProblem(eq, mesh, [(one, one), (one, one)])
+    ic1 = pml.DiscreteInitialCondition(cp1, np.ones(cp1.y_vertices_shape), True)
+    with pytest.raises(ValueError):
+        PararealOperator(f, g, 1e-6, n_slices=2, coarse_increments=True).solve(
+            pml.InitialValueProblem(cp1, (0.0, 0.02), ic1))
+    frozen = CorrectedCoarseOperator(FDMOperator(ForwardEulerMethod(), DIFF, 5e-4),
+                                     ConvSurrogate(1), base_levels=1, freeze=True)
+    with pytest.raises(ValueError):
+        PararealOperator(f, frozen, 1e-6, n_slices=2, coarse_increments=True).solve(ivp)
+
+
 @pytest.mark.parametrize("prec", ["float64", "float32"])
 def test_grid_transfers_match_oracle(prec):
     """pml_restrict2x / pml_prolong2x / pml_prolong2x_cubic against the

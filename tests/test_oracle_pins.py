@@ -237,6 +237,49 @@ def test_parareal_emulation_stops_at_first_small_update():
     assert it_seq == k + 1
 
 
+def test_parareal_increment_form_equals_the_reference_update_for_linear_g():
+    """U_new[j+1] = F(U_old[j]) + G(U_new[j] - U_old[j]) is the reference's
+    G(U_new[j]) + F(U_old[j]) - G(U_old[j]) when G is linear and the
+    constraints homogeneous: 2-D diffusion, zero Dirichlet edges, G = FE on
+    the 2x grid with injection and cubic prolongation."""
+    n = 17
+    eq = pml.DiffusionEquation(2, 0.5)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / (n - 1)] * 2)
+    bc = zero_bc(1)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    cmesh = pml.Mesh(mesh.x_intervals, [2.0 / (n - 1)] * 2)
+    ccp = pml.ConstrainedProblem(eq, cmesh, [(bc, bc), (bc, bc)])
+    y0 = pml.GaussianInitialCondition(
+        cp, [(np.array([0.4, 0.55]), 0.01 * np.eye(2))]).discrete_y_0(True)
+
+    def cons(y):
+        return ora.apply_constraints_last_axis(cp.static_y_vertex_constraints, y)
+
+    def f(y, a, b):
+        return ora.fdm_solve(cp, y, (a, b), 2e-4, "rk4")[1]
+
+    def g(y, a, b):
+        yc = ora.restrict_injection(y).copy()
+        traj = ora.fdm_solve(ccp, yc, (a, b), 1e-3, "fe")[1]
+        return np.stack([cons(ora.prolong_cubic(s)) for s in traj])
+
+    # (two of four iterations: after four both are the serial fine solution)
+    runs = [ora.parareal_solve(y0, (0.0, 0.04), 4, f, g, 2e-4, 1e-3, cons, 0.0,
+                               max_iterations=2, coarse_increments=inc) for inc in (False, True)]
+    assert runs[0][2] == runs[1][2] == 2
+    serial = f(y0, 0.0, 0.04)[49::50]
+    assert np.max(np.abs(runs[0][3][1:] - serial)) > 1e-9  # not yet converged
+    np.testing.assert_allclose(runs[1][3], runs[0][3], rtol=0,
+                               atol=1e-12 * np.abs(runs[0][3]).max())
+    # and it differs from the update when G is not linear (G(0) != 0)
+    shifted = ora.parareal_solve(y0, (0.0, 0.04), 4, f, lambda y, a, b: g(y, a, b) + 1e-3,
+                                 2e-4, 1e-3, cons, 0.0, max_iterations=1,
+                                 coarse_increments=True)
+    plain = ora.parareal_solve(y0, (0.0, 0.04), 4, f, lambda y, a, b: g(y, a, b) + 1e-3,
+                               2e-4, 1e-3, cons, 0.0, max_iterations=1)
+    assert np.max(np.abs(shifted[3] - plain[3])) > 1e-4
+
+
 def test_adam_step_closed_forms():
     params = so.init_params(1, 1, seed=3)
     rng = np.random.RandomState(7)
