@@ -318,6 +318,8 @@ def parareal_rows(cfg, log):
         p.solve(ivp)
         st = p.last_stats
         hit = next((k for k, (it, e) in enumerate(history) if e <= target), None)
+        log(f"parareal: {name}: errors " + " ".join(f"{e:.2e}" for _, e in history)
+            + f" in {sum(st['iteration_wall_s']):.2f} s")
         return {
             "iterations_to_target": history[hit][0] if hit is not None else None,
             "time_to_target_s": st["iteration_wall_s"][hit] if hit is not None else None,
@@ -415,7 +417,8 @@ def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
         plain_res = dict(fdm_res, same_as="fdm_coarse")
     else:
         g_plain = CorrectedCoarseOperator(base, ConvSurrogate(1, seed=0), fallback=g_fdm,
-                                          base_levels=lv, correct_on=on)
+                                          base_levels=lv, correct_on=on,
+                                          freeze=not cfg["live"])
         plain_res = run_parareal(g_plain, "coarse pipeline without the network",
                                  increments=inc)
     return {
