@@ -55,8 +55,8 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="ch256",
-                    choices=["ch256", "parareal", "swe-parareal", "swe-step", "copy",
-                             "surrogate"])
+                    choices=["ch256", "parareal", "parareal-fine-step", "swe-parareal",
+                             "swe-step", "copy", "surrogate"])
     ap.add_argument("--sur-batch", type=int, default=32)
     ap.add_argument("--sur-size", type=int, default=129, help="grid vertices per axis")
     # Parareal (2-D diffusion, learned vs FDM coarse); defaults: 513^2, 64 slices
@@ -99,6 +99,9 @@ def parse_args():
                          "scaled (0: the whole run)")
     ap.add_argument("--pr-no-learned", action="store_true",
                     help="skip the learned coarse operator (FDM coarse and serial only)")
+    ap.add_argument("--pr-reference-cache", default=None,
+                    help=".npz file holding the FP64 serial fine solution (and its time): "
+                         "read when it matches the configuration, written otherwise")
     # Shallow-water Parareal workload (learned vs FDM coarse on 1024^2)
     ap.add_argument("--swe-n", type=int, default=1024, help="cells per axis (vertices n + 1)")
     ap.add_argument("--swe-slices", type=int, default=64)
@@ -203,7 +206,7 @@ def cpu_baseline(cps, sizes, y0, budget_s):
 # prediction on the same grid in every coarse evaluation, cubic
 # prolongation to the fine grid.
 C3 = dict(n=512, slices=64, fine_steps=78125, coarse_steps=2560, same_grid_steps=10240,
-          base_steps=160, base_levels=3, correct_on="base", live=True, traj=32,
+          base_steps=640, base_levels=2, correct_on="base", live=True, traj=32,
           train_steps=10000, train_batch=16, mix=0.5, increments=True, target=1e-6,
           fine_precision="float32c", serial_sample=8, learned=True)
 
@@ -283,8 +286,21 @@ def parareal_rows(cfg, log):
         torch.cuda.synchronize()
         return time.perf_counter() - start, sol.discrete_y()
 
-    log("parareal: FP64 serial fine reference")
-    t_ref, ref = timed_serial("float64", S)
+    cache = cfg.get("reference_cache")
+    key = f"{n}-{S}-{cfg['fine_steps']}-{SEEDS[1]}"
+    cached = None
+    if cache and Path(cache).is_file():
+        with np.load(cache) as z:
+            if str(z["key"]) == key:
+                cached = float(z["seconds"]), z["ref"]
+    if cached is not None:
+        log("parareal: FP64 serial fine reference from " + cache)
+        t_ref, ref = cached
+    else:
+        log("parareal: FP64 serial fine reference")
+        t_ref, ref = timed_serial("float64", S)
+        if cache:
+            np.savez(cache, key=key, seconds=t_ref, ref=ref)
     ref_flat = ref.reshape(S, -1)
     ref_dev = torch.from_numpy(ref_flat).cuda()
     ref_norm = torch.linalg.norm(ref_dev, dim=1)
@@ -455,7 +471,8 @@ def parareal_config(args) -> dict:
                 train_steps=args.pr_train_steps, train_batch=args.pr_train_batch,
                 mix=args.pr_mix, increments=not args.pr_no_increments,
                 target=args.pr_target, fine_precision=args.pr_fine_precision,
-                serial_sample=args.pr_serial_sample, learned=not args.pr_no_learned)
+                serial_sample=args.pr_serial_sample, learned=not args.pr_no_learned,
+                reference_cache=args.pr_reference_cache)
 
 
 def parareal_workload(args):
@@ -736,62 +753,96 @@ def surrogate_workload(args):
     }), flush=True)
 
 
-def swe_step_workload(args):
-    """Stepping throughput of the shallow-water fine propagator of the
-    1025^2 Parareal workload: the 64 slices' states (64 grids of 1025^2
-    vertices, three fields, zero-Neumann edges, seeded Gaussian height bumps)
-    advanced together by RK4 in FP32 at d_t = 1e-3, ``--steps`` timed steps
-    after ``--warmup``, graph replays, kernel variant tuned first.  The
-    roofline counts 24 B per cell-step (three FP32 fields read and
-    written)."""
+def _batched_step(args, workload, cp, y0s, precision, integrator, d_t, bytes_per_cell,
+                  config):
+    """Times ``--steps`` graph-replayed steps (after ``--warmup``) of one
+    batch holding the states ``y0s`` of problem ``cp``, the kernel variant
+    tuned first; prints one JSON line with the roofline of the stepping
+    kernel (``bytes_per_cell`` algorithmic bytes per cell-step)."""
     import torch
 
-    import pararealml_amd as pml
     from pararealml_amd import _device as dev
-    from pararealml_amd import _native as nat
     from pararealml_amd.autotune import tune
 
     torch.cuda.set_device(0)
-    n, grids = args.swe_n, args.swe_slices
-    eq = pml.ShallowWaterEquation(h=1.0, v=0.01, g=9.81)
-    mesh = pml.Mesh([(-5.0, 5.0), (-5.0, 5.0)], [10.0 / n, 10.0 / n])
-    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True)
-    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
-    y0s = list(_swe_bumps(cp, grids, np.random.RandomState(SEEDS[2])))
     problem = dev.device_problem(cp)
-    cells = grids * (n + 1) ** 2
+    cells = len(y0s) * problem.elems // problem.n_fields
     stream = dev.solver_stream()
     stream.wait_stream(torch.cuda.current_stream())
     with torch.cuda.stream(stream):
-        batch = dev.FDMBatch([problem], [0] * grids, "float32", nat.PML_INT_RK4, stream)
+        batch = dev.FDMBatch([problem], [0] * len(y0s), precision, integrator, stream)
         batch.load_host(y0s)
-        cfg, timings = tune(batch, args.swe_dt, n_steps=8, repeats=2, store=False)
+        cfg, timings = tune(batch, d_t, n_steps=8, repeats=2, store=False)
         print(f"autotune: {timings} -> config {cfg}", file=sys.stderr)
         name = batch.lib.pml_fdm_config_name(batch.eq_id, batch.integrator, batch.prec,
                                              cfg).decode()
         batch.load_host(y0s)
-        batch.advance(args.swe_dt, args.warmup, tile_config=cfg, use_graph=True)
+        batch.advance(d_t, args.warmup, tile_config=cfg, use_graph=True)
+        batch.advance(d_t, args.steps, tile_config=cfg, use_graph=True, prepare=True)
         torch.cuda.synchronize()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
         ev0.record(stream)
-        batch.advance(args.swe_dt, args.steps, tile_config=cfg, use_graph=True)
+        batch.advance(d_t, args.steps, tile_config=cfg, use_graph=True)
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         step_s = ev0.elapsed_time(ev1) / 1e3 / args.steps
-    achieved = cells * 24 / step_s / 1e9
+    achieved = cells * bytes_per_cell / step_s / 1e9
     print(json.dumps({
-        "workload": "swe-step",
-        "config": {"grids": grids, "vertices": f"{n + 1}^2", "fields": 3, "integrator": "RK4",
-                   "dtype": "f32", "d_t": args.swe_dt, "kernel_variant": name},
+        "workload": workload,
+        "config": dict(config, kernel_variant=name, d_t=d_t, cells=cells),
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": step_s * 1e3, "wall_s": wall,
         "Gcell_steps_per_s": cells / step_s / 1e9,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS},
     }), flush=True)
+
+
+def swe_step_workload(args):
+    """Stepping throughput of the shallow-water fine propagator of the
+    1025^2 Parareal workload: the 64 slices' states (64 grids of 1025^2
+    vertices, three fields, zero-Neumann edges, seeded Gaussian height bumps)
+    advanced together by RK4 in FP32 at d_t = 1e-3.  The roofline counts
+    24 B per cell-step (three FP32 fields read and written)."""
+    import pararealml_amd as pml
+    from pararealml_amd import _native as nat
+
+    n, grids = args.swe_n, args.swe_slices
+    eq = pml.ShallowWaterEquation(h=1.0, v=0.01, g=9.81)
+    mesh = pml.Mesh([(-5.0, 5.0), (-5.0, 5.0)], [10.0 / n, 10.0 / n])
+    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    y0s = list(_swe_bumps(cp, grids, np.random.RandomState(SEEDS[2])))
+    _batched_step(args, "swe-step", cp, y0s, "float32", nat.PML_INT_RK4, args.swe_dt, 24,
+                  {"grids": grids, "vertices": f"{n + 1}^2", "fields": 3, "integrator": "RK4",
+                   "dtype": "f32"})
+
+
+def parareal_fine_step_workload(args):
+    """Stepping throughput of the Parareal fine propagator of the diffusion
+    workload: all slices' states (``--pr-slices`` grids of (``--pr-n`` +
+    1)^2 vertices, Dirichlet zero, seeded Gaussian mixtures) advanced
+    together by RK4 at the fine step in the fine precision (compensated FP32
+    by default: hi and lo planes read and written, 16 B per cell-step)."""
+    import pararealml_amd as pml
+    from pararealml_amd import _native as nat
+    from pararealml_amd.operators.ml import gaussian_mixture_states
+
+    n, S = args.pr_n, args.pr_slices
+    eq = pml.DiffusionEquation(2, 0.5)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / n, 1.0 / n])
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    y0s = list(gaussian_mixture_states(cp, S, seed=SEEDS[2]))
+    prec = args.pr_fine_precision
+    per_cell = {"float32": 8, "float32c": 16, "float64": 16}[prec]
+    d_t = 1.0 / S / args.pr_fine_steps
+    _batched_step(args, "parareal-fine-step", cp, y0s, prec, nat.PML_INT_RK4, d_t, per_cell,
+                  {"grids": S, "vertices": f"{n + 1}^2", "fields": 1, "integrator": "RK4",
+                   "dtype": prec})
 
 
 def copy_workload(args):
@@ -838,6 +889,9 @@ def main():
         return
     if args.workload == "swe-step":
         swe_step_workload(args)
+        return
+    if args.workload == "parareal-fine-step":
+        parareal_fine_step_workload(args)
         return
     if args.workload == "copy":
         copy_workload(args)

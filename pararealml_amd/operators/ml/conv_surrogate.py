@@ -331,6 +331,9 @@ class ConvSurrogate:
         network also learns how the map responds to differences of states,
         which is what Parareal's correction G(U_{k+1}) - G(U_k) evaluates."""
         t = self.torch
+        if not (bool(t.isfinite(inputs).all()) and bool(t.isfinite(targets).all())):
+            raise ValueError("training pairs hold non-finite values (an unstable base or data "
+                             "solve?)")
         gen = t.Generator(device="cpu")
         gen.manual_seed(seed)
         n = inputs.shape[0]

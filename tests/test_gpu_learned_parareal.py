@@ -13,7 +13,10 @@ Parareal (parareal_operator.py:102-197) driven by the oracle network
   RK4 base);
 * cubic prolongation (the default) throughout, and the bilinear one once;
 * the increment form of the correction (``coarse_increments``, the network
-  evaluated on U_new[j] - U_old[j]) with the live corrected operator.
+  evaluated on U_new[j] - U_old[j]) with the live corrected operator, once
+  with the correction on the fine grid and once as the benchmark runs it:
+  FE base on the 4x-coarsened grid, the network on that grid, prolonged
+  twice.
 
 The device network accumulates in FP32 and the oracle in FP64 with BF16
 rounding at the same points, so the coarse predictions agree to about 1e-3
@@ -57,6 +60,7 @@ def _oracle_g(kind, cp, params, base_dt, base_int, delta, t0, k_net, prolong):
     correction, live or frozen per slice), Dirichlet values imposed on
     every grid as the device propagators do."""
     ccp = _coarse_cp(cp)
+    ccp2 = _coarse_cp(ccp)
 
     def cf(y):
         return ora.apply_constraints_last_axis(cp.static_y_vertex_constraints, y)
@@ -64,11 +68,20 @@ def _oracle_g(kind, cp, params, base_dt, base_int, delta, t0, k_net, prolong):
     def cc(y):
         return ora.apply_constraints_last_axis(ccp.static_y_vertex_constraints, y)
 
+    def cc2(y):
+        return ora.apply_constraints_last_axis(ccp2.static_y_vertex_constraints, y)
+
     frozen = {}
 
     def one(y, a):
         j = int(np.rint((a - t0) / delta))
         yc = cc(ora.restrict_injection(y).copy())
+        if kind == "live-base":
+            y2 = cc2(ora.restrict_injection(yc).copy())
+            y2 = cc2(ora.fdm_solve(ccp2, y2, (a, a + delta), base_dt, base_int,
+                                   keep="last")[1][0])
+            y2 = cc2(so.predict(params, y2[np.newaxis])[0])
+            return cf(prolong(cc(prolong(y2))))
         if kind == "learned":
             for _ in range(k_net):
                 yc = cc(so.predict(params, yc[np.newaxis])[0])
@@ -109,7 +122,8 @@ def _setup(problem):
                           ("learned", "diffusion", "cubic", False),
                           ("live", "swe", "cubic", False),
                           ("live", "diffusion", "bilinear", False),
-                          ("live", "diffusion", "cubic", True)])
+                          ("live", "diffusion", "cubic", True),
+                          ("live-base", "diffusion", "cubic", True)])
 def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongation, increments):
     import torch
 
@@ -162,6 +176,10 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongatio
     net.set_params(params)
     if kind == "learned":
         g = LearnedCoarseOperator(net, delta / k_net, coarsen=True, prolongation=prolongation)
+    elif kind == "live-base":
+        base = FDMOperator(ints[cfg["base"]](), diff, cfg["base_dt"])
+        g = CorrectedCoarseOperator(base, net, base_levels=2, correct_on="base",
+                                    freeze=False, prolongation=prolongation)
     else:
         base = FDMOperator(ints[cfg["base"]](), diff, cfg["base_dt"])
         g = CorrectedCoarseOperator(base, net, base_levels=1, correct_on="fine",
