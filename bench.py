@@ -57,6 +57,9 @@ def parse_args():
     ap.add_argument("--workload", default="ch256",
                     choices=["ch256", "parareal", "parareal-fine-step", "swe-parareal",
                              "swe-step", "copy", "surrogate"])
+    ap.add_argument("--ch-ics", type=int, default=256,
+                    help="initial conditions of the Cahn-Hilliard batch (256: the configured "
+                         "workload; 1024 puts the state beyond the 256 MB MALL)")
     ap.add_argument("--sur-batch", type=int, default=32)
     ap.add_argument("--sur-size", type=int, default=129, help="grid vertices per axis")
     # Parareal (2-D diffusion, learned vs FDM coarse); defaults: 513^2, 64 slices
@@ -906,7 +909,7 @@ def main():
     from pararealml_amd import _native as nat
 
     torch.cuda.set_device(0)
-    sizes, cps, c0 = ch_workload()
+    sizes, cps, c0 = ch_workload(args.ch_ics)
     y0 = mu0_on_device(cps, sizes, c0)
     size_list = sorted(cps)
     problems = [dev.device_problem(cps[n]) for n in size_list]
@@ -968,7 +971,7 @@ def main():
 
     traffic = None
     tpath = ROOT / "profiles" / "traffic_ch256_rk4_fp32.json"
-    if tpath.exists():
+    if tpath.exists() and len(sizes) == 256:
         try:
             tj = json.loads(tpath.read_text())
             if tj.get("config") == cfg_name:
@@ -1003,7 +1006,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded uniform noise initial conditions)",
         "config": {
-            "workload": "batched Cahn-Hilliard RK4 FP32: 256 ICs on uneven {64,96,128,192,256}^2 grids, d_t=1e-4",
+            "workload": (f"batched Cahn-Hilliard RK4 FP32: {len(sizes)} ICs on uneven "
+                         "{64,96,128,192,256}^2 grids, d_t=1e-4"),
             "n_grids": len(sizes),
             "cells": cells,
             "kernel_variant": cfg_name,
