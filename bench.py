@@ -338,7 +338,7 @@ def parareal_rows(cfg, log):
         st = p.last_stats
         hit = next((k for k, (it, e) in enumerate(history) if e <= target), None)
         log(f"parareal: {name}: errors " + " ".join(f"{e:.2e}" for _, e in history)
-            + f" in {sum(st['iteration_wall_s']):.2f} s")
+            + f" in {st['iteration_wall_s'][-1]:.2f} s")
         return {
             "iterations_to_target": history[hit][0] if hit is not None else None,
             "time_to_target_s": st["iteration_wall_s"][hit] if hit is not None else None,

@@ -1120,10 +1120,11 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
         if (kE) {
           e_lo[f] = lane_value(pe[f], k);
           e_hi[f] = lane_value(pe[f], 32 + k);
+          // Both broadcasts are read unconditionally: behind a lane-varying
+          // condition the compiler would branch on EXEC around each one.
+          const D d_l = lane_value(pd[f], k), d_h = lane_value(pd[f], 32 + k);
           D v = lo0 ? rd_lo[f] : (hi0 ? rd_hi[f] : static_cast<D>(NAN));
-          v = (v == v) ? v
-                       : (l1 ? lane_value(pd[f], k)
-                             : (h1 ? lane_value(pd[f], 32 + k) : static_cast<D>(NAN)));
+          v = (v == v) ? v : (l1 ? d_l : (h1 ? d_h : static_cast<D>(NAN)));
           dv[f] = v;
         }
       }
