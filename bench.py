@@ -23,9 +23,12 @@ serial fine, FDM-coarse (2x grid and fine grid) and learned-coarse Parareal
 and its ablation (``parareal_rows``; ``--no-parareal`` leaves it out).
 
 Other workloads: ``parareal`` (that measurement alone, options ``--pr-*``),
-``swe-parareal`` (shallow water on 1025^2, 64 slices), ``swe-step`` (its
-batched fine stepping throughput), ``surrogate`` (conv
-surrogate training throughput) and ``copy`` (the HBM counter calibration).
+``parareal-fine-step`` (the batched stepping throughput of its fine
+propagator: all slices' states, compensated FP32 RK4), ``swe-parareal``
+(shallow water on 1025^2, 64 slices), ``swe-step`` (its batched fine
+stepping throughput), ``surrogate`` (conv surrogate training throughput)
+and ``copy`` (the HBM counter calibration).  ``--ch-ics 1024`` runs the
+Cahn-Hilliard batch beyond the 256 MB MALL.
 """
 
 import argparse

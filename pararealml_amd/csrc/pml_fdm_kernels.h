@@ -1746,6 +1746,10 @@ const std::vector<Variant> &variants_for() {
       out.push_back(PML_STREAM2(EQ, IN, PR, 8, 2, "stream2-rw8-pd2-s2"));
       out.push_back(PML_STREAMN(EQ, IN, PR, 8, 2, 3, "streamn-rw8-pd2-s3"));
       out.push_back(PML_STREAMN(EQ, IN, PR, 12, 2, 3, "streamn-rw12-pd2-s3"));
+      // short strips: more wavefronts and shorter row pipelines for single
+      // grids too small to fill the chip (Parareal coarse solves)
+      out.push_back(PML_STREAMN(EQ, IN, PR, 4, 1, 3, "streamn-rw4-pd1-s3"));
+      out.push_back(PML_STREAMN(EQ, IN, PR, 2, 1, 3, "streamn-rw2-pd1-s3"));
       if constexpr (wide) {
         out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
       } else {
@@ -1767,6 +1771,7 @@ const std::vector<Variant> &variants_for() {
       out.push_back(PML_STREAMRK(EQ, IN, PR, 16, 1, "streamrk-rw16-pd1"));
       out.push_back(PML_STREAMRK(EQ, IN, PR, 8, 2, "streamrk-rw8-pd2"));
       out.push_back(PML_STREAMRK(EQ, IN, PR, 4, 1, "streamrk-rw4-pd1"));
+      out.push_back(PML_STREAMRK(EQ, IN, PR, 2, 1, "streamrk-rw2-pd1"));
       if constexpr (wide) {
         out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
       } else {
