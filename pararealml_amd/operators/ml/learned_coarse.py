@@ -38,19 +38,67 @@ __all__ = ["coarse_problem", "LearnedCoarseOperator", "CorrectedCoarseOperator",
            "corrected_pairs", "gaussian_mixture_states", "generate_pairs"]
 
 
-def _fallback_check(op, fallback, u_f64, out_f64, k, first):
-    """Recomputes slice first + k by the fallback when the prediction is not
-    finite or its magnitude exceeds op.guard times the input's."""
-    import torch
+class _FallbackGuard:
+    """The fallback test of a learned propagator: a slice whose prediction
+    is not finite, or whose magnitude exceeds ``op.guard`` times its
+    input's, is recomputed by the fallback propagator (and counted in
+    ``op.fallbacks``).
 
-    if fallback is None:
-        return
-    src, dst = u_f64[k], out_f64[k]
-    bad = (~torch.isfinite(dst)).any() | (
-        dst.abs().max() > op.guard * src.abs().max().clamp_min(1e-300))
-    if bool(bad.item()):
-        op.fallbacks += 1
-        fallback.run(u_f64[k : k + 1], first + k, first + k + 1, out_f64[k : k + 1])
+    By default each slice is tested as it is predicted, which costs one host
+    synchronisation per slice.  With ``deferred`` set (the Parareal driver
+    sets it for its coarse sweeps) a test only records a device flag;
+    ``first_bad()`` reads the flags once per sweep, and the driver re-runs
+    the sweep from the first flagged slice after ``force_fallback`` on it,
+    which gives the result of testing every slice as it is made."""
+
+    supports_deferral = True
+
+    def _guard_init(self, n_slices: int) -> None:
+        self.deferred = False
+        self._n_slices = n_slices
+        self._flags = None
+        self._forced = set()
+
+    def _run_fallback(self, u_f64, out_f64, k: int, first: int) -> None:
+        self.op.fallbacks += 1
+        self.fallback.run(u_f64[k : k + 1], first + k, first + k + 1, out_f64[k : k + 1])
+
+    def _bad(self, u_f64, out_f64, k: int, first: int):
+        """Device flag: slice first + k's prediction needs the fallback."""
+        torch = self.torch
+        src, dst = u_f64[k], out_f64[k]
+        return (~torch.isfinite(dst)).any() | (
+            dst.abs().max() > self.op.guard * src.abs().max().clamp_min(1e-300))
+
+    def _guard(self, u_f64, out_f64, k: int, first: int) -> None:
+        """Runs on the propagator's stream (the caller's context)."""
+        if self.fallback is None:
+            return
+        torch = self.torch
+        bad = self._bad(u_f64, out_f64, k, first)
+        if self.deferred:
+            if self._flags is None:
+                self._flags = torch.zeros(self._n_slices, dtype=torch.bool, device="cuda")
+            self._flags[first + k] |= bad
+        elif bool(bad.item()):
+            self._run_fallback(u_f64, out_f64, k, first)
+
+    def first_bad(self):
+        """Index of the first slice flagged since the last call (None if
+        none); clears the flags.  One host synchronisation."""
+        if self._flags is None:
+            return None
+        with self.torch.cuda.stream(self.stream):
+            idx = self.torch.nonzero(self._flags).flatten().cpu()
+            self._flags.zero_()
+        return int(idx[0]) if idx.numel() else None
+
+    def force_fallback(self, j: int) -> None:
+        """Slice j is computed by the fallback until ``clear_forced``."""
+        self._forced.add(int(j))
+
+    def clear_forced(self) -> None:
+        self._forced.clear()
 
 
 class LearnedCoarseOperator:
@@ -85,11 +133,12 @@ class LearnedCoarseOperator:
             fb = self.fallback.slice_propagator(
                 cp, n_slices, int(round(slice_steps * self._d_t / self.fallback.d_t)), stream
             )
-        return LearnedPropagator(self, cp, slice_steps, stream, fb)
+        return LearnedPropagator(self, cp, slice_steps, stream, fb, n_slices)
 
 
-class LearnedPropagator:
-    def __init__(self, op: LearnedCoarseOperator, cp, steps: int, stream, fallback_prop):
+class LearnedPropagator(_FallbackGuard):
+    def __init__(self, op: LearnedCoarseOperator, cp, steps: int, stream, fallback_prop,
+                 n_slices: int = 1):
         import torch
 
         self.op = op
@@ -103,6 +152,7 @@ class LearnedPropagator:
         self.buf = torch.empty(
             (1, self.work.n_fields, self.work.n0, self.work.n1), dtype=torch.float64, device="cuda"
         )
+        self._guard_init(n_slices)
 
     def run(self, u_f64, first: int, last: int, out_f64):
         torch = self.torch
@@ -113,6 +163,9 @@ class LearnedPropagator:
         # the propagator's stream, in order with its kernels
         with torch.cuda.stream(self.stream):
             for k in range(last - first):
+                if first + k in self._forced:
+                    self._run_fallback(u_f64, out_f64, k, first)
+                    continue
                 if self.level:
                     g.down(u_f64[k].data_ptr(), self.buf.data_ptr(), 1, 0, self.level, sp)
                 else:
@@ -125,7 +178,7 @@ class LearnedPropagator:
                     g.up(self.buf.data_ptr(), out_f64[k].data_ptr(), 1, 0, self.level, sp)
                 else:
                     out_f64[k].copy_(self.buf.view(-1))
-                _fallback_check(self.op, self.fallback, u_f64, out_f64, k, first)
+                self._guard(u_f64, out_f64, k, first)
 
 
 def _level(correct_on: Union[str, int], base_levels: int) -> int:
@@ -204,7 +257,7 @@ class CorrectedCoarseOperator:
         return CorrectedPropagator(self, cp, n_slices, slice_steps, stream, fb)
 
 
-class CorrectedPropagator:
+class CorrectedPropagator(_FallbackGuard):
     def __init__(self, op: CorrectedCoarseOperator, cp, n_slices: int, steps: int, stream,
                  fallback_prop):
         import torch
@@ -228,6 +281,7 @@ class CorrectedPropagator:
             self.corr = torch.zeros((n_slices, cpb.elems), **f64)
             self.base_pred = torch.empty(cpb.elems, **f64)
             self.have = np.zeros(n_slices, dtype=bool)
+        self._guard_init(n_slices)
 
     def _correct(self, x, n: int, sp):
         """Applies slice n's correction to x ((1, C, H, W) on the correction
@@ -254,6 +308,9 @@ class CorrectedPropagator:
         # runs on the propagator's stream, in order with its kernels
         with torch.cuda.stream(self.stream):
             for k in range(last - first):
+                if first + k in self._forced:
+                    self._run_fallback(u_f64, out_f64, k, first)
+                    continue
                 g.down(u_f64[k].data_ptr(), self.b_in.data_ptr(), 1, 0, lb, sp)
                 self.inner.run(self.b_in, 0, 1, self.b_out)
                 g._dirichlet(self.b_out.data_ptr(), lb, 1, sp)
@@ -266,7 +323,7 @@ class CorrectedPropagator:
                     g.up(self.cbuf.data_ptr(), out_f64[k].data_ptr(), 1, 0, lc, sp)
                 else:
                     out_f64[k].copy_(self.cbuf.view(-1))
-                _fallback_check(self.op, self.fallback, u_f64, out_f64, k, first)
+                self._guard(u_f64, out_f64, k, first)
 
 
 def corrected_pairs(base_op, cp, snaps, slice_dt: float, stream=None, base_levels: int = 2,

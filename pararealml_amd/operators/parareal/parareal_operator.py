@@ -165,7 +165,7 @@ class PararealOperator(Operator):
             g_end = torch.empty((n, elems), **f64)
             f_end = torch.empty((n, elems), **f64)
             corr = torch.empty((n, elems), **f64)
-            delta = torch.empty((1, elems), **f64)
+            inc = torch.empty((1, elems), **f64)  # U_new[j] - U_old[j]
             sumsq = torch.zeros((n, n_fields), **f64)
             reduce_work = torch.empty(nat.PML_REDUCE_SLOTS * n_fields, **f64)
 
@@ -176,13 +176,35 @@ class PararealOperator(Operator):
                 stream.synchronize()
                 timings[key] += time.perf_counter() - start
 
+            # A learned coarse propagator tests each slice for its fallback;
+            # deferred, the tests cost one host synchronisation per sweep:
+            # a sweep is re-run from the first flagged slice with that slice
+            # computed by the fallback, as a test per slice would have done.
+            deferred = (getattr(coarse, "supports_deferral", False)
+                        and coarse.fallback is not None)
+            if deferred:
+                coarse.deferred = True
+
+            def guarded(sweep_from, start, before_redo=None):
+                sweep_from(start)
+                if deferred:
+                    while True:
+                        jb = coarse.first_bad()
+                        if jb is None:
+                            break
+                        coarse.force_fallback(jb)
+                        if before_redo is not None:
+                            before_redo(jb)
+                        sweep_from(jb)
+                    coarse.clear_forced()
+
             # initial coarse sweep
-            def first_sweep():
-                for j in range(n):
+            def first_sweep(start):
+                for j in range(start, n):
                     coarse.run(u[j : j + 1], j, j + 1, g_end[j : j + 1])
                     u[j + 1].copy_(g_end[j])
 
-            sync_time("coarse_s", first_sweep)
+            sync_time("coarse_s", lambda: guarded(first_sweep, 0))
 
             iterations = 0
             norms_hist = []
@@ -202,18 +224,24 @@ class PararealOperator(Operator):
                 if not increments:
                     sync_time("correction_s", correct)
                 old = u[1:].clone() if callable(tc) else None
-                prev = u.clone() if increments else None
+                prev = u.clone() if increments or deferred else None
                 sumsq.zero_()
 
-                def sweep():
-                    for j in range(i, n):
+                def redo(jb):
+                    # the update kernel adds to the norms and measures the
+                    # change against u: restore both from slice jb on
+                    u[jb + 1 :].copy_(prev[jb + 1 :])
+                    sumsq[jb:].zero_()
+
+                def sweep(start):
+                    for j in range(start, n):
                         if increments:
                             # U[j+1] = G(U_new[j] - U_old[j]) + F(U_old[j])
                             if j > i:
                                 nat.check(lib.pml_parareal_correction(
                                     nat.PML_F64, u[j].data_ptr(), None, elems,
-                                    prev[j].data_ptr(), delta[0].data_ptr(), elems, 1, sp))
-                                coarse.run(delta, j, j + 1, g_end[j : j + 1])
+                                    prev[j].data_ptr(), inc[0].data_ptr(), elems, 1, sp))
+                                coarse.run(inc, j, j + 1, g_end[j : j + 1])
                             else:
                                 g_end[j].zero_()  # U_new[i] == U_old[i]
                             src = f_end[j]
@@ -229,7 +257,7 @@ class PararealOperator(Operator):
                             )
                         )
 
-                sync_time("coarse_s", sweep)
+                sync_time("coarse_s", lambda: guarded(sweep, i, redo))
                 iter_wall.append(sum(timings.values()))
                 if self._monitor is not None and self._monitor(i, u):
                     break

@@ -234,7 +234,9 @@ def test_increment_form_equals_the_reference_update_for_fdm_coarse():
         seen = []
         p = PararealOperator(f, g, 1e-8, n_slices=4, output="endpoints", coarse_increments=inc,
                              monitor=lambda i, u, seen=seen: seen.append(u.cpu().numpy()) and False)
-        p.solve(ivp)
+        sol = p.solve(ivp)
+        # the end points are one slice apart (a plain float, not a device buffer)
+        assert isinstance(sol.d_t, float) and sol.d_t == pytest.approx(0.01)
         its.append((p.last_iterations, seen))
     assert its[0][0] == its[1][0] >= 2
     for a, b in zip(its[0][1], its[1][1]):
