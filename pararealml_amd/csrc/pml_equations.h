@@ -15,6 +15,24 @@
 
 namespace pml {
 
+// x * s + y for a power of two s (so x * s is exact): one fused
+// multiply-add with the same result as the separate multiply and add, in
+// either precision.
+template <typename C>
+__device__ __forceinline__ C fma_pow2(C s, C x, C y) {
+  if constexpr (sizeof(C) == 4) return __builtin_fmaf(s, x, y);
+  else return __builtin_fma(s, x, y);
+}
+
+// acc + x * y: one fused multiply-add in FP32, where the multi-field
+// kernels are bound by VALU issue; in FP64 the separate multiply and add of
+// the reference's operation order (bitwise the same as acc - (-x) * y).
+template <typename C>
+__device__ __forceinline__ C mad(C x, C y, C acc) {
+  if constexpr (sizeof(C) == 4) return __builtin_fmaf(x, y, acc);
+  else return acc + x * y;
+}
+
 template <typename C>
 struct Quant {
   C v;    // value
@@ -43,7 +61,7 @@ struct EqConvectionDiffusion {
   static constexpr bool kInvariant = false;
   template <typename C>
   __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
-    r[0] = p[0] * q[0].lap - p[1] * q[0].d0 - p[2] * q[0].d1;
+    r[0] = mad(-p[2], q[0].d1, mad(-p[1], q[0].d0, p[0] * q[0].lap));
   }
   template <typename C>
   __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
@@ -77,7 +95,7 @@ struct EqCahnHilliard {
   template <typename C>
   __device__ __forceinline__ static void alg(const C *p, const Quant<C> *q, C *r) {
     const C c = q[0].v;
-    r[1] = (c * c * c - c) - p[1] * q[0].lap;
+    r[1] = mad(-p[1], q[0].lap, c * c * c - c);
   }
 };
 
@@ -88,7 +106,7 @@ struct EqBurgers1 {
   static constexpr bool kInvariant = false;
   template <typename C>
   __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
-    r[0] = p[0] * q[0].lap - q[0].v * q[0].d0;
+    r[0] = mad(-q[0].v, q[0].d0, p[0] * q[0].lap);
   }
   template <typename C>
   __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
@@ -101,8 +119,8 @@ struct EqBurgers2 {
   static constexpr bool kInvariant = false;
   template <typename C>
   __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
-    r[0] = p[0] * q[0].lap - (q[0].v * q[0].d0 + q[1].v * q[0].d1);
-    r[1] = p[0] * q[1].lap - (q[0].v * q[1].d0 + q[1].v * q[1].d1);
+    r[0] = mad(p[0], q[0].lap, -mad(q[1].v, q[0].d1, q[0].v * q[0].d0));
+    r[1] = mad(p[0], q[1].lap, -mad(q[1].v, q[1].d1, q[0].v * q[1].d0));
   }
   template <typename C>
   __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
@@ -118,12 +136,12 @@ struct EqShallowWater {
   __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
     const C h = p[0], b = p[1], nu = p[2], f = p[3], g = p[4];
     const C eta = q[0].v, u = q[1].v, v = q[2].v;
-    r[0] = -h * (q[1].d0 + q[2].d1) - eta * q[1].d0 - u * q[0].d0 -
-           eta * q[2].d1 - v * q[0].d1;
-    r[1] = nu * q[1].lap - u * q[1].d0 - v * q[1].d1 - g * q[0].d0 - b * u +
-           f * v;
-    r[2] = nu * q[2].lap - u * q[2].d0 - v * q[2].d1 - g * q[0].d1 - b * v -
-           f * u;
+    r[0] = mad(-v, q[0].d1,
+               mad(-eta, q[2].d1, mad(-u, q[0].d0, mad(-eta, q[1].d0, -h * (q[1].d0 + q[2].d1)))));
+    r[1] = mad(f, v, mad(-b, u, mad(-g, q[0].d0, mad(-v, q[1].d1, mad(-u, q[1].d0,
+                                                                        nu * q[1].lap)))));
+    r[2] = mad(-f, u, mad(-b, v, mad(-g, q[0].d1, mad(-v, q[2].d1, mad(-u, q[2].d0,
+                                                                         nu * q[2].lap)))));
   }
   template <typename C>
   __device__ __forceinline__ static void alg(const C *, const Quant<C> *, C *) {}
@@ -141,7 +159,7 @@ struct EqNavierStokes {
   static constexpr bool kInvariant = false;
   template <typename C>
   __device__ __forceinline__ static void rhs(const C *p, const Quant<C> *q, C *r) {
-    r[0] = p[0] * q[0].lap - (q[2].v * q[0].d0 + q[3].v * q[0].d1);
+    r[0] = mad(p[0], q[0].lap, -mad(q[3].v, q[0].d1, q[2].v * q[0].d0));
   }
   template <typename C>
   __device__ __forceinline__ static void alg(const C *, const Quant<C> *q, C *r) {

@@ -133,6 +133,19 @@ __device__ __forceinline__ typename P::C state_plus(typename P::C hi, typename P
   }
 }
 
+// Working value of y + d / 2 (the midpoint and RK4 stage inputs), with the
+// halving fused into the add (exact: d / 2 needs no rounding).
+template <class P>
+__device__ __forceinline__ typename P::C state_plus_half(typename P::C hi, typename P::C lo,
+                                                         typename P::C d) {
+  using C = typename P::C;
+  if constexpr (P::kComp) {
+    return hi + fma_pow2(C(0.5), d, lo);
+  } else {
+    return fma_pow2(C(0.5), d, hi);
+  }
+}
+
 template <class P>
 __device__ __forceinline__ void state_set(typename P::C &hi, typename P::C &lo, double v) {
   if constexpr (P::kComp) {
@@ -188,10 +201,10 @@ __device__ __forceinline__ void field_quant(Quant<C> &q, C yc, C yw, C ye, C yn,
   if (!WL && !WD) return;
   C lap1 = 0, d1 = 0, lap0 = 0, d0 = 0;
   if (!(lo1 || hi1 || lo0 || hi0)) {
-    if (WL) lap1 = ((ye - C(2) * yc) + yw) * K.inv_dx1sq;
+    if (WL) lap1 = (fma_pow2(C(-2), yc, ye) + yw) * K.inv_dx1sq;
     if (WD) d1 = (ye - yw) * K.inv_2dx1;
     if (ax0) {
-      if (WL) lap0 = ((ys - C(2) * yc) + yn) * K.inv_dx0sq;
+      if (WL) lap0 = (fma_pow2(C(-2), yc, ys) + yn) * K.inv_dx0sq;
       if (WD) d0 = (ys - yn) * K.inv_2dx0;
     }
   } else {
@@ -202,7 +215,7 @@ __device__ __forceinline__ void field_quant(Quant<C> &q, C yc, C yw, C ye, C yn,
       C gw = zw, ge = ze;
       if (lo1) gw = (g_lo1 == g_lo1) ? ye + K.m2dx1 * static_cast<C>(g_lo1) : C(0);
       if (hi1) ge = (g_hi1 == g_hi1) ? yw + K.p2dx1 * static_cast<C>(g_hi1) : C(0);
-      lap1 = ((ge - C(2) * yc) + gw) * K.inv_dx1sq;
+      lap1 = (fma_pow2(C(-2), yc, ge) + gw) * K.inv_dx1sq;
     }
     if (WD) {
       d1 = (ze - zw) * K.inv_2dx1;
@@ -217,7 +230,7 @@ __device__ __forceinline__ void field_quant(Quant<C> &q, C yc, C yw, C ye, C yn,
         C gn = zn, gs = zs;
         if (lo0) gn = (g_lo0 == g_lo0) ? ys + K.m2dx0 * static_cast<C>(g_lo0) : C(0);
         if (hi0) gs = (g_hi0 == g_hi0) ? yn + K.p2dx0 * static_cast<C>(g_hi0) : C(0);
-        lap0 = ((gs - C(2) * yc) + gn) * K.inv_dx0sq;
+        lap0 = (fma_pow2(C(-2), yc, gs) + gn) * K.inv_dx0sq;
       }
       if (WD) {
         d0 = (zs - zn) * K.inv_2dx0;
@@ -305,8 +318,8 @@ __device__ __forceinline__ void field_quant_row(Quant<C> &q, C yc, C yw, C ye, C
       gw = lo1 ? (E.n_lo == E.n_lo ? ye + K.m2dx1 * E.n_lo : C(0)) : yw;
       ge = hi1 ? (E.n_hi == E.n_hi ? yw + K.p2dx1 * E.n_hi : C(0)) : ye;
     }
-    lap1 = ((ge - C(2) * yc) + gw) * K.inv_dx1sq;
-    if (ax0) lap0 = ((ys - C(2) * yc) + yn) * K.inv_dx0sq;
+    lap1 = (fma_pow2(C(-2), yc, ge) + gw) * K.inv_dx1sq;
+    if (ax0) lap0 = (fma_pow2(C(-2), yc, ys) + yn) * K.inv_dx0sq;
   }
   if (WD) {
     if (kCols) {
@@ -536,7 +549,7 @@ __global__ __launch_bounds__(256) void fdm_stream(FdmArgs a) {
           const C kk = dt * rr[f];
           C d = kk;
           if (Eq::kInvariant && Integ::kStages == 4)
-            d = (((kk + C(2) * kk) + C(2) * kk) + kk) * C(1.0 / 6.0);
+            d = (fma_pow2(C(2), kk, fma_pow2(C(2), kk, kk)) + kk) * C(1.0 / 6.0);
           state_add<P>(hv, lv, d);
         }
         if (dv && dv[c][f] == dv[c][f]) state_set<P>(hv, lv, dv[c][f]);
@@ -710,11 +723,11 @@ __device__ __forceinline__ void field_quant_sel(Quant<C> &q, C yc, C yw, C ye, C
   if (WL) {
     const C gw = lo1 ? (e_lo == e_lo ? ye + K.m2dx1 * e_lo : C(0)) : yw;
     const C ge = hi1 ? (e_hi == e_hi ? yw + K.p2dx1 * e_hi : C(0)) : ye;
-    lap1 = ((ge - C(2) * yc) + gw) * K.inv_dx1sq;
+    lap1 = (fma_pow2(C(-2), yc, ge) + gw) * K.inv_dx1sq;
     if (ax0) {
       const C gn = lo0 ? (r_lo == r_lo ? ys + K.m2dx0 * r_lo : C(0)) : yn;
       const C gs = hi0 ? (r_hi == r_hi ? yn + K.p2dx0 * r_hi : C(0)) : ys;
-      lap0 = ((gs - C(2) * yc) + gn) * K.inv_dx0sq;
+      lap0 = (fma_pow2(C(-2), yc, gs) + gn) * K.inv_dx0sq;
     }
   }
   if (WD) {
@@ -907,7 +920,7 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
           const C kk = dt * rr[f];
           C d = kk;
           if (Eq::kInvariant && Integ::kStages == 4)
-            d = (((kk + C(2) * kk) + C(2) * kk) + kk) * C(1.0 / 6.0);
+            d = (fma_pow2(C(2), kk, fma_pow2(C(2), kk, kk)) + kk) * C(1.0 / 6.0);
           state_add<P>(hv, lv, d);
         }
         if (kCols || kRows) {
@@ -1195,12 +1208,12 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
               nv = state_plus<P>(yh, yl, kk[f]);
             } else if (s == 1) {
               acc[f] = kk[f];
-              nv = state_plus<P>(yh, yl, kk[f] / C(2));
+              nv = state_plus_half<P>(yh, yl, kk[f]);
             } else if (s == 2) {
-              acc[f] = acc[f] + C(2) * kk[f];
-              nv = state_plus<P>(yh, yl, kk[f] / C(2));
+              acc[f] = fma_pow2(C(2), kk[f], acc[f]);
+              nv = state_plus_half<P>(yh, yl, kk[f]);
             } else {
-              acc[f] = acc[f] + C(2) * kk[f];
+              acc[f] = fma_pow2(C(2), kk[f], acc[f]);
               nv = state_plus<P>(yh, yl, kk[f]);
             }
             y[s - 1][t % 3][f] = fix ? static_cast<C>(dv[f]) : nv;
@@ -1385,7 +1398,7 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
           const C kk = dt * rr[f];
           C d = kk;
           if (Eq::kInvariant && Integ::kStages == 4)
-            d = (((kk + C(2) * kk) + C(2) * kk) + kk) * C(1.0 / 6.0);
+            d = (fma_pow2(C(2), kk, fma_pow2(C(2), kk, kk)) + kk) * C(1.0 / 6.0);
           state_add<P>(hv, lv, d);
         }
         if (kE) {
@@ -1611,7 +1624,7 @@ __global__ __launch_bounds__(NT) void fdm_fused(FdmArgs a) {
             } else if (Eq::kInvariant) {
               const C kk = kn[k][f];
               C d = kk;
-              if (Integ::kStages == 4) d = (((kk + C(2) * kk) + C(2) * kk) + kk) * C(1.0 / 6.0);
+              if (Integ::kStages == 4) d = (fma_pow2(C(2), kk, fma_pow2(C(2), kk, kk)) + kk) * C(1.0 / 6.0);
               state_add<P>(yh[k][f], yl[k][f], d);
               nv = yh[k][f];
             } else if (Integ::kStages == 1) {
@@ -1628,12 +1641,12 @@ __global__ __launch_bounds__(NT) void fdm_fused(FdmArgs a) {
               const C kk = kn[k][f];
               if (stage == 0) {
                 kacc[k][f] = kk;
-                nv = state_plus<P>(yh[k][f], yl[k][f], kk / C(2));
+                nv = state_plus_half<P>(yh[k][f], yl[k][f], kk);
               } else if (stage == 1) {
-                kacc[k][f] = kacc[k][f] + C(2) * kk;
-                nv = state_plus<P>(yh[k][f], yl[k][f], kk / C(2));
+                kacc[k][f] = fma_pow2(C(2), kk, kacc[k][f]);
+                nv = state_plus_half<P>(yh[k][f], yl[k][f], kk);
               } else if (stage == 2) {
-                kacc[k][f] = kacc[k][f] + C(2) * kk;
+                kacc[k][f] = fma_pow2(C(2), kk, kacc[k][f]);
                 nv = state_plus<P>(yh[k][f], yl[k][f], kk);
               } else {
                 kacc[k][f] = kacc[k][f] + kk;
