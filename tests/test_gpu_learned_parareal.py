@@ -221,3 +221,74 @@ def test_parareal_with_fixed_surrogate_matches_oracle(kind, problem, prolongatio
                              coarse_increments=increments)
         p.solve(ivp)
         assert p.last_iterations == it_ora == k + 1
+
+
+def test_trained_network_reduces_the_parareal_error():
+    """What the trained network gains inside Parareal: 2-D diffusion on
+    129^2, 16 slices of 1/64, RK4 fine (FP64), the corrected coarse operator
+    of the benchmark (FE on the 4x-coarsened grid at 80 % of its stability
+    limit, the network on that grid, cubic prolongation, increment form),
+    the network trained here on 16 seeded Gaussian-mixture trajectories
+    (1024 pairs, 2000 Adam steps); the same pipeline with the untrained
+    network (the identity) is the baseline.  Against the serial fine
+    solution the trained network lowers the slice-end error after the
+    first iteration (measured 3.5e-5 against 3.9e-5) and after the second
+    by a factor of about four (2.4e-8 against 1.0e-7)."""
+    from pararealml_amd.operators.fdm import (
+        RK4,
+        FDMOperator,
+        ForwardEulerMethod,
+        ThreePointCentralDifferenceMethod,
+    )
+    from pararealml_amd.operators.ml import (
+        ConvSurrogate,
+        CorrectedCoarseOperator,
+        corrected_pairs,
+        gaussian_mixture_states,
+        generate_pairs,
+    )
+    from pararealml_amd.operators.parareal import CoarseGridFDMOperator, PararealOperator
+    from tests.helpers import zero_bc
+
+    print("case: trained network gain in parareal, 129^2, 16 slices", flush=True)
+    n, S, lv = 128, 16, 2
+    delta = 1.0 / 64
+    diff = ThreePointCentralDifferenceMethod()
+    eq = pml.DiffusionEquation(2, 0.5)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / n, 1.0 / n])
+    bc = zero_bc(1)
+    cp = pml.ConstrainedProblem(eq, mesh, [(bc, bc), (bc, bc)])
+    y0 = gaussian_mixture_states(cp, 1, seed=434342745)[0]
+    ivp = pml.InitialValueProblem(cp, (0.0, S * delta), pml.DiscreteInitialCondition(cp, y0, True))
+    f_steps = 388  # RK4 at 95 % of its stability limit on this grid
+    f = FDMOperator(RK4(), diff, delta / f_steps)
+    ref = FDMOperator(RK4(), diff, delta / f_steps, snapshot_every=f_steps).solve(ivp)
+    ref = ref.discrete_y().reshape(S, -1)
+    base_steps = 40  # FE at 80 % of its stability limit on the 33^2 grid
+    base = FDMOperator(ForwardEulerMethod(), diff, delta / base_steps)
+    data_op = FDMOperator(RK4(), diff, delta / f_steps, precision="float32")
+    snaps = generate_pairs(data_op, cp, gaussian_mixture_states(cp, 16, seed=1462316), 1.0,
+                           delta, level=lv)
+    inputs, targets = corrected_pairs(base, cp, snaps, delta, base_levels=lv, correct_on="base")
+    net = ConvSurrogate(1, seed=0)
+    net.fit(inputs, targets, steps=2000, batch=16, lr=1e-3, mix=0.5)
+    fallback = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), diff, delta / base_steps))
+    errors = {}
+    for name, surrogate in (("trained", net), ("identity", ConvSurrogate(1, seed=0))):
+        g = CorrectedCoarseOperator(base, surrogate, fallback=fallback, base_levels=lv,
+                                    correct_on="base", freeze=False)
+        errs = []
+
+        def monitor(i, u, errs=errs):
+            y = u[1:].cpu().numpy().reshape(S, -1)
+            errs.append(float(np.max(np.linalg.norm(y - ref, axis=1)
+                                     / np.linalg.norm(ref, axis=1))))
+            return False
+
+        PararealOperator(f, g, 0.0, max_iterations=2, n_slices=S, output="endpoints",
+                         monitor=monitor, coarse_increments=True).solve(ivp)
+        errors[name] = errs
+    print(f"errors after iterations 1, 2: trained {errors['trained']}, "
+          f"identity {errors['identity']}", flush=True)
+    assert errors["trained"][0] < 0.95 * errors["identity"][0]
+    assert errors["trained"][1] < 0.5 * errors["identity"][1]
