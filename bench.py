@@ -202,15 +202,44 @@ def cpu_baseline(cps, sizes, y0, budget_s):
     }
 
 
+def parareal_cpu_baseline(cp, y0, d_t, n_slices, steps_per_slice, budget_s):
+    """The NumPy oracle (FP64 RK4, the reference's op-by-op cost) on the
+    Parareal problem's fine grid for about ``budget_s`` seconds, extrapolated
+    to the serial fine solve and to one fine phase of the reference's SPMD
+    Parareal (one slice per process, min(slices, host cores) processes)."""
+    from oracle import fdm_numpy as ora
+
+    def run(steps):
+        start = time.perf_counter()
+        ora.fdm_solve(cp, y0, (0.0, steps * d_t), d_t, "rk4", keep="last")
+        return time.perf_counter() - start
+
+    probe = run(1)
+    steps = int(max(2, min(200, budget_s / max(probe, 1e-3))))
+    per_step = run(steps) / steps
+    procs = min(n_slices, os.cpu_count() or 1)
+    total = n_slices * steps_per_slice
+    return {
+        "serial_fine_s": per_step * total,
+        "parareal_fine_phase_s": per_step * steps_per_slice * -(-n_slices // procs),
+        "processes": procs,
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"NumPy FP64 oracle, RK4 on the {y0.shape[0]}x{y0.shape[1]} grid, {steps} "
+                   f"steps at {per_step * 1e3:.1f} ms per step on one core, extrapolated to "
+                   f"{total} serial steps and to one fine phase of {procs} processes"),
+    }
+
+
 # The Parareal configuration the metric is quoted on: 2-D diffusion d = 0.5
 # on [0, 1]^2 with 513^2 vertices, 64 slices over t in [0, 1], fine RK4 at
 # d_t = 2e-7 (78,125 steps per slice), FDM coarse forward Euler on the
 # 2x-coarsened grid at 1/(64 * 2560) (the configured 1e-5 does not divide a
 # slice and exceeds the 2x grid's stability limit of 7.6e-6).
-# Learned coarse: forward Euler at 80 % of its stability limit on the 8x
-# coarsened grid (65^2, 160 steps per slice), the network correcting that
+# Learned coarse: forward Euler at 80 % of its stability limit on the 4x
+# coarsened grid (129^2, 640 steps per slice), the network correcting that
 # prediction on the same grid in every coarse evaluation, cubic
-# prolongation to the fine grid.
+# prolongation to the fine grid, Parareal's correction in increment form.
 C3 = dict(n=512, slices=64, fine_steps=78125, coarse_steps=2560, same_grid_steps=10240,
           base_steps=640, base_levels=2, correct_on="base", live=True, traj=32,
           train_steps=10000, train_batch=16, mix=0.5, increments=True, target=1e-6,
@@ -379,6 +408,8 @@ def parareal_rows(cfg, log):
         "serial_fine_sample": {"slices": sample, "seconds": t_sample,
                                "scaled_to_slices": S},
         "serial_fine_error": err_serial,
+        "cpu_baseline": (parareal_cpu_baseline(cp, y0, dt_f, S, cfg["fine_steps"], 3.0)
+                         if cfg.get("cpu_baseline", True) else None),
         "fdm_coarse": fdm_res,
         "fdm_coarse_same_grid": (
             dict(same_res, fe_steps_per_slice=cfg["same_grid_steps"]) if same_res else None),
@@ -481,7 +512,7 @@ def parareal_config(args) -> dict:
                 mix=args.pr_mix, increments=not args.pr_no_increments,
                 target=args.pr_target, fine_precision=args.pr_fine_precision,
                 serial_sample=args.pr_serial_sample, learned=not args.pr_no_learned,
-                reference_cache=args.pr_reference_cache)
+                reference_cache=args.pr_reference_cache, cpu_baseline=not args.no_cpu_baseline)
 
 
 def parareal_workload(args):
