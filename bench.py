@@ -415,6 +415,17 @@ def parareal_rows(cfg, log):
             dict(same_res, fe_steps_per_slice=cfg["same_grid_steps"]) if same_res else None),
     }
     out.update(learned)
+    # The fine sweep's stepping kernel, from the FDM-coarse run's fine time:
+    # every slice's state advanced together, one launch per RK4 step.
+    per_cell = {"float32": 8, "float32c": 16, "float64": 16}[fp]
+    step_s = fdm_res["fine_s"] / (cfg["fine_steps"] * len(fdm_res["iteration_wall_s"]))
+    achieved = S * (n + 1) ** 2 * per_cell / step_s / 1e9
+    out["fine_sweep_roofline"] = {
+        "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS, "s_per_step": step_s,
+        "basis": f"{per_cell} B per cell-step x {S} x {n + 1}^2 cells per step, fine time of "
+                 f"the FDM-coarse run (launch gaps included)",
+    }
     return out
 
 
