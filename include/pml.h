@@ -427,6 +427,15 @@ int pml_ode_solve(int precision, pml_program program, int integrator_id,
  *                         a 64 -> C layer in BF16.
  *   pml_conv_small_wgrad: dw[9][ci][co] and db[co] += gradients (FP32); x is
  *                         FP32 for a C -> 64 layer, BF16 for 64 -> C.
+ *                         Both weight-gradient entries take `part`: NULL
+ *                         adds each workgroup's sums atomically (the order,
+ *                         and so the last bits, vary from run to run);
+ *                         a device buffer of at least
+ *                         pml_grad_partials_floats(Ci, Co, B, H, W) floats
+ *                         receives each workgroup's sums, which are then
+ *                         added in workgroup order (bitwise reproducible).
+ *   pml_grad_partials_floats: size of that buffer for a Ci -> Co layer
+ *                         (64 -> 64, C -> 64 or 64 -> C) on B x H x W.
  *   pml_bias_grad64     : db[64] += sum over pixels of dz.
  *   pml_adam            : Adam step on FP32 masters p/m/v with gradient g
  *                         (step = 1, 2, ...); writes BF16 weights in the
@@ -445,12 +454,14 @@ int pml_conv64(int mode, const void *in, const void *wk, const float *bias,
                const void *zprev, void *z_out, void *a_out, int B, int H, int W,
                void *stream);
 int pml_wgrad64(const void *x, const void *dz, float *dw, float *db, int B, int H,
-                int W, void *stream);
+                int W, float *part, int64_t part_floats, void *stream);
 int pml_conv_small(int mode, const void *in, const void *w, const float *bias,
                    const void *zprev, void *z_out, void *a_out, float *y_out,
                    int Ci, int Co, int B, int H, int W, void *stream);
 int pml_conv_small_wgrad(const void *x, const void *dz, float *dw, float *db,
-                         int Ci, int Co, int B, int H, int W, void *stream);
+                         int Ci, int Co, int B, int H, int W, float *part,
+                         int64_t part_floats, void *stream);
+int pml_grad_partials_floats(int Ci, int Co, int B, int H, int W, int64_t *floats);
 int pml_bias_grad64(const void *dz, float *db, int64_t pixels, void *stream);
 int pml_adam(float *p, float *m, float *v, const float *g, int64_t n, float lr,
              float beta1, float beta2, float eps, int step, void *plain,

@@ -165,7 +165,9 @@ SIGNATURES = {
         [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int,
          c_int, c_void_p],
     ),
-    "pml_wgrad64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_void_p]),
+    "pml_wgrad64": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int,
+                            c_void_p, c_int64, c_void_p]),
+    "pml_grad_partials_floats": (c_int, [c_int, c_int, c_int, c_int, c_int, c_void_p]),
     "pml_conv_small": (
         c_int,
         [c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int,
@@ -173,7 +175,8 @@ SIGNATURES = {
     ),
     "pml_conv_small_wgrad": (
         c_int,
-        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p],
+        [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
+         c_int64, c_void_p],
     ),
     "pml_bias_grad64": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
     "pml_adam": (

@@ -324,10 +324,15 @@ struct WgradArgs {
   const bf16 *dz;  // [B, H, W, 64] gradient of the pre-activation
   float *dw;       // [9][64 ci][64 co], accumulated
   float *db;       // [64], accumulated (may be null)
+  float *part;     // per-block partial sums (kWgradPart floats a block), or null
   int B, H, W;
   int tiles_h, tiles_w, n_tiles;
   unsigned bytes;  // bytes of each [B, H, W, 64] tensor (< 2^31)
 };
+// Per-block partial sums of the weight-gradient kernels (deterministic
+// mode): the weight gradient, then the bias gradient.
+constexpr int kWgradDw = 9 * 64 * 64;
+constexpr int kWgradPart = kWgradDw + 64;
 
 __global__ __launch_bounds__(kWgradThreads) void k_wgrad64(WgradArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -442,18 +447,51 @@ __global__ __launch_bounds__(kWgradThreads) void k_wgrad64(WgradArgs a) {
   }
   // dw[g * 3 + tw][ci][co] += acc[tw]: element r of lane l is
   // (ci = 32 m + 8 (r / 4) + 4 (l / 32) + r % 4, co = 32 n + l % 32).
+  // With a partial buffer every block writes its sums (each element once;
+  // k_sum_blocks adds them in block order), else it adds them atomically.
+  float *pb = a.part ? a.part + static_cast<size_t>(blockIdx.x) * kWgradPart : nullptr;
 #pragma unroll
   for (int tw = 0; tw < 3; ++tw)
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int ci = m * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
-      atomicAdd(&a.dw[((g * 3 + tw) * 64 + ci) * 64 + n * 32 + i], acc[tw][r]);
+      const int idx = ((g * 3 + tw) * 64 + ci) * 64 + n * 32 + i;
+      if (pb) pb[idx] = acc[tw][r];
+      else atomicAdd(&a.dw[idx], acc[tw][r]);
     }
   // Lanes i and i + 32 summed the two column halves of channel 32 n + i.
   if (bias_wave) {
     bsum += __shfl_xor(bsum, 32);
-    if (kh == 0) atomicAdd(&a.db[n * 32 + i], bsum);
+    if (kh == 0) {
+      if (pb) pb[kWgradDw + n * 32 + i] = bsum;
+      else atomicAdd(&a.db[n * 32 + i], bsum);
+    }
   }
+}
+
+// The partials of a weight-gradient launch are added in a fixed order, so
+// the result does not depend on the order the workgroups ran in: pass 1
+// sums segments of kSumSeg consecutive workgroups (seg[y][i] = sum over
+// b in segment y of part[b][i]), pass 2 adds the segments in order into
+// dw[i] (i < n_dw) and db[i - n_dw] (n_dw <= i < n).
+constexpr int kSumSeg = 16;
+__global__ void k_sum_segments(const float *part, int n_blocks, int per_block, int n,
+                               float *seg) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b0 = blockIdx.y * kSumSeg, b1 = min(b0 + kSumSeg, n_blocks);
+  float s = 0.0f;
+  for (int b = b0; b < b1; ++b) s += part[static_cast<size_t>(b) * per_block + i];
+  seg[static_cast<size_t>(blockIdx.y) * n + i] = s;
+}
+__global__ void k_sum_blocks(const float *seg, int n_segs, int n, float *dw, int n_dw,
+                             float *db) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int y = 0; y < n_segs; ++y) s += seg[static_cast<size_t>(y) * n + i];
+  if (i < n_dw) dw[i] += s;
+  else db[i - n_dw] += s;
 }
 
 // ---------------------------------------------------------------------------
@@ -711,7 +749,7 @@ __device__ __forceinline__ void block_reduce16(float (&v)[NV], float (*red)[4][1
 // summed over pixels.  16 lanes per pixel, 4 output channels per lane.
 template <int CI>
 __global__ __launch_bounds__(256) void k_in_wgrad(const float *x, const bf16 *dz, float *dw,
-                                                  float *db, int B, int H, int W) {
+                                                  float *db, int B, int H, int W, float *part) {
   constexpr int NA = 9 * CI * 4, NV = NA + 4;
   __shared__ float red[NV][4][16];
   const int j = threadIdx.x & 15;
@@ -758,11 +796,16 @@ __global__ __launch_bounds__(256) void k_in_wgrad(const float *x, const bf16 *dz
     }
   }
   block_reduce16<NV>(v, red);
+  // per-block partials: dw [9 CI 64], then db [64]
+  float *pb = part ? part + static_cast<size_t>(blockIdx.x) * (9 * CI * 64 + 64) : nullptr;
   for (int e = threadIdx.x; e < NV * 16; e += 256) {
     const int i = e >> 4, l = e & 15;
     const float s = red[i][0][l] + red[i][1][l] + red[i][2][l] + red[i][3][l];
-    if (i < NA) atomicAdd(&dw[(i >> 2) * 64 + 4 * l + (i & 3)], s);  // [t][ci][co = 4l + k]
-    else atomicAdd(&db[4 * l + (i - NA)], s);
+    const int idx = i < NA ? (i >> 2) * 64 + 4 * l + (i & 3)  // [t][ci][co = 4l + k]
+                           : 9 * CI * 64 + 4 * l + (i - NA);
+    if (pb) pb[idx] = s;
+    else if (i < NA) atomicAdd(&dw[idx], s);
+    else atomicAdd(&db[idx - 9 * CI * 64], s);
   }
 }
 
@@ -770,7 +813,7 @@ __global__ __launch_bounds__(256) void k_in_wgrad(const float *x, const bf16 *dz
 // 16 lanes per pixel, 4 input channels per lane.
 template <int CO>
 __global__ __launch_bounds__(256) void k_out_wgrad(const bf16 *x, const bf16 *dz, float *dw,
-                                                   float *db, int B, int H, int W) {
+                                                   float *db, int B, int H, int W, float *part) {
   constexpr int NA = 9 * 4 * CO, NV = NA + CO;
   __shared__ float red[NV][4][16];
   const int j = threadIdx.x & 15;
@@ -819,14 +862,19 @@ __global__ __launch_bounds__(256) void k_out_wgrad(const bf16 *x, const bf16 *dz
     }
   }
   block_reduce16<NV>(v, red);
+  // per-block partials: dw [9 64 CO], then db [CO]
+  float *pb = part ? part + static_cast<size_t>(blockIdx.x) * (9 * 64 * CO + CO) : nullptr;
   for (int e = threadIdx.x; e < NV * 16; e += 256) {
     const int i = e >> 4, l = e & 15;
     const float s = red[i][0][l] + red[i][1][l] + red[i][2][l] + red[i][3][l];
     if (i < NA) {
       const int t = i / (4 * CO), k = (i / CO) & 3, o = i % CO;
-      atomicAdd(&dw[(t * 64 + 4 * l + k) * CO + o], s);  // [t][ci = 4l + k][o]
+      const int idx = (t * 64 + 4 * l + k) * CO + o;  // [t][ci = 4l + k][o]
+      if (pb) pb[idx] = s;
+      else atomicAdd(&dw[idx], s);
     } else if (l == 0) {
-      atomicAdd(&db[i - NA], s);
+      if (pb) pb[9 * 64 * CO + (i - NA)] = s;
+      else atomicAdd(&db[i - NA], s);
     }
   }
 }
@@ -1056,8 +1104,38 @@ extern "C" int pml_conv64(int mode, const void *in, const void *wk, const float 
   return PML_OK;
 }
 
+// Floats of partial-sum buffer a launch of n_blocks workgroups of
+// per_block sums needs: the partials, then one row per segment.
+static int64_t partial_floats(int n_blocks, int per_block) {
+  const int64_t segs = (n_blocks + kSumSeg - 1) / kSumSeg;
+  return (static_cast<int64_t>(n_blocks) + segs) * per_block;
+}
+
+// Adds a launch's per-block partials (`n_blocks` blocks of `per_block`
+// floats: n_dw weight entries, then the bias entries when db is set) to dw
+// and db in a fixed order.
+static int sum_blocks(float *part, int n_blocks, int per_block, float *dw, int n_dw, float *db,
+                      hipStream_t st) {
+  const int n = db ? per_block : n_dw;
+  const int segs = (n_blocks + kSumSeg - 1) / kSumSeg;
+  float *seg = part + static_cast<size_t>(n_blocks) * per_block;
+  hipLaunchKernelGGL(k_sum_segments, dim3((n + 255) / 256, segs), dim3(256), 0, st, part,
+                     n_blocks, per_block, n, seg);
+  PML_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_sum_blocks, dim3((n + 255) / 256), dim3(256), 0, st, seg, segs, n, dw,
+                     n_dw, db);
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+// Blocks of one pml_wgrad64 launch over nb images of H x W.
+static int wgrad64_blocks(int nb, int H, int W) {
+  const int n_tiles = nb * ((H + kGR - 1) / kGR) * ((W + kGC - 1) / kGC);
+  return std::min(n_tiles, cu_count());
+}
+
 extern "C" int pml_wgrad64(const void *x, const void *dz, float *dw, float *db, int B, int H,
-                           int W, void *stream) {
+                           int W, float *part, int64_t part_floats, void *stream) {
   if (B <= 0 || H <= 0 || W <= 0 || !x || !dz || !dw)
     return fail(PML_E_INVALID, "bad wgrad64 arguments");
   // Each launch covers whole images whose tensors stay below 2^31 bytes.
@@ -1081,9 +1159,17 @@ extern "C" int pml_wgrad64(const void *x, const void *dz, float *dw, float *db, 
     a.tiles_w = (W + kGC - 1) / kGC;
     a.n_tiles = nb * a.tiles_h * a.tiles_w;
     a.bytes = static_cast<unsigned>(nb * img_bytes);
-    hipLaunchKernelGGL(k_wgrad64, dim3(std::min(a.n_tiles, cu_count())), dim3(kWgradThreads),
-                       kWgradLds, as_stream(stream), a);
+    const int blocks = wgrad64_blocks(nb, H, W);
+    if (part && part_floats < partial_floats(blocks, kWgradPart))
+      return fail(PML_E_INVALID, "partial-sum buffer too small (see pml_grad_partials_floats)");
+    a.part = part;
+    hipLaunchKernelGGL(k_wgrad64, dim3(blocks), dim3(kWgradThreads), kWgradLds,
+                       as_stream(stream), a);
     PML_LAUNCH_CHECK();
+    if (part) {
+      const int rc = sum_blocks(part, blocks, kWgradPart, dw, kWgradDw, db, as_stream(stream));
+      if (rc) return rc;
+    }
   }
   return PML_OK;
 }
@@ -1166,7 +1252,8 @@ extern "C" int pml_conv_small(int mode, const void *in, const void *w, const flo
 }
 
 extern "C" int pml_conv_small_wgrad(const void *x, const void *dz, float *dw, float *db, int Ci,
-                                    int Co, int B, int H, int W, void *stream) {
+                                    int Co, int B, int H, int W, float *part,
+                                    int64_t part_floats, void *stream) {
   if (B <= 0 || H <= 0 || W <= 0 || !x || !dz || !dw || !db)
     return fail(PML_E_INVALID, "bad weight-gradient arguments");
   if (static_cast<int64_t>(H) * W * 128 >= (int64_t(1) << 31))
@@ -1185,14 +1272,39 @@ extern "C" int pml_conv_small_wgrad(const void *x, const void *dz, float *dw, fl
     const bf16 *dzb = static_cast<const bf16 *>(dz) + px * Co;
     // two pixels per lane group and iteration: 32 pixels per block
     const dim3 grid(small_grid(nb * hw, 32, 4)), block(256);
-    if (Ci == 1) hipLaunchKernelGGL(k_in_wgrad<1>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W);
-    else if (Ci == 2) hipLaunchKernelGGL(k_in_wgrad<2>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W);
-    else if (Ci == 3) hipLaunchKernelGGL(k_in_wgrad<3>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W);
-    else if (Co == 1) hipLaunchKernelGGL(k_out_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
-    else if (Co == 2) hipLaunchKernelGGL(k_out_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
-    else hipLaunchKernelGGL(k_out_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W);
+    const int n_dw = 9 * Ci * Co, per_block = n_dw + Co;
+    if (part && part_floats < partial_floats(static_cast<int>(grid.x), per_block))
+      return fail(PML_E_INVALID, "partial-sum buffer too small (see pml_grad_partials_floats)");
+    if (Ci == 1) hipLaunchKernelGGL(k_in_wgrad<1>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W, part);
+    else if (Ci == 2) hipLaunchKernelGGL(k_in_wgrad<2>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W, part);
+    else if (Ci == 3) hipLaunchKernelGGL(k_in_wgrad<3>, grid, block, 0, st, xf, dzb, dw, db, nb, H, W, part);
+    else if (Co == 1) hipLaunchKernelGGL(k_out_wgrad<1>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W, part);
+    else if (Co == 2) hipLaunchKernelGGL(k_out_wgrad<2>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W, part);
+    else hipLaunchKernelGGL(k_out_wgrad<3>, grid, block, 0, st, xb, dzb, dw, db, nb, H, W, part);
     PML_LAUNCH_CHECK();
+    if (part) {
+      const int rc = sum_blocks(part, static_cast<int>(grid.x), per_block, dw, n_dw, db, st);
+      if (rc) return rc;
+    }
   }
+  return PML_OK;
+}
+
+extern "C" int pml_grad_partials_floats(int Ci, int Co, int B, int H, int W, int64_t *floats) {
+  if (B <= 0 || H <= 0 || W <= 0 || !floats) return fail(PML_E_INVALID, "bad arguments");
+  if (Ci == 64 && Co == 64) {
+    const int64_t img_bytes = static_cast<int64_t>(H) * W * 128;
+    if (img_bytes >= (int64_t(1) << 31)) return fail(PML_E_INVALID, "image too large");
+    const int per = static_cast<int>(std::max<int64_t>(1, ((int64_t(1) << 31) - 1) / img_bytes));
+    *floats = partial_floats(wgrad64_blocks(std::min(per, B), H, W), kWgradPart);
+    return PML_OK;
+  }
+  const bool in_layer = Co == 64 && Ci >= 1 && Ci <= 3;
+  const bool out_layer = Ci == 64 && Co >= 1 && Co <= 3;
+  if (!in_layer && !out_layer) return fail(PML_E_INVALID, "unsupported layer shape");
+  const int per = small_chunk(H, W);
+  const int64_t hw = static_cast<int64_t>(H) * W;
+  *floats = partial_floats(small_grid(std::min(per, B) * hw, 32, 4), 9 * Ci * Co + Co);
   return PML_OK;
 }
 

@@ -188,6 +188,15 @@ class ConvSurrogate:
             ws["dy"] = t.empty(px * c, **bf)
             ws["dz"] = [t.empty(px * HIDDEN, **bf) for _ in range(2)]
             ws["loss"] = t.zeros(1, dtype=t.float64, device="cuda")
+            # per-workgroup weight-gradient sums, added in a fixed order:
+            # training is bitwise reproducible for a given seed
+            need = 0
+            for layer in range(N_LAYERS):
+                n = ctypes.c_int64(0)
+                nat.check(self.lib.pml_grad_partials_floats(
+                    self.chans[layer], self.chans[layer + 1], batch, h, w, ctypes.byref(n)))
+                need = max(need, n.value)
+            ws["gpart"] = t.empty(need, **f32)
         return ws
 
     def _out_s(self, ws, stream):
@@ -275,9 +284,10 @@ class ConvSurrogate:
             ws["dy"].data_ptr(), ws["loss"].data_ptr(), batch, c, h * w, sp))
         last = N_LAYERS - 1
         # layer 5 (64 -> C): weight / bias gradients and the input gradient
+        gpart, gfloats = ws["gpart"].data_ptr(), ws["gpart"].numel()
         nat.check(lib.pml_conv_small_wgrad(
             ws["a"][last - 1].data_ptr(), ws["dy"].data_ptr(), self.gw[last].data_ptr(),
-            self.gb[last].data_ptr(), HIDDEN, c, batch, h, w, sp))
+            self.gb[last].data_ptr(), HIDDEN, c, batch, h, w, gpart, gfloats, sp))
         dz_cur, dz_next = ws["dz"]
         nat.check(lib.pml_conv_small(
             1, ws["dy"].data_ptr(), self.plain[last].data_ptr(), None,
@@ -287,7 +297,7 @@ class ConvSurrogate:
         for layer in range(last - 1, 0, -1):
             nat.check(lib.pml_wgrad64(
                 ws["a"][layer - 1].data_ptr(), dz_cur.data_ptr(), self.gw[layer].data_ptr(),
-                self.gb[layer].data_ptr(), batch, h, w, sp))
+                self.gb[layer].data_ptr(), batch, h, w, gpart, gfloats, sp))
             nat.check(lib.pml_conv64(
                 1, dz_cur.data_ptr(), self.wb[layer].data_ptr(), None,
                 ws["z"][layer - 1].data_ptr(), None, dz_next.data_ptr(), batch, h, w, sp))
@@ -295,7 +305,7 @@ class ConvSurrogate:
         # layer 0 (C -> 64)
         nat.check(lib.pml_conv_small_wgrad(
             ws["x"].data_ptr(), dz_cur.data_ptr(), self.gw[0].data_ptr(),
-            self.gb[0].data_ptr(), c, HIDDEN, batch, h, w, sp))
+            self.gb[0].data_ptr(), c, HIDDEN, batch, h, w, gpart, gfloats, sp))
         return float(ws["loss"].item()) / (batch * c * h * w)
 
     def adam_step(self, lr: float, beta1=0.9, beta2=0.999, eps=1e-8):

@@ -2,6 +2,8 @@
 rounding points, FP64 accumulation there vs FP32 on the device), training,
 and the learned coarse propagator with its FDM fallback."""
 
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -94,16 +96,32 @@ def test_wgrad64_matches_oracle(shape, with_bias):
     db = torch.zeros(64, dtype=torch.float32, device="cuda")
     xt = torch.tensor(x, dtype=torch.bfloat16, device="cuda")
     dzt = torch.tensor(dz, dtype=torch.bfloat16, device="cuda")
-    nat.check(lib.pml_wgrad64(xt.data_ptr(), dzt.data_ptr(), dw.data_ptr(),
-                              db.data_ptr() if with_bias else None, b, h, w, None))
-    torch.cuda.synchronize()
-    assert rel(dw.cpu().numpy().reshape(9, 64, 64), ref) <= 1e-5
-    if with_bias:
-        # the bias gradient is the dz column sum over all pixels
-        db_ref = dz.reshape(-1, 64).sum(axis=0)
-        assert rel(db.cpu().numpy(), db_ref) <= 1e-5
-    else:
-        assert not db.any()
+    n = ctypes.c_int64(0)
+    nat.check(lib.pml_grad_partials_floats(64, 64, b, h, w, ctypes.byref(n)))
+    part = torch.empty(n.value, dtype=torch.float32, device="cuda")
+    runs = []
+    # atomic sums, then twice the per-workgroup partials added in a fixed order
+    for buf in (None, part, part):
+        dw.zero_()
+        db.zero_()
+        nat.check(lib.pml_wgrad64(xt.data_ptr(), dzt.data_ptr(), dw.data_ptr(),
+                                  db.data_ptr() if with_bias else None, b, h, w,
+                                  None if buf is None else buf.data_ptr(),
+                                  0 if buf is None else buf.numel(), None))
+        torch.cuda.synchronize()
+        runs.append((dw.cpu().numpy().copy(), db.cpu().numpy().copy()))
+        assert rel(runs[-1][0].reshape(9, 64, 64), ref) <= 1e-5
+        if with_bias:
+            # the bias gradient is the dz column sum over all pixels
+            db_ref = dz.reshape(-1, 64).sum(axis=0)
+            assert rel(runs[-1][1], db_ref) <= 1e-5
+        else:
+            assert not runs[-1][1].any()
+    np.testing.assert_array_equal(runs[1][0], runs[2][0])
+    np.testing.assert_array_equal(runs[1][1], runs[2][1])
+    with pytest.raises(ValueError):  # a partial buffer that is too small is refused
+        nat.check(lib.pml_wgrad64(xt.data_ptr(), dzt.data_ptr(), dw.data_ptr(), None, b, h, w,
+                                  part.data_ptr(), 16, None))
 
 
 @pytest.mark.parametrize("c", [1, 3])
@@ -468,3 +486,30 @@ def test_deferred_fallback_tests_equal_per_slice_tests(monkeypatch):
         np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(y_d, y_s)
     torch.cuda.synchronize()
+
+
+def test_training_is_bitwise_reproducible():
+    """The weight gradients are summed per workgroup and then in a fixed
+    order, so two trainings from the same seed on the same pairs end with
+    bitwise-identical parameters (the small layers, the 64 -> 64 layers and
+    the biases alike)."""
+    import torch
+
+    from pararealml_amd.operators.ml import ConvSurrogate
+
+    print("case: reproducible training", flush=True)
+    rng = np.random.RandomState(9)
+    u = torch.tensor(rng.standard_normal((24, 1, 33, 33)), dtype=torch.float32, device="cuda")
+    t = u + 0.01 * torch.tensor(rng.standard_normal((24, 1, 33, 33)), dtype=torch.float32,
+                                device="cuda")
+    params = []
+    for _ in range(2):
+        net = ConvSurrogate(1, seed=4)
+        net.fit(u, t, steps=30, batch=8, lr=1e-3, seed=1, mix=0.5)
+        torch.cuda.synchronize()
+        params.append([(w.cpu().numpy().copy(), b.cpu().numpy().copy())
+                       for w, b in zip(net.w32, net.b32)])
+    for (w0, b0), (w1, b1) in zip(*params):
+        np.testing.assert_array_equal(w0, w1)
+        np.testing.assert_array_equal(b0, b1)
+    assert any(np.any(w != 0) for w, _ in params[0])
