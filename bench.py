@@ -1022,8 +1022,7 @@ def main():
     if tpath.exists() and len(sizes) == 256:
         try:
             tj = json.loads(tpath.read_text())
-            if tj.get("config") == cfg_name:
-                traffic = tj.get("hbm_bytes_per_launch")
+            traffic = tj.get("variants", {}).get(cfg_name, {}).get("hbm_bytes_per_launch")
         except ValueError:
             traffic = None
 

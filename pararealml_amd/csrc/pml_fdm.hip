@@ -453,6 +453,14 @@ static int fdm_solve_impl(int eq_id, const double *eq_params, int n_params,
         hipGraphDestroy(graph);
         return fail(PML_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
       }
+      // Move the graph's launch data to the device now (stream-ordered),
+      // so that the first replay costs what the later ones do.
+      e = hipGraphUpload(exec, st);
+      if (e != hipSuccess) {
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        return fail(PML_E_HIP, std::string("hipGraphUpload: ") + hipGetErrorString(e));
+      }
       std::lock_guard<std::mutex> lock(g_mu);
       if (g_graphs.size() >= kMaxGraphs) {
         rc = evict_oldest_graph();
