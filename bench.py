@@ -26,8 +26,9 @@ Other workloads: ``parareal`` (that measurement alone, options ``--pr-*``),
 ``parareal-fine-step`` (the batched stepping throughput of its fine
 propagator: all slices' states, compensated FP32 RK4), ``swe-parareal``
 (shallow water on 1025^2, 64 slices), ``swe-step`` (its batched fine
-stepping throughput), ``surrogate`` (conv surrogate training throughput)
-and ``copy`` (the HBM counter calibration).  ``--ch-ics 1024`` runs the
+stepping throughput), ``surrogate`` (conv surrogate training throughput),
+``solution-y`` (``Solution.y`` / ``diff`` of the Parareal output on the
+device) and ``copy`` (the HBM counter calibration).  ``--ch-ics 1024`` runs the
 Cahn-Hilliard batch beyond the 256 MB MALL.
 """
 
@@ -59,7 +60,9 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="ch256",
                     choices=["ch256", "parareal", "parareal-fine-step", "swe-parareal",
-                             "swe-step", "copy", "surrogate"])
+                             "swe-step", "copy", "surrogate", "solution-y"])
+    ap.add_argument("--sol-points", type=int, default=1 << 22,
+                    help="query points of the solution-y workload")
     ap.add_argument("--ch-ics", type=int, default=256,
                     help="initial conditions of the Cahn-Hilliard batch (256: the configured "
                          "workload; 1024 puts the state beyond the 256 MB MALL)")
@@ -896,6 +899,96 @@ def parareal_fine_step_workload(args):
                    "dtype": prec})
 
 
+def solution_workload(args):
+    """Solution post-processing on the device (``DeviceSolution``): ``y`` of
+    the Parareal diffusion workload's output -- 65 snapshots (the initial
+    state and the 64 slice ends) of 513^2 vertices, FP64, in the fused
+    solvers' layout -- at ``--sol-points`` scattered points in and around the
+    domain (linear interpolation, one ``pml_interp_regular`` launch per
+    call), the cell-centre orientation change of the same stack (512^2
+    regular points), and ``diff`` against a second stack (65 matched
+    snapshots, one ``pml_snapshot_diff`` launch).  Roofline of the
+    interpolation: algorithmic bytes = the query points (16 B each) + the
+    output (8 B per value) + the snapshots read once (8 B per value).  The
+    CPU baseline is the oracle's restatement of the SciPy interpolation the
+    reference calls (one core, on the first points, scaled)."""
+    import torch
+
+    import pararealml_amd as pml
+    from pararealml_amd.solution import DeviceSolution
+
+    torch.cuda.set_device(0)
+    n, n_t, n_pts = args.pr_n + 1, args.pr_slices + 1, args.sol_points
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / (n - 1)] * 2)
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(2, 0.5), mesh, [(bc, bc)] * 2)
+    ivp = pml.InitialValueProblem(cp, (0.0, 1.0), pml.ContinuousInitialCondition(
+        cp, lambda x: np.zeros((len(x), 1))))
+    rng = np.random.RandomState(SEEDS[3])
+    host = rng.standard_normal((n_t, n, n, 1))
+    t = np.linspace(0.0, 1.0, n_t)
+    dev_y = torch.from_numpy(np.ascontiguousarray(np.moveaxis(host, -1, 1))).cuda()
+    sol = DeviceSolution(ivp, t, host, dev_y, vertex_oriented=True, d_t=t[1] - t[0],
+                         field_major=True)
+    other = DeviceSolution(ivp, t, host[::-1].copy(), dev_y.flip(0).contiguous(),
+                           vertex_oriented=True, d_t=t[1] - t[0], field_major=True)
+    x = torch.from_numpy(rng.uniform(-0.1, 1.1, (n_pts, 2))).cuda()
+    cells = torch.from_numpy(mesh.all_index_coordinates(False).reshape(-1, 2)).cuda()
+
+    def timed(fn, reps):
+        for _ in range(max(1, args.warmup // 100)):
+            fn()
+        torch.cuda.synchronize()
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            fn()
+        ev1.record()
+        torch.cuda.synchronize()
+        return ev0.elapsed_time(ev1) / 1e3 / reps
+
+    reps = max(1, args.steps // 100)
+    s_rand = timed(lambda: sol.y_device(x), reps)
+    s_cells = timed(lambda: sol.y_device(cells), reps)
+    s_diff = timed(lambda: sol.diff_device([other]), reps)
+
+    def roof(points, seconds):
+        algo = points * 16 + n_t * points * 8 + n_t * n * n * 8
+        achieved = algo / seconds / 1e9
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "algorithmic_bytes": algo}
+
+    diff_bytes = 3 * n_t * n * n * 8
+    out = {
+        "workload": "solution-y",
+        "config": {"snapshots": n_t, "vertices": f"{n}^2", "fields": 1, "dtype": "f64",
+                   "points": n_pts, "method": "linear", "layout": "field-major"},
+        "steps": reps,
+        "scattered": {"ms": s_rand * 1e3, "Gvalues_per_s": n_t * n_pts / s_rand / 1e9,
+                      "roofline": roof(n_pts, s_rand)},
+        "cell_centres": {"points": len(cells), "ms": s_cells * 1e3,
+                         "Gvalues_per_s": n_t * len(cells) / s_cells / 1e9,
+                         "roofline": roof(len(cells), s_cells)},
+        "diff": {"ms": s_diff * 1e3, "GB_per_s": diff_bytes / s_diff / 1e9,
+                 "frac": diff_bytes / s_diff / 1e9 / HBM_PEAK_GBS},
+    }
+    if not args.no_cpu_baseline:
+        from oracle import solution_numpy as osol
+
+        axes = mesh.axis_coordinates(True)
+        sample = min(n_pts, 1 << 21)
+        xs = x[:sample].cpu().numpy()
+        t0 = time.perf_counter()
+        osol.interp_regular(axes, host, xs)
+        cpu_s = (time.perf_counter() - t0) * n_pts / sample
+        out["cpu_baseline"] = {"value": n_t * n_pts / cpu_s / 1e9, "unit": "Gvalues/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{sample} of the {n_pts} points, all {n_t} "
+                                         "snapshots, scaled"}
+    print(json.dumps(out), flush=True)
+
+
 def copy_workload(args):
     """A plain device-to-device copy of 1 GiB, repeated: the calibration
     reference for the FETCH_SIZE / WRITE_SIZE counters (each dispatch moves
@@ -946,6 +1039,9 @@ def main():
         return
     if args.workload == "copy":
         copy_workload(args)
+        return
+    if args.workload == "solution-y":
+        solution_workload(args)
         return
     if args.workload == "surrogate":
         surrogate_workload(args)

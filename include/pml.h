@@ -406,6 +406,41 @@ int pml_ode_solve(int precision, pml_program program, int integrator_id,
                   int64_t snapshot_every, void *stream);
 
 /*
+ * Solution post-processing on the device.  A snapshot stack holds n_t
+ * snapshots of a C-order grid of shape[0..n_dims-1] points with n_fields
+ * values per point, channel-last ([point][field], the reference's layout)
+ * or, with field_major, [field][point] (the fused solvers' layout).
+ *   pml_interp_regular: values is such a stack (1 <= n_dims <= 3); coords holds the ascending axis
+ *     coordinates one axis after the other (shape[0] + ... values, FP64);
+ *     xi the n_points query points (n_points x n_dims, FP64); out receives
+ *     out[(t * n_points + p) * n_fields + f] in the precision of values.
+ *     Method PML_INTERP_LINEAR or PML_INTERP_NEAREST with SciPy's
+ *     RegularGridInterpolator semantics (scipy 1.15) under bounds_error
+ *     False and fill_value None: outside the grid the boundary interval
+ *     extrapolates, NaN coordinates give NaN; FP64 results equal SciPy's.
+ *     Replaces the interpn call of Solution.y (solution.py:114-145), which
+ *     Solution.discrete_y's orientation change also uses (solution.py:147-180).
+ *   pml_snapshot_diff: out[k] = b[ib[k]] - a[ia[k]] for k < n_pairs, two
+ *     stacks of one layout and `points` x n_fields values per snapshot
+ *     (FP32 or FP64), out channel-last; ia, ib are device int64 index
+ *     arrays, an index outside [0, n_a) or [0, n_b) gives NaNs for its pair.
+ *     Replaces the matched-time subtraction of Solution.diff
+ *     (solution.py:241-254).
+ */
+#define PML_INTERP_LINEAR 0
+#define PML_INTERP_NEAREST 1
+int pml_interp_regular(int precision, int method, int field_major,
+                       const void *values, int64_t n_t, int n_dims,
+                       const int64_t *shape, int n_fields,
+                       const double *coords, const double *xi,
+                       int64_t n_points, void *out, void *stream);
+int pml_snapshot_diff(int precision, int field_major, const void *a,
+                      const int64_t *ia, int64_t n_a, const void *b,
+                      const int64_t *ib, int64_t n_b, void *out,
+                      int64_t n_pairs, int64_t points, int n_fields,
+                      void *stream);
+
+/*
  * Convolutional surrogate (no counterpart in the reference; it replaces the
  * coarse propagator).  Tensors are NHWC BF16 unless stated; accumulation is
  * FP32; 3x3 "same" convolutions with zero padding.

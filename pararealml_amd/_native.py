@@ -42,6 +42,9 @@ PML_INT_FE = 0
 PML_INT_MIDPOINT = 1
 PML_INT_RK4 = 2
 
+PML_INTERP_LINEAR = 0
+PML_INTERP_NEAREST = 1
+
 
 class PmlGrid(ctypes.Structure):
     """``pml_grid`` of include/pml.h."""
@@ -96,6 +99,16 @@ SIGNATURES = {
         c_int,
         [c_int, c_void_p, c_int, c_int64, c_void_p, c_void_p, c_double, c_int64, c_void_p,
          c_void_p, c_int64, c_void_p],
+    ),
+    "pml_interp_regular": (
+        c_int,
+        [c_int, c_int, c_int, c_void_p, c_int64, c_int, POINTER(c_int64), c_int, c_void_p,
+         c_void_p, c_int64, c_void_p, c_void_p],
+    ),
+    "pml_snapshot_diff": (
+        c_int,
+        [c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,
+         c_int64, c_int64, c_int, c_void_p],
     ),
     "pml_fdm_configs": (c_int, [c_int, c_int, c_int]),
     "pml_fdm_config_name": (c_char_p, [c_int, c_int, c_int, c_int]),

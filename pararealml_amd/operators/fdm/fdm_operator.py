@@ -52,7 +52,7 @@ from pararealml_amd.operators.fdm.numerical_differentiator import (
     ThreePointCentralDifferenceMethod,
 )
 from pararealml_amd.operators.fdm.numerical_integrator import ImplicitMethod, NumericalIntegrator
-from pararealml_amd.solution import Solution
+from pararealml_amd.solution import DeviceSolution, Solution
 
 
 class _SymbolicDeviceRHS:
@@ -298,15 +298,19 @@ class FDMOperator(Operator):
                 tile_config=self.tile_config_for(problem),
                 use_graph=self._use_graph,
             )
-            flat = batch.snapshots_f64(snaps, n_snap).cpu().numpy()
+            dev_y = batch.snapshots_f64(snaps, n_snap)
+            flat = dev_y.cpu().numpy()
+        torch.cuda.current_stream().wait_stream(stream)
         flat = flat.reshape(n_snap, problem.n_fields, problem.n0, problem.n1)
         y = problem.from_device_layout(flat, cp.mesh.vertices_shape)
-        return Solution(
+        return DeviceSolution(
             ivp,
             t[every::every],
             y,
+            dev_y,
             vertex_oriented=True,
             d_t=self._d_t * every,
+            field_major=True,
         )
 
     def _ode_precision(self) -> int:
@@ -368,9 +372,11 @@ class FDMOperator(Operator):
                                         step.data_ptr(), float(self._d_t), int(n_steps),
                                         y.data_ptr(), snaps.data_ptr(), int(every),
                                         stream.cuda_stream))
-            out = snaps.view(n_snap, n).to(torch.float64).cpu().numpy()
+            dev_y = snaps.view(n_snap, n).to(torch.float64)
+            out = dev_y.cpu().numpy()
         torch.cuda.current_stream().wait_stream(stream)
-        return Solution(ivp, t[every::every], out, vertex_oriented=True, d_t=self._d_t * every)
+        return DeviceSolution(ivp, t[every::every], out, dev_y, vertex_oriented=True,
+                              d_t=self._d_t * every)
 
     def _solve_composed(self, ivp: InitialValueProblem) -> Solution:
         """Polar, cylindrical and spherical meshes, 3-D Cartesian ones and
@@ -829,8 +835,10 @@ class FDMOperator(Operator):
             cache.clear()
             if (step + 1) % every == 0:
                 snaps.append(y.clone())
-        out = torch.stack(snaps).to(torch.float64).cpu().numpy()
-        return Solution(ivp, t[every::every], out, vertex_oriented=True, d_t=dt * every)
+        dev_y = torch.stack(snaps).to(torch.float64)
+        out = dev_y.cpu().numpy()
+        return DeviceSolution(ivp, t[every::every], out, dev_y, vertex_oriented=True,
+                              d_t=dt * every)
 
     def _solve_with_poisson(self, ivp, problem, integ, t, n_steps, every, y0) -> Solution:
         """The time loop for equations with LHS.Y_LAPLACIAN components
@@ -883,6 +891,8 @@ class FDMOperator(Operator):
                 if (i + 1) % every == 0:
                     batch.state_f64(traj[(i + 1) // every - 1])
             flat = traj.cpu().numpy()
+        torch.cuda.current_stream().wait_stream(stream)
         flat = flat.reshape(n_snap, problem.n_fields, problem.n0, problem.n1)
         y = problem.from_device_layout(flat, mesh.vertices_shape)
-        return Solution(ivp, t[every::every], y, vertex_oriented=True, d_t=self._d_t * every)
+        return DeviceSolution(ivp, t[every::every], y, traj, vertex_oriented=True,
+                              d_t=self._d_t * every, field_major=True)

@@ -140,7 +140,7 @@ def _weights(src: "_Arr", coef, shape):
     import torch
 
     dtype = np.float64 if src.prec == nat.PML_F64 else np.float32
-    arr = np.ascontiguousarray(np.broadcast_to(coef, tuple(shape)), dtype=dtype)
+    arr = np.array(np.broadcast_to(coef, tuple(shape)), dtype=dtype)
     return torch.from_numpy(arr).to("cuda")
 
 

@@ -53,7 +53,7 @@ from pararealml_amd import _device as dev
 from pararealml_amd import _native as nat
 from pararealml_amd.initial_value_problem import InitialValueProblem
 from pararealml_amd.operator import Operator, discretize_time_domain
-from pararealml_amd.solution import Solution
+from pararealml_amd.solution import DeviceSolution, Solution
 
 TerminationCondition = Union[
     float, Sequence[float], Callable[[np.ndarray, np.ndarray], bool]
@@ -277,7 +277,8 @@ class PararealOperator(Operator):
 
             self.last_iterations = iterations
             if self._output == "endpoints":
-                shifted = (f_end + (u[1:] - f_end)).cpu().numpy()
+                dev_y = f_end + (u[1:] - f_end)
+                shifted = dev_y.cpu().numpy()
                 y = problem.from_device_layout(
                     shifted.reshape(n, n_fields, problem.n0, problem.n1),
                     spatial,
@@ -287,9 +288,9 @@ class PararealOperator(Operator):
             else:
                 traj = fine.trajectory_f64()  # (steps, n, elems)
                 shift = (u[1:] - traj[-1]).unsqueeze(0)
-                traj = (traj + shift).permute(1, 0, 2).reshape(n * f_steps, elems)
+                dev_y = (traj + shift).permute(1, 0, 2).reshape(n * f_steps, elems)
                 y = problem.from_device_layout(
-                    traj.cpu().numpy().reshape(n * f_steps, n_fields, problem.n0, problem.n1),
+                    dev_y.cpu().numpy().reshape(n * f_steps, n_fields, problem.n0, problem.n1),
                     spatial,
                 )
                 t = discretize_time_domain(ivp.t_interval, f.d_t)[1:]
@@ -298,7 +299,8 @@ class PararealOperator(Operator):
         self.last_stats = dict(
             timings, iterations=iterations, update_norms=norms_hist, iteration_wall_s=iter_wall
         )
-        return Solution(ivp, t, y, vertex_oriented=self._vertex_oriented, d_t=d_t_out)
+        return DeviceSolution(ivp, t, y, dev_y, vertex_oriented=self._vertex_oriented,
+                              d_t=d_t_out, field_major=True)
 
 
 _LINEAR_FAMILIES = ("diffusion", "convection_diffusion", "wave")

@@ -1,0 +1,258 @@
+"""Solution post-processing on the device (DeviceSolution: pml_interp_regular
+and pml_snapshot_diff) against the oracle restatement of
+pararealml/solution.py:114-258, which is pinned bit for bit to the
+scipy.interpolate.interpn call the reference makes.  FP64 results are
+compared for exact equality."""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+import pararealml_amd as pml
+from oracle import solution_numpy as osol
+from pararealml_amd import _native as nat
+from pararealml_amd.mesh import CoordinateSystem, Mesh
+from pararealml_amd.operators.fdm import (
+    RK4,
+    FDMOperator,
+    ForwardEulerMethod,
+    ThreePointCentralDifferenceMethod,
+)
+from pararealml_amd.solution import DeviceSolution, Solution
+
+pytestmark = pytest.mark.gpu
+
+DIFF = ThreePointCentralDifferenceMethod()
+
+
+def _queries(mesh, n, seed, pad=0.25):
+    """Random points inside and around the mesh, every vertex of a diagonal,
+    the upper corner and one NaN coordinate."""
+    rng = np.random.default_rng(seed)
+    lo = np.array([iv[0] for iv in mesh.x_intervals])
+    hi = np.array([iv[1] for iv in mesh.x_intervals])
+    span = hi - lo
+    x = lo - pad * span + (1 + 2 * pad) * span * rng.random((n, len(lo)))
+    coords = mesh.axis_coordinates(True)
+    m = min(len(c) for c in coords)
+    x[:m] = np.stack([c[:m] for c in coords], axis=-1)
+    x[m] = hi
+    x[m + 1, 0] = np.nan
+    return x
+
+
+def _check_y(sol, x, method="linear"):
+    cp = sol.initial_value_problem.constrained_problem
+    want = osol.interp_regular(cp.mesh.axis_coordinates(True), sol.discrete_y(), x, method)
+    got = sol.y(x, method)
+    assert got.shape == want.shape
+    np.testing.assert_array_equal(got, want)
+
+
+def _diffusion(n=33, d=0.2):
+    mesh = Mesh([(0.0, 1.0), (0.0, 2.0)], [1.0 / (n - 1), 2.0 / (n - 1)])
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(2, d), mesh, [(bc, bc)] * 2)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.4, 1.1]), 0.02 * np.eye(2))])
+    return cp, ic
+
+
+@pytest.mark.parametrize("method", ["linear", "nearest"])
+def test_fused_2d_solution_interpolates_as_scipy(method):
+    print(f"case: 2-d diffusion y {method}", flush=True)
+    cp, ic = _diffusion()
+    sol = FDMOperator(RK4(), DIFF, 1e-4).solve(pml.InitialValueProblem(cp, (0.0, 0.01), ic))
+    assert isinstance(sol, DeviceSolution) and sol.field_major
+    x = _queries(cp.mesh, 3000, 1).reshape(30, 100, 2)
+    _check_y(sol, x, method)
+    # slinear is evaluated as linear
+    np.testing.assert_allclose(sol.y(x, "slinear"), sol.y(x, "linear"), rtol=0, atol=0)
+    with pytest.raises(NotImplementedError):
+        sol.y(x, "cubic")
+    with pytest.raises(ValueError):
+        sol.y(np.zeros((4, 3)))
+
+
+def test_multi_field_1d_3d_and_polar_solutions():
+    print("case: shallow water, 1-d, 3-d and polar y", flush=True)
+    # three fields, fused kernels (field-major device layout)
+    mesh = Mesh([(0.0, 1.0), (0.0, 1.0)], [0.05, 0.05])
+    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.ShallowWaterEquation(0.5), mesh, [(bc, bc)] * 2)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.5, 0.5]), 0.01 * np.eye(2))] * 3,
+                                      [1.0, 0.0, 0.0])
+    sol = FDMOperator(RK4(), DIFF, 1e-3).solve(pml.InitialValueProblem(cp, (0.0, 0.02), ic))
+    assert sol.field_major and sol.discrete_y().shape[-1] == 3
+    _check_y(sol, _queries(mesh, 1500, 2))
+    _check_y(sol, _queries(mesh, 500, 3), "nearest")
+
+    # 1-D (the fused kernels' n0 = 1 layout)
+    mesh = Mesh([(0.0, 2.0)], [0.05])
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(1, 0.1), mesh, [(bc, bc)])
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.8]), np.array([[0.05]]))])
+    sol = FDMOperator(RK4(), DIFF, 1e-3).solve(pml.InitialValueProblem(cp, (0.0, 0.05), ic))
+    _check_y(sol, _queries(mesh, 700, 4))
+
+    # 3-D Cartesian (composed loop, channel-last device layout)
+    mesh = Mesh([(0.0, 1.0)] * 3, [0.125] * 3)
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(3, 0.2), mesh, [(bc, bc)] * 3)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.4, 0.5, 0.6]), 0.02 * np.eye(3))])
+    sol = FDMOperator(ForwardEulerMethod(), DIFF, 1e-3).solve(
+        pml.InitialValueProblem(cp, (0.0, 0.005), ic))
+    assert isinstance(sol, DeviceSolution) and not sol.field_major
+    _check_y(sol, _queries(mesh, 2000, 5))
+    _check_y(sol, _queries(mesh, 2000, 6), "nearest")
+
+    # polar mesh: interpolation in (r, theta) coordinates
+    mesh = Mesh([(1.0, 3.0), (0.0, 1.5)], [0.1, 0.1], CoordinateSystem.POLAR)
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.full((len(x), 1), 0.5), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(2, 0.3), mesh, [(bc, bc)] * 2)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([2.0, 0.7]), 0.1 * np.eye(2))])
+    sol = FDMOperator(RK4(), DIFF, 1e-3).solve(pml.InitialValueProblem(cp, (0.0, 0.01), ic))
+    _check_y(sol, _queries(mesh, 1500, 7))
+
+
+def test_cell_centre_orientation_and_diff():
+    print("case: discrete_y(False) and diff", flush=True)
+    cp, ic = _diffusion(n=21)
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.02), ic)
+    a = FDMOperator(RK4(), DIFF, 2e-3).solve(ivp)
+    b = FDMOperator(RK4(), DIFF, 1e-3).solve(ivp)
+    cells = a.discrete_y(False)
+    want = osol.interp_regular(cp.mesh.axis_coordinates(True), a.discrete_y(),
+                               cp.mesh.all_index_coordinates(False))
+    np.testing.assert_array_equal(cells, want)
+
+    # device other, host other, and no match
+    host_b = Solution(ivp, b.t_coordinates, b.discrete_y(), vertex_oriented=True, d_t=b.d_t)
+    diffs = a.diff([b, host_b])
+    times, want = osol.diff(a.discrete_y(), [b.discrete_y()] * 2,
+                            [a.t_coordinates, b.t_coordinates, b.t_coordinates],
+                            [a.d_t, b.d_t, b.d_t])
+    np.testing.assert_array_equal(diffs.matching_time_points, times)
+    assert len(times) == 10
+    for got in diffs.differences:
+        np.testing.assert_array_equal(got, want[0])
+    np.testing.assert_array_equal(b.diff([a]).differences[0], -want[0])
+    shifted = Solution(ivp, a.t_coordinates + 1e-3, a.discrete_y(), vertex_oriented=True,
+                       d_t=a.d_t)
+    empty = a.diff([shifted])
+    assert empty.matching_time_points.shape == (0,) and empty.differences[0].shape == (0,)
+    with pytest.raises(ValueError):
+        a.diff([])
+
+
+def test_parareal_and_ode_solutions():
+    print("case: parareal and ode device solutions", flush=True)
+    from pararealml_amd.operators.parareal import PararealOperator
+
+    cp, ic = _diffusion(n=17)
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.04), ic)
+    f = FDMOperator(RK4(), DIFF, 1e-4)
+    g = FDMOperator(ForwardEulerMethod(), DIFF, 1e-3)
+    sol = PararealOperator(f, g, 1e-6, n_slices=4).solve(ivp)
+    assert isinstance(sol, DeviceSolution)
+    _check_y(sol, _queries(cp.mesh, 800, 8))
+
+    cp = pml.ConstrainedProblem(pml.LorenzEquation())
+    ic = pml.ContinuousInitialCondition(cp, lambda _: np.array([1.0, 1.0, 1.0]))
+    ivp = pml.InitialValueProblem(cp, (0.0, 1.0), ic)
+    a = FDMOperator(RK4(), DIFF, 0.01).solve(ivp)
+    b = FDMOperator(RK4(), DIFF, 0.005).solve(ivp)
+    assert isinstance(a, DeviceSolution)
+    np.testing.assert_array_equal(a.y(), a.discrete_y())
+    times, want = osol.diff(a.discrete_y(), [b.discrete_y()],
+                            [a.t_coordinates, b.t_coordinates], [a.d_t, b.d_t])
+    got = a.diff([b])
+    np.testing.assert_array_equal(got.matching_time_points, times)
+    np.testing.assert_array_equal(got.differences[0], want[0])
+
+
+def test_c_abi_fp32_and_edge_cases():
+    import torch
+
+    print("case: interpolation / difference c-abi edge cases", flush=True)
+    lib = nat.device_lib()
+    rng = np.random.default_rng(9)
+    axes = [np.linspace(0.0, 1.0, 9), np.linspace(-1.0, 1.0, 7)]
+    vals = rng.standard_normal((3, 9, 7, 2))
+    x = rng.uniform(-0.5, 1.5, (400, 2))
+    dev_v = torch.from_numpy(vals.astype(np.float32)).cuda()
+    coords = torch.from_numpy(np.concatenate(axes)).cuda()
+    xt = torch.from_numpy(x).cuda()
+    out = torch.empty((3, 400, 2), dtype=torch.float32, device="cuda")
+    shape = (ctypes.c_int64 * 2)(9, 7)
+    s = torch.cuda.current_stream().cuda_stream
+    nat.check(lib.pml_interp_regular(nat.PML_F32, nat.PML_INTERP_LINEAR, 0, dev_v.data_ptr(),
+                                     3, 2, shape, 2, coords.data_ptr(), xt.data_ptr(), 400,
+                                     out.data_ptr(), s))
+    want = osol.interp_regular(axes, vals.astype(np.float32).astype(np.float64), x)
+    np.testing.assert_allclose(out.cpu().numpy(), want, rtol=1e-6, atol=1e-6)
+    # no points: nothing to do
+    nat.check(lib.pml_interp_regular(nat.PML_F64, nat.PML_INTERP_LINEAR, 0, dev_v.data_ptr(),
+                                     3, 2, shape, 2, coords.data_ptr(), 0, 0, 0, s))
+    with pytest.raises(NotImplementedError):
+        nat.check(lib.pml_interp_regular(nat.PML_F64, 7, 0, dev_v.data_ptr(), 3, 2, shape, 2,
+                                         coords.data_ptr(), xt.data_ptr(), 400,
+                                         out.data_ptr(), s))
+    with pytest.raises(ValueError):
+        nat.check(lib.pml_interp_regular(nat.PML_F64, nat.PML_INTERP_LINEAR, 0,
+                                         dev_v.data_ptr(), 3, 4, shape, 2, coords.data_ptr(),
+                                         xt.data_ptr(), 400, out.data_ptr(), s))
+    # an index outside the stack gives a NaN row, the other rows are exact
+    a = torch.from_numpy(rng.standard_normal((4, 6, 2))).cuda()
+    b = torch.from_numpy(rng.standard_normal((5, 6, 2))).cuda()
+    ia = torch.tensor([0, 3, 1], dtype=torch.int64, device="cuda")
+    ib = torch.tensor([4, 2, 5], dtype=torch.int64, device="cuda")
+    d = torch.empty((3, 6, 2), dtype=torch.float64, device="cuda")
+    nat.check(lib.pml_snapshot_diff(nat.PML_F64, 0, a.data_ptr(), ia.data_ptr(), 4,
+                                    b.data_ptr(), ib.data_ptr(), 5, d.data_ptr(), 3, 6, 2, s))
+    d = d.cpu().numpy()
+    an, bn = a.cpu().numpy(), b.cpu().numpy()
+    np.testing.assert_array_equal(d[0], bn[4] - an[0])
+    np.testing.assert_array_equal(d[1], bn[2] - an[3])
+    assert np.all(np.isnan(d[2]))
+    # field-major inputs give channel-last differences
+    af = torch.from_numpy(np.ascontiguousarray(np.moveaxis(an, -1, 1))).cuda()
+    bf = torch.from_numpy(np.ascontiguousarray(np.moveaxis(bn, -1, 1))).cuda()
+    d2 = torch.empty((2, 6, 2), dtype=torch.float64, device="cuda")
+    nat.check(lib.pml_snapshot_diff(nat.PML_F64, 1, af.data_ptr(), ia.data_ptr(), 4,
+                                    bf.data_ptr(), ib.data_ptr(), 5, d2.data_ptr(), 2, 6, 2, s))
+    np.testing.assert_array_equal(d2.cpu().numpy(), d[:2])
+
+
+def test_full_size_vertex_identity_and_bilinear_reproduction():
+    """At the 65 x 513^2 size of the Parareal benchmark's output: the
+    vertices themselves return the snapshots exactly, and a bilinear field
+    is reproduced at scattered points inside the grid and up to ten cells
+    outside it (further out, the extrapolation weights grow as the squared
+    distance in cells and the rounding of the corner sum with them)."""
+    import torch
+
+    print("case: 65 x 513^2 interpolation properties", flush=True)
+    n, n_t = 513, 65
+    mesh = Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / (n - 1)] * 2)
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(2, 0.5), mesh, [(bc, bc)] * 2)
+    ic = pml.ContinuousInitialCondition(cp, lambda x: np.zeros((len(x), 1)))
+    ivp = pml.InitialValueProblem(cp, (0.0, 1.0), ic)
+    g0, g1 = np.meshgrid(*mesh.axis_coordinates(True), indexing="ij")
+    k = np.arange(n_t)[:, None, None]
+    field = (0.5 + 0.1 * k) + (1.0 - 0.01 * k) * g0 - 0.3 * g1 + (2.0 + 0.02 * k) * g0 * g1
+    field = field[..., None]
+    dev_y = torch.from_numpy(np.ascontiguousarray(field)).cuda()
+    sol = DeviceSolution(ivp, np.linspace(1.0 / 64, 1.0, n_t), field, dev_y,
+                         vertex_oriented=True, d_t=1.0 / 64)
+    vertices = mesh.all_index_coordinates(True)
+    got = sol.y_device(vertices)
+    assert torch.equal(got, dev_y)
+    rng = np.random.default_rng(11)
+    x = rng.uniform(-0.02, 1.02, (1 << 18, 2))
+    got = sol.y(x)[..., 0]
+    kk = np.arange(n_t)[:, None]
+    want = ((0.5 + 0.1 * kk) + (1.0 - 0.01 * kk) * x[None, :, 0] - 0.3 * x[None, :, 1]
+            + (2.0 + 0.02 * kk) * x[None, :, 0] * x[None, :, 1])
+    np.testing.assert_allclose(got, want, rtol=0, atol=1e-12)
