@@ -909,8 +909,10 @@ def solution_workload(args):
     regular points), and ``diff`` against a second stack (65 matched
     snapshots, one ``pml_snapshot_diff`` launch).  Roofline of the
     interpolation: algorithmic bytes = the query points (16 B each) + the
-    output (8 B per value) + the snapshots read once (8 B per value).  The
-    CPU baseline is the oracle's restatement of the SciPy interpolation the
+    output (8 B per value) + the snapshots read once (8 B per value); the
+    time-inner copy of the stack is made once before the timed calls
+    (``in_place_ms``: the same calls reading the solver's layout directly).
+    The CPU baseline is the oracle's restatement of the SciPy interpolation the
     reference calls (one core, on the first points, scaled)."""
     import torch
 
@@ -934,6 +936,7 @@ def solution_workload(args):
                            vertex_oriented=True, d_t=t[1] - t[0], field_major=True)
     x = torch.from_numpy(rng.uniform(-0.1, 1.1, (n_pts, 2))).cuda()
     cells = torch.from_numpy(mesh.all_index_coordinates(False).reshape(-1, 2)).cuda()
+    sol._time_inner_stack()  # the one-time copy, outside the timed calls
 
     def timed(fn, reps):
         for _ in range(max(1, args.warmup // 100)):
@@ -949,8 +952,10 @@ def solution_workload(args):
         return ev0.elapsed_time(ev1) / 1e3 / reps
 
     reps = max(1, args.steps // 100)
-    s_rand = timed(lambda: sol.y_device(x), reps)
-    s_cells = timed(lambda: sol.y_device(cells), reps)
+    s_rand = timed(lambda: sol._interp(x, "linear", True), reps)
+    s_rand_in_place = timed(lambda: sol._interp(x, "linear", False), reps)
+    s_cells = timed(lambda: sol._interp(cells, "linear", True), reps)
+    s_cells_in_place = timed(lambda: sol._interp(cells, "linear", False), reps)
     s_diff = timed(lambda: sol.diff_device([other]), reps)
 
     def roof(points, seconds):
@@ -966,10 +971,12 @@ def solution_workload(args):
                    "points": n_pts, "method": "linear", "layout": "field-major"},
         "steps": reps,
         "scattered": {"ms": s_rand * 1e3, "Gvalues_per_s": n_t * n_pts / s_rand / 1e9,
-                      "roofline": roof(n_pts, s_rand)},
+                      "roofline": roof(n_pts, s_rand),
+                      "in_place_ms": s_rand_in_place * 1e3},
         "cell_centres": {"points": len(cells), "ms": s_cells * 1e3,
                          "Gvalues_per_s": n_t * len(cells) / s_cells / 1e9,
-                         "roofline": roof(len(cells), s_cells)},
+                         "roofline": roof(len(cells), s_cells),
+                         "in_place_ms": s_cells_in_place * 1e3},
         "diff": {"ms": s_diff * 1e3, "GB_per_s": diff_bytes / s_diff / 1e9,
                  "frac": diff_bytes / s_diff / 1e9 / HBM_PEAK_GBS},
     }

@@ -408,10 +408,17 @@ int pml_ode_solve(int precision, pml_program program, int integrator_id,
 /*
  * Solution post-processing on the device.  A snapshot stack holds n_t
  * snapshots of a C-order grid of shape[0..n_dims-1] points with n_fields
- * values per point, channel-last ([point][field], the reference's layout)
- * or, with field_major, [field][point] (the fused solvers' layout).
- *   pml_interp_regular: values is such a stack (1 <= n_dims <= 3); coords holds the ascending axis
- *     coordinates one axis after the other (shape[0] + ... values, FP64);
+ * values per point, in one of three layouts:
+ *   PML_LAYOUT_CHANNEL_LAST  [t][point][field] (the reference's layout),
+ *   PML_LAYOUT_FIELD_MAJOR   [t][field][point] (the fused solvers' layout),
+ *   PML_LAYOUT_TIME_INNER    [point][field][t] (every point's history
+ *                            contiguous: the layout scattered queries read
+ *                            fastest).
+ *   pml_snapshots_time_inner: copies a channel-last or field-major stack
+ *     (layout) of `points` grid points into the time-inner layout (out).
+ *   pml_interp_regular: values is such a stack (1 <= n_dims <= 3) in
+ *     `layout`; coords holds the ascending axis coordinates one axis after
+ *     the other (shape[0] + ... values, FP64);
  *     xi the n_points query points (n_points x n_dims, FP64); out receives
  *     out[(t * n_points + p) * n_fields + f] in the precision of values.
  *     Method PML_INTERP_LINEAR or PML_INTERP_NEAREST with SciPy's
@@ -421,7 +428,8 @@ int pml_ode_solve(int precision, pml_program program, int integrator_id,
  *     Replaces the interpn call of Solution.y (solution.py:114-145), which
  *     Solution.discrete_y's orientation change also uses (solution.py:147-180).
  *   pml_snapshot_diff: out[k] = b[ib[k]] - a[ia[k]] for k < n_pairs, two
- *     stacks of one layout and `points` x n_fields values per snapshot
+ *     stacks of one layout (field_major: PML_LAYOUT_FIELD_MAJOR, else
+ *     channel-last) and `points` x n_fields values per snapshot
  *     (FP32 or FP64), out channel-last; ia, ib are device int64 index
  *     arrays, an index outside [0, n_a) or [0, n_b) gives NaNs for its pair.
  *     Replaces the matched-time subtraction of Solution.diff
@@ -429,7 +437,13 @@ int pml_ode_solve(int precision, pml_program program, int integrator_id,
  */
 #define PML_INTERP_LINEAR 0
 #define PML_INTERP_NEAREST 1
-int pml_interp_regular(int precision, int method, int field_major,
+#define PML_LAYOUT_CHANNEL_LAST 0
+#define PML_LAYOUT_FIELD_MAJOR 1
+#define PML_LAYOUT_TIME_INNER 2
+int pml_snapshots_time_inner(int precision, int layout, const void *values,
+                             int64_t n_t, int64_t points, int n_fields,
+                             void *out, void *stream);
+int pml_interp_regular(int precision, int method, int layout,
                        const void *values, int64_t n_t, int n_dims,
                        const int64_t *shape, int n_fields,
                        const double *coords, const double *xi,

@@ -44,6 +44,9 @@ PML_INT_RK4 = 2
 
 PML_INTERP_LINEAR = 0
 PML_INTERP_NEAREST = 1
+PML_LAYOUT_CHANNEL_LAST = 0
+PML_LAYOUT_FIELD_MAJOR = 1
+PML_LAYOUT_TIME_INNER = 2
 
 
 class PmlGrid(ctypes.Structure):
@@ -99,6 +102,9 @@ SIGNATURES = {
         c_int,
         [c_int, c_void_p, c_int, c_int64, c_void_p, c_void_p, c_double, c_int64, c_void_p,
          c_void_p, c_int64, c_void_p],
+    ),
+    "pml_snapshots_time_inner": (
+        c_int, [c_int, c_int, c_void_p, c_int64, c_int64, c_int, c_void_p, c_void_p]
     ),
     "pml_interp_regular": (
         c_int,

@@ -210,6 +210,7 @@ class DeviceSolution(Solution):
             ivp.constrained_problem.differential_equation.x_dimension
         )
         self._coords = None
+        self._time_inner = None
 
     @property
     def device_y(self):
@@ -228,7 +229,36 @@ class DeviceSolution(Solution):
         return points, c
 
     def y_device(self, x, interpolation_method: str = "linear"):
-        """``y(x)`` as a device tensor ``(n_t, *x.shape[:-1], y_dim)``."""
+        """``y(x)`` as a device tensor ``(n_t, *x.shape[:-1], y_dim)``.
+
+        When the output is at least as large as the snapshot stack (and
+        there are 16 snapshots or more), the stack is first copied once
+        into the time-inner layout (kept with the solution), in which every
+        corner read of a query point is one contiguous run of its
+        snapshots; smaller queries read the stack in place.  Both paths
+        give identical values."""
+        return self._interp(x, interpolation_method, None)
+
+    def _time_inner_stack(self):
+        import torch
+
+        from pararealml_amd import _native as nat
+
+        if self._time_inner is None:
+            points, c = self._layout()
+            n_t = len(self._t_coordinates)
+            vt = torch.empty_like(self._device_y)
+            stream = torch.cuda.current_stream(self._device_y.device)
+            self._device_y.record_stream(stream)
+            layout = (nat.PML_LAYOUT_FIELD_MAJOR if self._field_major
+                      else nat.PML_LAYOUT_CHANNEL_LAST)
+            nat.check(nat.device_lib().pml_snapshots_time_inner(
+                nat.PML_F64, layout, self._device_y.data_ptr(), n_t, points, c,
+                vt.data_ptr(), stream.cuda_stream))
+            self._time_inner = vt
+        return self._time_inner
+
+    def _interp(self, x, interpolation_method: str, time_inner: Optional[bool]):
         import ctypes
 
         import torch
@@ -261,15 +291,23 @@ class DeviceSolution(Solution):
             self._coords = torch.from_numpy(flat).to(device)
         shape = (ctypes.c_int64 * d)(*[len(a) for a in axes])
         n_t = len(self._t_coordinates)
-        _, c = self._layout()
+        points, c = self._layout()
+        if time_inner is None:
+            time_inner = n_t >= 16 and pts.shape[0] >= points
+        if time_inner:
+            values, layout = self._time_inner_stack(), nat.PML_LAYOUT_TIME_INNER
+        elif self._field_major:
+            values, layout = self._device_y, nat.PML_LAYOUT_FIELD_MAJOR
+        else:
+            values, layout = self._device_y, nat.PML_LAYOUT_CHANNEL_LAST
         out = torch.empty((n_t, pts.shape[0], c), dtype=torch.float64, device=device)
         stream = torch.cuda.current_stream(device)
-        self._device_y.record_stream(stream)
+        values.record_stream(stream)
         nat.check(
             lib.pml_interp_regular(
-                nat.PML_F64, method, int(self._field_major), self._device_y.data_ptr(), n_t,
-                d, shape, c, self._coords.data_ptr(), pts.data_ptr(), pts.shape[0],
-                out.data_ptr(), stream.cuda_stream,
+                nat.PML_F64, method, layout, values.data_ptr(), n_t, d, shape, c,
+                self._coords.data_ptr(), pts.data_ptr(), pts.shape[0], out.data_ptr(),
+                stream.cuda_stream,
             )
         )
         return out.view((n_t,) + tuple(xt.shape[:-1]) + (c,))

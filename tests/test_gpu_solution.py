@@ -171,6 +171,43 @@ def test_parareal_and_ode_solutions():
     np.testing.assert_array_equal(got.differences[0], want[0])
 
 
+@pytest.mark.parametrize("method,n_snap", [("linear", 70), ("nearest", 70), ("linear", 150)])
+def test_time_inner_layout_equals_in_place(method, n_snap):
+    """Both device paths (the stack read in place, and its time-inner copy
+    read by the LDS-tiled kernel) give the oracle's values bit for bit:
+    three fields in the field-major layout over 70 snapshots (one snapshot
+    tile) or 150 (two tiles of 75), one field channel-last in 3-D, ragged
+    point counts."""
+    print(f"case: time-inner vs in-place {method} {n_snap}", flush=True)
+    mesh = Mesh([(0.0, 1.0), (0.0, 1.0)], [0.05, 0.05])
+    bc = pml.NeumannBoundaryCondition(lambda x, t: np.zeros((len(x), 3)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.ShallowWaterEquation(0.5), mesh, [(bc, bc)] * 2)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.5, 0.5]), 0.01 * np.eye(2))] * 3,
+                                      [1.0, 0.0, 0.0])
+    sol = FDMOperator(RK4(), DIFF, 1e-3).solve(
+        pml.InitialValueProblem(cp, (0.0, n_snap * 1e-3), ic))
+    assert len(sol.t_coordinates) == n_snap and sol.field_major
+    axes = mesh.axis_coordinates(True)
+    for n_pts in (1, 63, 65, 1000):
+        x = _queries(mesh, max(n_pts, 30), n_pts)[:n_pts]
+        want = osol.interp_regular(axes, sol.discrete_y(), x, method)
+        for ti in (False, True):
+            got = sol._interp(x, method, ti).cpu().numpy()
+            np.testing.assert_array_equal(got, want)
+
+    mesh = Mesh([(0.0, 1.0)] * 3, [0.125] * 3)
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(3, 0.2), mesh, [(bc, bc)] * 3)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.4, 0.5, 0.6]), 0.02 * np.eye(3))])
+    sol = FDMOperator(ForwardEulerMethod(), DIFF, 1e-3).solve(
+        pml.InitialValueProblem(cp, (0.0, 0.02), ic))
+    assert not sol.field_major
+    x = _queries(mesh, 777, 12)
+    want = osol.interp_regular(mesh.axis_coordinates(True), sol.discrete_y(), x, method)
+    for ti in (False, True):
+        np.testing.assert_array_equal(sol._interp(x, method, ti).cpu().numpy(), want)
+
+
 def test_c_abi_fp32_and_edge_cases():
     import torch
 
@@ -222,6 +259,20 @@ def test_c_abi_fp32_and_edge_cases():
     nat.check(lib.pml_snapshot_diff(nat.PML_F64, 1, af.data_ptr(), ia.data_ptr(), 4,
                                     bf.data_ptr(), ib.data_ptr(), 5, d2.data_ptr(), 2, 6, 2, s))
     np.testing.assert_array_equal(d2.cpu().numpy(), d[:2])
+    # time-inner copies of both layouts, FP32, 70 snapshots
+    st = rng.standard_normal((70, 50, 3)).astype(np.float32)
+    vt = torch.empty(70 * 50 * 3, dtype=torch.float32, device="cuda")
+    want = np.ascontiguousarray(np.moveaxis(st, 0, -1))
+    for layout, arr in ((nat.PML_LAYOUT_CHANNEL_LAST, st),
+                        (nat.PML_LAYOUT_FIELD_MAJOR, np.moveaxis(st, -1, 1))):
+        src = torch.from_numpy(np.ascontiguousarray(arr)).cuda()
+        vt.zero_()
+        nat.check(lib.pml_snapshots_time_inner(nat.PML_F32, layout, src.data_ptr(), 70, 50, 3,
+                                               vt.data_ptr(), s))
+        np.testing.assert_array_equal(vt.cpu().numpy().reshape(50, 3, 70), want)
+    with pytest.raises(ValueError):
+        nat.check(lib.pml_snapshots_time_inner(nat.PML_F32, nat.PML_LAYOUT_TIME_INNER,
+                                               src.data_ptr(), 70, 50, 3, vt.data_ptr(), s))
 
 
 def test_full_size_vertex_identity_and_bilinear_reproduction():
