@@ -455,6 +455,24 @@ int pml_snapshot_diff(int precision, int field_major, const void *a,
                       void *stream);
 
 /*
+ * Rows of the supervised-learning data set (FP64, device arrays), as
+ * SupervisedMLOperator._generate_data lays them out
+ * (operators/ml/supervised/supervised_ml_operator.py:362-379, 440-499).
+ * n_blocks = iterations x n_steps blocks (iteration-major); block b's
+ * recorded state is states[b / steps_per_state] and its next state
+ * nexts[b], both field-major (n_fields x points).  Row r = b * points + p
+ * of `inputs` (width n_fields * points + (tcol != NULL) + x_dim) is the
+ * recorded state flattened channel-last, then tcol[b % n_steps] (time or
+ * step column, omitted when tcol is NULL), then coords[p] (x_dim values);
+ * row r of `targets` (n_fields wide) is the next state at p.
+ */
+int pml_supervised_rows(const double *states, int64_t steps_per_state,
+                        const double *nexts, int64_t n_blocks, int64_t n_steps,
+                        int64_t points, int n_fields, const double *tcol,
+                        const double *coords, int x_dim, double *inputs,
+                        double *targets, void *stream);
+
+/*
  * Convolutional surrogate (no counterpart in the reference; it replaces the
  * coarse propagator).  Tensors are NHWC BF16 unless stated; accumulation is
  * FP32; 3x3 "same" convolutions with zero padding.

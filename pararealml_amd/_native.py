@@ -116,6 +116,11 @@ SIGNATURES = {
         [c_int, c_int, c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_int64, c_void_p,
          c_int64, c_int64, c_int, c_void_p],
     ),
+    "pml_supervised_rows": (
+        c_int,
+        [c_void_p, c_int64, c_void_p, c_int64, c_int64, c_int64, c_int, c_void_p, c_void_p,
+         c_int, c_void_p, c_void_p, c_void_p],
+    ),
     "pml_fdm_configs": (c_int, [c_int, c_int, c_int]),
     "pml_fdm_config_name": (c_char_p, [c_int, c_int, c_int, c_int]),
     "pml_stencil_first": (

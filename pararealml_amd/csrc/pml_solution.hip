@@ -430,3 +430,63 @@ extern "C" int pml_snapshots_time_inner(int precision, int layout, const void *v
   PML_LAUNCH_CHECK();
   return PML_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Rows of the supervised-learning data set (supervised_ml_operator.py:
+// 362-379, 440-499): one workgroup per row r = b * points + p of block b
+// (iteration-major, time step minor); the row is the recorded state of the
+// block flattened channel-last, then the block's time column (when tcol is
+// given), then the point's coordinates; its target is the next state at p.
+namespace pml {
+namespace {
+
+constexpr int kRowBlock = 256;
+
+__global__ __launch_bounds__(kRowBlock) void k_supervised_rows(
+    const double *__restrict__ states, int64_t steps_per_state,
+    const double *__restrict__ nexts, int64_t n_rows, int64_t n_steps, int64_t points, int C,
+    const double *__restrict__ tcol, const double *__restrict__ coords, int x_dim,
+    double *__restrict__ inputs, double *__restrict__ targets) {
+  const int64_t n_y = points * C;
+  const int64_t width = n_y + (tcol ? 1 : 0) + x_dim;
+  for (int64_t r = blockIdx.x; r < n_rows; r += gridDim.x) {
+    const int64_t b = r / points, p = r - b * points;
+    const double *st = states + (b / steps_per_state) * n_y;
+    double *row = inputs + r * width;
+    if (C == 1) {
+      for (int64_t col = threadIdx.x; col < n_y; col += kRowBlock) row[col] = st[col];
+    } else {
+      for (int64_t col = threadIdx.x; col < n_y; col += kRowBlock) {
+        const int64_t q = col / C, c = col - q * C;
+        row[col] = st[c * points + q];
+      }
+    }
+    if (threadIdx.x < C) targets[r * C + threadIdx.x] = nexts[(b * C + threadIdx.x) * points + p];
+    if (tcol && threadIdx.x == 0) row[n_y] = tcol[b % n_steps];
+    if ((int)threadIdx.x < x_dim)
+      row[n_y + (tcol ? 1 : 0) + threadIdx.x] = coords[p * x_dim + threadIdx.x];
+  }
+}
+
+}  // namespace
+}  // namespace pml
+
+extern "C" int pml_supervised_rows(const double *states, int64_t steps_per_state,
+                                   const double *nexts, int64_t n_blocks, int64_t n_steps,
+                                   int64_t points, int n_fields, const double *tcol,
+                                   const double *coords, int x_dim, double *inputs,
+                                   double *targets, void *stream) {
+  if (n_blocks < 0 || n_steps < 1 || points < 1 || n_fields < 1 || n_fields > kRowBlock ||
+      x_dim < 0 || x_dim > kRowBlock || steps_per_state < 1 || n_blocks % n_steps != 0)
+    return fail(PML_E_INVALID, "bad supervised-row arguments");
+  if (n_blocks == 0) return PML_OK;
+  if (!states || !nexts || !inputs || !targets || (x_dim > 0 && !coords))
+    return fail(PML_E_INVALID, "bad supervised-row arguments");
+  const int64_t rows = n_blocks * points;
+  const unsigned grid = (unsigned)(rows < (1LL << 20) ? rows : (1LL << 20));
+  hipLaunchKernelGGL(k_supervised_rows, dim3(grid), dim3(kRowBlock), 0, as_stream(stream),
+                     states, steps_per_state, nexts, rows, n_steps, points, n_fields, tcol,
+                     coords, x_dim, inputs, targets);
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}

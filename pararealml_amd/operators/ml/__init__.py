@@ -1,5 +1,6 @@
 """Learned coarse propagator: a BF16 convolutional surrogate trained on the
-device FDM solver's trajectories."""
+device FDM solver's trajectories; the reference's supervised-learning data
+protocol with batched device solves."""
 
 from pararealml_amd.operators.ml.conv_surrogate import ConvSurrogate
 from pararealml_amd.operators.ml.learned_coarse import (
@@ -9,6 +10,7 @@ from pararealml_amd.operators.ml.learned_coarse import (
     gaussian_mixture_states,
     generate_pairs,
 )
+from pararealml_amd.operators.ml.supervised_data import SupervisedDataProtocol
 
 __all__ = [
     "ConvSurrogate",
@@ -17,4 +19,5 @@ __all__ = [
     "LearnedCoarseOperator",
     "gaussian_mixture_states",
     "generate_pairs",
+    "SupervisedDataProtocol",
 ]
