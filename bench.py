@@ -28,7 +28,8 @@ propagator: all slices' states, compensated FP32 RK4), ``swe-parareal``
 (shallow water on 1025^2, 64 slices), ``swe-step`` (its batched fine
 stepping throughput), ``surrogate`` (conv surrogate training throughput),
 ``solution-y`` (``Solution.y`` / ``diff`` of the Parareal output on the
-device) and ``copy`` (the HBM counter calibration).  ``--ch-ics 1024`` runs the
+device), ``supervised-data`` (the supervised-learning data protocol with
+batched device solves) and ``copy`` (the HBM counter calibration).  ``--ch-ics 1024`` runs the
 Cahn-Hilliard batch beyond the 256 MB MALL.
 """
 
@@ -60,7 +61,10 @@ def parse_args():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="ch256",
                     choices=["ch256", "parareal", "parareal-fine-step", "swe-parareal",
-                             "swe-step", "copy", "surrogate", "solution-y"])
+                             "swe-step", "copy", "surrogate", "solution-y",
+                             "supervised-data"])
+    ap.add_argument("--sup-iterations", type=int, default=64,
+                    help="iterations of the supervised-data workload")
     ap.add_argument("--sol-points", type=int, default=1 << 22,
                     help="query points of the solution-y workload")
     ap.add_argument("--ch-ics", type=int, default=256,
@@ -996,6 +1000,74 @@ def solution_workload(args):
     print(json.dumps(out), flush=True)
 
 
+def supervised_workload(args):
+    """The supervised-learning data protocol on the device
+    (``SupervisedDataProtocol.generate_data_device``): 2-D diffusion (d =
+    0.2) on 33^2 vertices with Dirichlet edges, operator step 0.01 over
+    [0, 0.1] (10 steps), FDM oracle RK4 at 1e-4 (100 steps per sub-IVP),
+    ``--sup-iterations`` iterations with isolated Gaussian perturbations
+    (seeded): every sub-IVP of every iteration in one batched solve, then the
+    reference's rows (one per iteration, step and vertex: the 1089-vertex
+    state, the time, the coordinates).  Roofline of the row kernel: bytes
+    written (inputs and targets) per launch.  CPU baseline: the oracle's
+    restatement (sub-IVPs solved one by one in NumPy, one core) on the first
+    iterations, scaled."""
+    import torch
+
+    import pararealml_amd as pml
+    from pararealml_amd.operators.fdm import RK4, FDMOperator, ThreePointCentralDifferenceMethod
+    from pararealml_amd.operators.ml import SupervisedDataProtocol
+
+    torch.cuda.set_device(0)
+    mesh = pml.Mesh([(0.0, 1.0), (0.0, 1.0)], [1.0 / 32, 1.0 / 32])
+    bc = pml.DirichletBoundaryCondition(lambda x, t: np.zeros((len(x), 1)), is_static=True)
+    cp = pml.ConstrainedProblem(pml.DiffusionEquation(2, 0.2), mesh, [(bc, bc)] * 2)
+    ic = pml.GaussianInitialCondition(cp, [(np.array([0.4, 0.6]), 0.02 * np.eye(2))])
+    ivp = pml.InitialValueProblem(cp, (0.0, 0.1), ic)
+    oracle_op = FDMOperator(RK4(), ThreePointCentralDifferenceMethod(), 1e-4)
+    proto = SupervisedDataProtocol(0.01, True, time_variant=True)
+    noise = lambda t, y: y + np.random.normal(0.0, 0.01, y.shape)  # noqa: E731
+    iters = args.sup_iterations
+
+    def run():
+        return proto.generate_data_device(ivp, oracle_op, iters, noise,
+                                          isolate_perturbations=True, seeds=[SEEDS[0]])
+
+    run()
+    torch.cuda.synchronize()
+    reps = max(1, args.steps // 500)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        inputs, targets = run()
+    torch.cuda.synchronize()
+    call_s = (time.perf_counter() - t0) / reps
+    row_bytes = (inputs.numel() + targets.numel()) * 8
+    out = {
+        "workload": "supervised-data",
+        "config": {"grid": "33^2", "iterations": iters, "operator_steps": 10,
+                   "oracle_steps_per_sub_ivp": 100, "integrator": "RK4", "dtype": "f64",
+                   "mode": "isolated perturbations, time variant"},
+        "calls": reps,
+        "ms_per_call": call_s * 1e3,
+        "sub_ivps_per_s": iters * 10 / call_s,
+        "rows": int(inputs.shape[0]), "row_width": int(inputs.shape[1]),
+        "row_bytes_per_call": row_bytes,
+    }
+    if not args.no_cpu_baseline:
+        from oracle import supervised_numpy as osup
+
+        sample = max(1, min(iters, 2))
+        t0 = time.perf_counter()
+        osup.generate_data(cp, ic.discrete_y_0(True), (0.0, 0.1), 0.01, 1e-4, "rk4", sample,
+                           noise, time_variant=True, isolate_perturbations=True,
+                           seed=SEEDS[0])
+        cpu_s = (time.perf_counter() - t0) * iters / sample
+        out["cpu_baseline"] = {"value": iters * 10 / cpu_s, "unit": "sub-IVPs/s", "cores": 1,
+                               "kind": "port",
+                               "sample": f"{sample} of the {iters} iterations, scaled"}
+    print(json.dumps(out), flush=True)
+
+
 def copy_workload(args):
     """A plain device-to-device copy of 1 GiB, repeated: the calibration
     reference for the FETCH_SIZE / WRITE_SIZE counters (each dispatch moves
@@ -1049,6 +1121,9 @@ def main():
         return
     if args.workload == "solution-y":
         solution_workload(args)
+        return
+    if args.workload == "supervised-data":
+        supervised_workload(args)
         return
     if args.workload == "surrogate":
         surrogate_workload(args)
