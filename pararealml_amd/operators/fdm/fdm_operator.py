@@ -87,7 +87,7 @@ class _SymbolicDeviceRHS:
         self.x = {}
         for prefix, idx in self.specs:
             if prefix == "x" and idx[0] not in self.x:
-                grid = np.ascontiguousarray(cp.mesh.vertex_coordinate_grids[idx[0]])
+                grid = np.array(cp.mesh.vertex_coordinate_grids[idx[0]])
                 self.x[idx[0]] = torch.from_numpy(grid).to(device="cuda", dtype=dtype)
 
     def _derivative(self, prefix, idx, y, dbc):
