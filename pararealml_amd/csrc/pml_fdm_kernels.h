@@ -1050,8 +1050,28 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
   const bool lo1 = col == 0, hi1 = col == n1 - 1;
   const int cc = min(max(col, 0), n1 - 1);
   const int r_first = i_beg - (S - 1);  // first stage-1 row
-  const bool edge = !ax0 || c0 <= 0 || c0 + 63 >= n1 - 1 || r_first <= 0 ||
-                    i_end - 1 + (S - 1) >= n0 - 1;  // wave-uniform
+  // Which grid edges the unit's rows and lanes reach (wave-uniform): units
+  // beside one axis-1 edge only run a path without the other edges' rules,
+  // and when every row of the unit holds a Dirichlet value on that edge,
+  // the edge column's own stencil is never used (its value is replaced at
+  // every stage), so that path runs the interior stencils.
+  const bool rows_edge = !ax0 || r_first <= 0 || i_end - 1 + (S - 1) >= n0 - 1;
+  const bool lo_edge = c0 <= 0, hi_edge = c0 + 63 >= n1 - 1;
+  int kind = rows_edge || (lo_edge && hi_edge) ? 3 : (lo_edge ? 1 : (hi_edge ? 2 : 0));
+  if (kind == 1 || kind == 2) {
+    // lane k (k < NT) holds row r_first + k of the lower edge, lane 32 + k
+    // the same row of the upper edge
+    const int rk = min(max(r_first + (lane & 31), 0), n0 - 1) + (lane < 32 ? 0 : n0);
+    const unsigned long long rows = (1ull << NT) - 1ull;
+    const unsigned long long want = kind == 1 ? rows : rows << 32;
+    bool all = true;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const double v = bcg[f * fstride + 2 * n1 + rk];
+      all = all && (__ballot(v == v) & want) == want;
+    }
+    if (all) kind += 3;  // 4: Dirichlet lower edge, 5: Dirichlet upper edge
+  }
 
   const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
   const C *in_lo = P::kComp ? static_cast<const C *>(a.in_lo) + G.off : nullptr;
@@ -1093,8 +1113,17 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
     }
   };
 
-  auto run = [&](auto edge_tag) {
-    constexpr bool kE = decltype(edge_tag)::value;
+  auto run = [&](auto kind_tag) {
+    // 0: interior; 1 / 2: beside the lower / upper axis-1 edge only, rows
+    // interior; 3: any edge (1-D grids included); 4 / 5: as 1 / 2 with a
+    // Dirichlet value on every row of that edge
+    constexpr int kKind = decltype(kind_tag)::value;
+    constexpr bool kE = kKind != 0;
+    constexpr bool kRows = kKind == 3;
+    constexpr bool kLo1 = kKind == 1 || kKind == 3;  // edge stencil rules
+    constexpr bool kHi1 = kKind == 2 || kKind == 3;
+    constexpr bool kNeedLo = kLo1 || kKind == 4;     // edge values needed
+    constexpr bool kNeedHi = kHi1 || kKind == 5;
     // Edge values: the lane's column on rows 0 / n0-1, and the axis-1 edge
     // values of stage-1 row r_first + k, lower edge in lane k and upper edge
     // in lane 32 + k (one register for both).
@@ -1107,7 +1136,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
       if (kE) {
         const double *fb = bcg + f * fstride;
         const double *nb = fb + 2 * (n0 + n1);
-        if (ax0) {
+        if (kRows && ax0) {
           rd_lo[f] = static_cast<D>(fb[cc]);
           rd_hi[f] = static_cast<D>(fb[n1 + cc]);
           rn_lo[f] = static_cast<C>(nb[cc]);
@@ -1123,21 +1152,30 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
     auto stage_k = [&](int k, const C *yu, const C *yc, const C *yd, const C *fill, C w, C *kk,
                        D *dv) {
       const int r = r_first + k;
-      const bool lo0 = kE && ax0 && r == 0, hi0 = kE && ax0 && r == n0 - 1;
-      const bool l1 = kE && lo1, h1 = kE && hi1;
+      const bool lo0 = kRows && ax0 && r == 0, hi0 = kRows && ax0 && r == n0 - 1;
+      const bool l1 = kLo1 && lo1, h1 = kHi1 && hi1;
       C e_lo[F], e_hi[F];
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         e_lo[f] = e_hi[f] = static_cast<C>(NAN);
         dv[f] = static_cast<D>(NAN);
         if (kE) {
-          e_lo[f] = lane_value(pe[f], k);
-          e_hi[f] = lane_value(pe[f], 32 + k);
-          // Both broadcasts are read unconditionally: behind a lane-varying
-          // condition the compiler would branch on EXEC around each one.
-          const D d_l = lane_value(pd[f], k), d_h = lane_value(pd[f], 32 + k);
+          // The broadcasts a path needs are read unconditionally: behind a
+          // lane-varying condition the compiler would branch on EXEC
+          // around each one.
+          D d_l = static_cast<D>(NAN), d_h = static_cast<D>(NAN);
+          if (kNeedLo) {
+            if (kLo1) e_lo[f] = lane_value(pe[f], k);
+            d_l = lane_value(pd[f], k);
+          }
+          if (kNeedHi) {
+            if (kHi1) e_hi[f] = lane_value(pe[f], 32 + k);
+            d_h = lane_value(pd[f], 32 + k);
+          }
           D v = lo0 ? rd_lo[f] : (hi0 ? rd_hi[f] : static_cast<D>(NAN));
-          v = (v == v) ? v : (l1 ? d_l : (h1 ? d_h : static_cast<D>(NAN)));
+          v = (v == v) ? v
+                       : ((kNeedLo && lo1) ? d_l
+                                           : ((kNeedHi && hi1) ? d_h : static_cast<D>(NAN)));
           dv[f] = v;
         }
       }
@@ -1201,7 +1239,9 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
 #pragma unroll
         for (int f = 0; f < F; ++f) {
           const C yh = w[my][f], yl = P::kComp ? wl[my][f] : C(0);
-          const bool fix = dv[f] == dv[f];
+          // the Dirichlet-edge paths know their fixed lane (every row of
+          // the edge holds a value): no FP64 compare
+          const bool fix = kKind == 4 ? lo1 : (kKind == 5 ? hi1 : dv[f] == dv[f]);
           if (s < S) {
             C nv;
             if (S == 2) {
@@ -1236,8 +1276,12 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
       }
     }
   };
-  if (edge) run(std::true_type{});
-  else run(std::false_type{});
+  if (kind == 0) run(std::integral_constant<int, 0>{});
+  else if (kind == 4) run(std::integral_constant<int, 4>{});
+  else if (kind == 5) run(std::integral_constant<int, 5>{});
+  else if (kind == 1) run(std::integral_constant<int, 1>{});
+  else if (kind == 2) run(std::integral_constant<int, 2>{});
+  else run(std::integral_constant<int, 3>{});
 }
 
 // ---------------------------------------------------------------------------
