@@ -344,6 +344,15 @@ int pml_to_f64(int precision, const void *src_hi, const void *src_lo,
                double *dst, int64_t n, void *stream);
 
 /*
+ * Fallback test of a learned coarse prediction (no counterpart in the
+ * reference): *flag is set to 1 when dst (n FP64 values) holds a non-finite
+ * value or, src holding no NaN, max|dst| > guard * max(max|src|, 1e-300),
+ * and left as it is otherwise.  work: 4 * PML_REDUCE_SLOTS doubles.
+ */
+int pml_slice_guard(const double *src, const double *dst, int64_t n,
+                    double guard, double *work, int32_t *flag, void *stream);
+
+/*
  * 2x grid transfer between a fine vertex grid (2m-1 points per axis) and a
  * coarse one (m points), batched over `batch` planes, FP64 or FP32 by
  * precision (PML_F32C not accepted).  No counterpart in the reference.

@@ -172,6 +172,9 @@ SIGNATURES = {
     ),
     "pml_from_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
     "pml_to_f64": (c_int, [c_int, c_void_p, c_void_p, c_void_p, c_int64, c_void_p]),
+    "pml_slice_guard": (
+        c_int, [c_void_p, c_void_p, c_int64, c_double, c_void_p, c_void_p, c_void_p]
+    ),
     "pml_restrict2x": (
         c_int, [c_int, c_void_p, c_void_p, c_int64, c_int64, c_int64, c_void_p]
     ),

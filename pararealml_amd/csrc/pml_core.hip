@@ -4,6 +4,8 @@
 
 #include <string>
 
+#include <cfloat>
+
 #include "pml_common.h"
 
 namespace pml {
@@ -101,6 +103,71 @@ __global__ void k_update(const double *g, const double *corr, double *u, int64_t
     }
     if (threadIdx.x == 0) work[static_cast<int64_t>(f) * gridDim.x + blockIdx.x] = red[0];
     __syncthreads();
+  }
+}
+
+// Fallback test of a learned coarse prediction: per block the largest |src|,
+// the largest |dst|, whether dst holds a non-finite value and whether src
+// holds a NaN, written to work[4 * block + 0..3]; k_guard_flag merges the
+// blocks and sets *flag when dst is not finite or (src has no NaN and)
+// max|dst| > guard * max(max|src|, 1e-300) -- a NaN in src makes the ratio
+// test false, as a comparison with a NaN bound is.
+__device__ __forceinline__ void guard_tree(double *a, double *b, double *c, double *d) {
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      a[threadIdx.x] = fmax(a[threadIdx.x], a[threadIdx.x + s]);
+      b[threadIdx.x] = fmax(b[threadIdx.x], b[threadIdx.x + s]);
+      c[threadIdx.x] = fmax(c[threadIdx.x], c[threadIdx.x + s]);
+      d[threadIdx.x] = fmax(d[threadIdx.x], d[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_guard_partials(const double *src, const double *dst, int64_t n,
+                                 double *work) {
+  __shared__ double s_src[256], s_dst[256], s_bad[256], s_nan[256];
+  double ms = 0.0, md = 0.0, bad = 0.0, snan = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double a = fabs(src[i]), d = fabs(dst[i]);
+    if (a == a) ms = a > ms ? a : ms;
+    else snan = 1.0;
+    if (d <= DBL_MAX) md = d > md ? d : md;
+    else bad = 1.0;  // infinite or NaN
+  }
+  s_src[threadIdx.x] = ms;
+  s_dst[threadIdx.x] = md;
+  s_bad[threadIdx.x] = bad;
+  s_nan[threadIdx.x] = snan;
+  __syncthreads();
+  guard_tree(s_src, s_dst, s_bad, s_nan);
+  if (threadIdx.x == 0) {
+    work[4 * blockIdx.x + 0] = s_src[0];
+    work[4 * blockIdx.x + 1] = s_dst[0];
+    work[4 * blockIdx.x + 2] = s_bad[0];
+    work[4 * blockIdx.x + 3] = s_nan[0];
+  }
+}
+
+__global__ void k_guard_flag(const double *work, int n_parts, double guard, int32_t *flag) {
+  __shared__ double s_src[256], s_dst[256], s_bad[256], s_nan[256];
+  double ms = 0.0, md = 0.0, bad = 0.0, snan = 0.0;
+  for (int b = threadIdx.x; b < n_parts; b += blockDim.x) {
+    ms = fmax(ms, work[4 * b + 0]);
+    md = fmax(md, work[4 * b + 1]);
+    bad = fmax(bad, work[4 * b + 2]);
+    snan = fmax(snan, work[4 * b + 3]);
+  }
+  s_src[threadIdx.x] = ms;
+  s_dst[threadIdx.x] = md;
+  s_bad[threadIdx.x] = bad;
+  s_nan[threadIdx.x] = snan;
+  __syncthreads();
+  guard_tree(s_src, s_dst, s_bad, s_nan);
+  if (threadIdx.x == 0) {
+    const double floor = s_src[0] > 1e-300 ? s_src[0] : 1e-300;
+    if (s_bad[0] > 0.0 || (s_nan[0] == 0.0 && s_dst[0] > guard * floor)) *flag = 1;
   }
 }
 
@@ -367,6 +434,20 @@ extern "C" int pml_parareal_update(const double *g_new, const double *corr, doub
                      work);
   PML_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_sum_partials, dim3(n_fields), dim3(256), 0, st, work, nb, sumsq);
+  PML_LAUNCH_CHECK();
+  return PML_OK;
+}
+
+extern "C" int pml_slice_guard(const double *src, const double *dst, int64_t n, double guard,
+                               double *work, int32_t *flag, void *stream) {
+  if (n <= 0 || !src || !dst || !work || !flag || !(guard > 0.0))
+    return fail(PML_E_INVALID, "bad guard arguments");
+  hipStream_t st = as_stream(stream);
+  int nb = blocks_for(n, 256);
+  if (nb > PML_REDUCE_SLOTS) nb = PML_REDUCE_SLOTS;
+  hipLaunchKernelGGL(k_guard_partials, dim3(nb), dim3(256), 0, st, src, dst, n, work);
+  PML_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_guard_flag, dim3(1), dim3(256), 0, st, work, nb, guard, flag);
   PML_LAUNCH_CHECK();
   return PML_OK;
 }

@@ -56,32 +56,41 @@ class _FallbackGuard:
     def _guard_init(self, n_slices: int) -> None:
         self.deferred = False
         self._n_slices = n_slices
-        self._flags = None
+        self._flags = None  # int32 per slice, set by pml_slice_guard
+        self._one = None    # int32, the immediate test's flag
+        self._guard_work = None
         self._forced = set()
 
     def _run_fallback(self, u_f64, out_f64, k: int, first: int) -> None:
         self.op.fallbacks += 1
         self.fallback.run(u_f64[k : k + 1], first + k, first + k + 1, out_f64[k : k + 1])
 
-    def _bad(self, u_f64, out_f64, k: int, first: int):
-        """Device flag: slice first + k's prediction needs the fallback."""
-        torch = self.torch
-        src, dst = u_f64[k], out_f64[k]
-        return (~torch.isfinite(dst)).any() | (
-            dst.abs().max() > self.op.guard * src.abs().max().clamp_min(1e-300))
-
     def _guard(self, u_f64, out_f64, k: int, first: int) -> None:
-        """Runs on the propagator's stream (the caller's context)."""
+        """Runs on the propagator's stream (the caller's context): one
+        ``pml_slice_guard`` (non-finite prediction, or magnitude above
+        ``op.guard`` times the input's) per slice."""
         if self.fallback is None:
             return
         torch = self.torch
-        bad = self._bad(u_f64, out_f64, k, first)
-        if self.deferred:
-            if self._flags is None:
-                self._flags = torch.zeros(self._n_slices, dtype=torch.bool, device="cuda")
-            self._flags[first + k] |= bad
-        elif bool(bad.item()):
+        if self._guard_work is None:
+            self._guard_work = torch.empty(4 * nat.PML_REDUCE_SLOTS, dtype=torch.float64,
+                                           device="cuda")
+            self._flags = torch.zeros(self._n_slices, dtype=torch.int32, device="cuda")
+            self._one = torch.zeros(1, dtype=torch.int32, device="cuda")
+        flag = self._flags[first + k] if self.deferred else self._one[0]
+        if not self.deferred:
+            flag.zero_()
+        self._test(u_f64, out_f64, k, first, flag)
+        if not self.deferred and int(flag.item()):
             self._run_fallback(u_f64, out_f64, k, first)
+
+    def _test(self, u_f64, out_f64, k: int, first: int, flag) -> None:
+        """Sets ``flag`` (a device int32) when slice first + k's prediction
+        needs the fallback."""
+        nat.check(nat.device_lib().pml_slice_guard(
+            u_f64[k].data_ptr(), out_f64[k].data_ptr(), u_f64[k].numel(),
+            float(self.op.guard), self._guard_work.data_ptr(), flag.data_ptr(),
+            ctypes.c_void_p(self.stream.cuda_stream)))
 
     def first_bad(self):
         """Index of the first slice flagged since the last call (None if
@@ -89,9 +98,10 @@ class _FallbackGuard:
         if self._flags is None:
             return None
         with self.torch.cuda.stream(self.stream):
-            idx = self.torch.nonzero(self._flags).flatten().cpu()
+            host = self._flags.cpu().numpy()
             self._flags.zero_()
-        return int(idx[0]) if idx.numel() else None
+        idx = np.flatnonzero(host)
+        return int(idx[0]) if idx.size else None
 
     def force_fallback(self, j: int) -> None:
         """Slice j is computed by the fallback until ``clear_forced``."""

@@ -426,6 +426,58 @@ def test_trained_correction_reduces_the_coarse_error_on_held_out_states(correct_
     assert err_net < 0.5 * err_base
 
 
+def test_slice_guard_matches_its_definition():
+    """pml_slice_guard sets the flag exactly when the prediction is not
+    finite or, the input holding no NaN, max|dst| > guard max(max|src|,
+    1e-300) -- the NumPy expression below -- and never clears it."""
+    import torch
+
+    from pararealml_amd import _native as nat
+
+    print("case: slice guard", flush=True)
+    lib = nat.device_lib()
+    rng = np.random.default_rng(5)
+    work = torch.empty(4 * nat.PML_REDUCE_SLOTS, dtype=torch.float64, device="cuda")
+    s = torch.cuda.current_stream().cuda_stream
+
+    def want(src, dst, guard):
+        if not np.all(np.isfinite(dst)):
+            return 1
+        if np.any(np.isnan(src)):
+            return 0
+        return int(np.abs(dst).max() > guard * max(np.abs(src).max(), 1e-300))
+
+    n = 300_000
+    base = rng.standard_normal(n)
+    cases = []
+    for scale in (1.0, 9.9, 10.1, 1e3):
+        cases.append((base, base * scale))
+    bad = base.copy()
+    bad[12345] = np.nan
+    cases.append((base, bad))
+    inf = base.copy()
+    inf[-1] = -np.inf
+    cases.append((base, inf))
+    cases.append((bad, base * 1e6))     # NaN input: no ratio test
+    cases.append((np.zeros(n), np.zeros(n)))
+    cases.append((np.zeros(n), np.full(n, 1e-290)))
+    for src, dst in cases:
+        ts = torch.from_numpy(src).cuda()
+        td = torch.from_numpy(dst).cuda()
+        flag = torch.zeros(1, dtype=torch.int32, device="cuda")
+        nat.check(lib.pml_slice_guard(ts.data_ptr(), td.data_ptr(), n, 10.0, work.data_ptr(),
+                                      flag.data_ptr(), s))
+        assert int(flag.item()) == want(src, dst, 10.0)
+        # a set flag stays set
+        flag.fill_(1)
+        nat.check(lib.pml_slice_guard(ts.data_ptr(), ts.data_ptr(), n, 10.0, work.data_ptr(),
+                                      flag.data_ptr(), s))
+        assert int(flag.item()) == 1
+    with pytest.raises(ValueError):
+        nat.check(lib.pml_slice_guard(ts.data_ptr(), td.data_ptr(), 0, 10.0, work.data_ptr(),
+                                      flag.data_ptr(), s))
+
+
 def test_deferred_fallback_tests_equal_per_slice_tests(monkeypatch):
     """The Parareal driver defers the learned propagator's fallback tests to
     one host synchronisation per sweep and re-runs a sweep from the first
@@ -464,10 +516,11 @@ def test_deferred_fallback_tests_equal_per_slice_tests(monkeypatch):
     net = ConvSurrogate(1)
     net.set_params(params)
 
-    def flag_slices_1_and_2(self, u_f64, out_f64, k, first):
-        return torch.tensor(first + k in (1, 2), device="cuda")
+    def flag_slices_1_and_2(self, u_f64, out_f64, k, first, flag):
+        if first + k in (1, 2):
+            flag.fill_(1)
 
-    monkeypatch.setattr(lc._FallbackGuard, "_bad", flag_slices_1_and_2)
+    monkeypatch.setattr(lc._FallbackGuard, "_test", flag_slices_1_and_2)
     runs = []
     for deferral in (True, False):
         monkeypatch.setattr(lc._FallbackGuard, "supports_deferral", deferral)
