@@ -1829,6 +1829,12 @@ const std::vector<Variant> &variants_for() {
       out.push_back(PML_STREAMRK(EQ, IN, PR, 8, 2, "streamrk-rw8-pd2"));
       out.push_back(PML_STREAMRK(EQ, IN, PR, 4, 1, "streamrk-rw4-pd1"));
       out.push_back(PML_STREAMRK(EQ, IN, PR, 2, 1, "streamrk-rw2-pd1"));
+      if constexpr (EQ::F <= 2) {
+        // longer strips with a deeper prefetch: the Parareal fine batch of
+        // compensated FP32 diffusion is latency-bound at pd1 (7 % slower)
+        out.push_back(PML_STREAMRK(EQ, IN, PR, 16, 4, "streamrk-rw16-pd4"));
+        out.push_back(PML_STREAMRK(EQ, IN, PR, 24, 4, "streamrk-rw24-pd4"));
+      }
       if constexpr (wide) {
         out.push_back(PML_TILE(EQ, IN, PR, 32, 8, 1, 256, "tile-ox32-oy8-s1"));
       } else {
