@@ -788,8 +788,12 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
   const int cc = min(max(col, 0), n1 - 1);
   // Wave-uniform unit kinds: does any computed column / row lie on (or
   // beyond) a grid edge?
-  const bool edge_cols = U.j0 - 1 <= 0 || U.j0 + 62 >= n1 - 1;
+  const bool lo_edge = U.j0 - 1 <= 0, hi_edge = U.j0 + 62 >= n1 - 1;
   const bool edge_rows = !ax0 || i_beg - 1 <= 0 || i_end >= n0 - 1;
+  // column kind: 1 / 2 the lower / upper axis-1 edge only, 3 both; units
+  // that also reach a row edge take 3 (fewer compiled paths)
+  const int cols = edge_rows ? (lo_edge || hi_edge ? 3 : 0)
+                             : (lo_edge ? 1 : 0) | (hi_edge ? 2 : 0);
 
   const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
   const C *in_lo = P::kComp ? static_cast<const C *>(a.in_lo) + G.off : nullptr;
@@ -832,10 +836,14 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
     }
   };
 
-  // The row walk for one unit kind: kCols / kRows select the edge rules
-  // compiled in (interior units run plain stencils).
+  // The row walk for one unit kind: kLoC / kHiC / kRows select the edge
+  // rules compiled in (interior units run plain stencils; a unit beside one
+  // axis-1 edge carries that edge's rules only).
   auto run = [&](auto cols_tag, auto rows_tag) {
-    constexpr bool kCols = decltype(cols_tag)::value;
+    constexpr int kColKind = decltype(cols_tag)::value;
+    constexpr bool kLoC = (kColKind & 1) != 0;
+    constexpr bool kHiC = (kColKind & 2) != 0;
+    constexpr bool kCols = kColKind != 0;
     constexpr bool kRows = decltype(rows_tag)::value;
     // Axis-0 edge values of the lane's column (rows 0 and n0-1).
     C rn_lo[F], rn_hi[F];
@@ -864,10 +872,14 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
       for (int f = 0; f < F; ++f) {
         const double *fb = bcg + f * fstride;
         const double *nb = fb + 2 * (n0 + n1);
-        pd_lo[f] = static_cast<D>(fb[2 * n1 + rk]);
-        pd_hi[f] = static_cast<D>(fb[2 * n1 + n0 + rk]);
-        pe_lo[f] = static_cast<C>(nb[2 * n1 + rk]);
-        pe_hi[f] = static_cast<C>(nb[2 * n1 + n0 + rk]);
+        if (kLoC) {
+          pd_lo[f] = static_cast<D>(fb[2 * n1 + rk]);
+          pe_lo[f] = static_cast<C>(nb[2 * n1 + rk]);
+        }
+        if (kHiC) {
+          pd_hi[f] = static_cast<D>(fb[2 * n1 + n0 + rk]);
+          pe_hi[f] = static_cast<C>(nb[2 * n1 + n0 + rk]);
+        }
       }
     }
     // One step of row r = i_beg - 1 + k from rows (yu, yc, yd) of the
@@ -883,14 +895,16 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
       for (int f = 0; f < F; ++f) {
         e_lo[f] = e_hi[f] = static_cast<C>(NAN);
         d_lo[f] = d_hi[f] = static_cast<D>(NAN);
-        if (kCols) {
+        if (kLoC) {
           d_lo[f] = lane_value(pd_lo[f], k);
-          d_hi[f] = lane_value(pd_hi[f], k);
           e_lo[f] = lane_value(pe_lo[f], k);
+        }
+        if (kHiC) {
+          d_hi[f] = lane_value(pd_hi[f], k);
           e_hi[f] = lane_value(pe_hi[f], k);
         }
       }
-      const bool l1 = kCols && lo1, h1 = kCols && hi1;
+      const bool l1 = kLoC && lo1, h1 = kHiC && hi1;
       Quant<C> q[F];
 #define PML_FQS(F_)                                                                          \
   if constexpr (F_ < F) {                                                                    \
@@ -975,12 +989,19 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
   };
   using T = std::true_type;
   using Fa = std::false_type;
-  if (edge_cols) {
-    if (edge_rows) run(T{}, T{});
-    else run(T{}, Fa{});
+  using C0 = std::integral_constant<int, 0>;
+  using C3 = std::integral_constant<int, 3>;
+  if (edge_rows) {
+    if (cols) run(C3{}, T{});
+    else run(C0{}, T{});
+  } else if (cols == 0) {
+    run(C0{}, Fa{});
+  } else if (cols == 1) {
+    run(std::integral_constant<int, 1>{}, Fa{});
+  } else if (cols == 2) {
+    run(std::integral_constant<int, 2>{}, Fa{});
   } else {
-    if (edge_rows) run(Fa{}, T{});
-    else run(Fa{}, Fa{});
+    run(C3{}, Fa{});
   }
 }
 
