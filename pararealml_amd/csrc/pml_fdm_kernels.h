@@ -1092,6 +1092,20 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
       all = all && (__ballot(v == v) & want) == want;
     }
     if (all) kind += 3;  // 4: Dirichlet lower edge, 5: Dirichlet upper edge
+  } else if (kind == 3 && ax0 && !lo_edge && !hi_edge) {
+    // 6: the unit reaches row 0 and / or row n0 - 1 only, and every one of
+    // its columns holds a Dirichlet value there: those rows are replaced at
+    // every stage, the rows beyond them feed only them
+    const bool top = r_first <= 0, bot = i_end - 1 + (S - 1) >= n0 - 1;
+    bool all = true;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const double *fb = bcg + f * fstride;
+      const double vt = fb[cc], vb = fb[n1 + cc];
+      if (top) all = all && __ballot(vt == vt) == ~0ull;
+      if (bot) all = all && __ballot(vb == vb) == ~0ull;
+    }
+    if (all) kind = 6;
   }
 
   const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
@@ -1137,10 +1151,12 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
   auto run = [&](auto kind_tag) {
     // 0: interior; 1 / 2: beside the lower / upper axis-1 edge only, rows
     // interior; 3: any edge (1-D grids included); 4 / 5: as 1 / 2 with a
-    // Dirichlet value on every row of that edge
+    // Dirichlet value on every row of that edge; 6: beside the axis-0 edges
+    // only, Dirichlet on all the unit's columns there
     constexpr int kKind = decltype(kind_tag)::value;
     constexpr bool kE = kKind != 0;
-    constexpr bool kRows = kKind == 3;
+    constexpr bool kRows = kKind == 3;    // axis-0 edge stencil rules
+    constexpr bool kRowDir = kKind == 6;  // axis-0 Dirichlet rows only
     constexpr bool kLo1 = kKind == 1 || kKind == 3;  // edge stencil rules
     constexpr bool kHi1 = kKind == 2 || kKind == 3;
     constexpr bool kNeedLo = kLo1 || kKind == 4;     // edge values needed
@@ -1157,9 +1173,11 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
       if (kE) {
         const double *fb = bcg + f * fstride;
         const double *nb = fb + 2 * (n0 + n1);
-        if (kRows && ax0) {
+        if ((kRows || kRowDir) && ax0) {
           rd_lo[f] = static_cast<D>(fb[cc]);
           rd_hi[f] = static_cast<D>(fb[n1 + cc]);
+        }
+        if (kRows && ax0) {
           rn_lo[f] = static_cast<C>(nb[cc]);
           rn_hi[f] = static_cast<C>(nb[n1 + cc]);
         }
@@ -1174,6 +1192,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
                        D *dv) {
       const int r = r_first + k;
       const bool lo0 = kRows && ax0 && r == 0, hi0 = kRows && ax0 && r == n0 - 1;
+      const bool dlo0 = kRowDir && r == 0, dhi0 = kRowDir && r == n0 - 1;
       const bool l1 = kLo1 && lo1, h1 = kHi1 && hi1;
       C e_lo[F], e_hi[F];
 #pragma unroll
@@ -1193,7 +1212,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
             if (kHi1) e_hi[f] = lane_value(pe[f], 32 + k);
             d_h = lane_value(pd[f], 32 + k);
           }
-          D v = lo0 ? rd_lo[f] : (hi0 ? rd_hi[f] : static_cast<D>(NAN));
+          D v = (lo0 || dlo0) ? rd_lo[f] : ((hi0 || dhi0) ? rd_hi[f] : static_cast<D>(NAN));
           v = (v == v) ? v
                        : ((kNeedLo && lo1) ? d_l
                                            : ((kNeedHi && hi1) ? d_h : static_cast<D>(NAN)));
@@ -1262,7 +1281,11 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
           const C yh = w[my][f], yl = P::kComp ? wl[my][f] : C(0);
           // the Dirichlet-edge paths know their fixed lane (every row of
           // the edge holds a value): no FP64 compare
-          const bool fix = kKind == 4 ? lo1 : (kKind == 5 ? hi1 : dv[f] == dv[f]);
+          const int rr_row = r_first + k;
+          const bool fix = kKind == 4   ? lo1
+                           : kKind == 5 ? hi1
+                           : kKind == 6 ? (rr_row == 0 || rr_row == n0 - 1)
+                                        : dv[f] == dv[f];
           if (s < S) {
             C nv;
             if (S == 2) {
@@ -1298,6 +1321,7 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
     }
   };
   if (kind == 0) run(std::integral_constant<int, 0>{});
+  else if (kind == 6) run(std::integral_constant<int, 6>{});
   else if (kind == 4) run(std::integral_constant<int, 4>{});
   else if (kind == 5) run(std::integral_constant<int, 5>{});
   else if (kind == 1) run(std::integral_constant<int, 1>{});
