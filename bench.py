@@ -389,8 +389,12 @@ def parareal_rows(cfg, log):
         }
 
     g_fdm = CoarseGridFDMOperator(FDMOperator(ForwardEulerMethod(), diff, dt_c))
+    # the surrogate is trained before any Parareal row runs, so the FDM and
+    # learned rows are measured in the same state of the GPU (the fine sweeps
+    # run right after 25 s of training were up to 4 % slower than before it)
+    trained = _train_learned_coarse(cfg, cp, diff, n, S, log) if cfg["learned"] else None
     fdm_res = run_parareal(g_fdm, "FDM coarse on the 2x grid")
-    learned = (_learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
+    learned = (_learned_coarse_rows(cfg, trained, g_fdm, fdm_res, run_parareal)
                if cfg["learned"] else {})
     # measured last: run before the learned rows, the fine sweeps that
     # followed it were up to 3 % slower (6.47 s against 6.27 s per sweep)
@@ -436,10 +440,9 @@ def parareal_rows(cfg, log):
     return out
 
 
-def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log):
-    """Data generation, training and the Parareal runs of the learned coarse
-    operator (and of its ablation when its base differs from the FDM coarse
-    operator)."""
+def _train_learned_coarse(cfg, cp, diff, n, S, log):
+    """Data generation and training of the learned coarse operator's
+    surrogate; returns what ``_learned_coarse_rows`` needs."""
     import torch
 
     from pararealml_amd.operators.fdm import RK4, FDMOperator, ForwardEulerMethod
@@ -479,6 +482,27 @@ def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
     n_pairs = int(inputs.shape[0])
     del inputs, targets
     torch.cuda.empty_cache()
+    return {"base": base, "net": net, "lv": lv, "on": on, "level": level,
+            "training": {
+                "trajectories": cfg["traj"],
+                "pairs": n_pairs,
+                "data_generation_s": t_data,
+                "train_steps": cfg["train_steps"],
+                "train_batch": cfg["train_batch"],
+                "mix": cfg["mix"],
+                "train_s": t_train,
+                "loss_first": float(np.mean(losses[:10])),
+                "loss_last": float(np.mean(losses[-10:])),
+            }}
+
+
+def _learned_coarse_rows(cfg, trained, g_fdm, fdm_res, run_parareal):
+    """The Parareal runs of the learned coarse operator (and of its ablation
+    when its base differs from the FDM coarse operator)."""
+    from pararealml_amd.operators.ml import ConvSurrogate, CorrectedCoarseOperator
+
+    base, net = trained["base"], trained["net"]
+    lv, on, level = trained["lv"], trained["on"], trained["level"]
     g_learned = CorrectedCoarseOperator(base, net, fallback=g_fdm, base_levels=lv,
                                         correct_on=on, freeze=not cfg["live"])
     inc = cfg["increments"] and cfg["live"]
@@ -507,17 +531,7 @@ def _learned_coarse_rows(cfg, cp, diff, n, S, g_fdm, fdm_res, run_parareal, log)
                     + ("; Parareal correction in increment form" if inc else ""))),
         "learned_fallbacks": g_learned.fallbacks,
         "cheap_coarse_without_network": plain_res,
-        "training": {
-            "trajectories": cfg["traj"],
-            "pairs": n_pairs,
-            "data_generation_s": t_data,
-            "train_steps": cfg["train_steps"],
-            "train_batch": cfg["train_batch"],
-            "mix": cfg["mix"],
-            "train_s": t_train,
-            "loss_first": float(np.mean(losses[:10])),
-            "loss_last": float(np.mean(losses[-10:])),
-        },
+        "training": trained["training"],
     }
 
 
