@@ -1371,8 +1371,36 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
   const bool lo1 = col == 0, hi1 = col == n1 - 1;
   const int cc = min(max(col, 0), n1 - 1);
   const int r_first = i_beg - (NSTEP - 1);  // first step-1 row
-  const bool edge = !ax0 || c0 <= 0 || c0 + 63 >= n1 - 1 || r_first <= 0 ||
-                    i_end - 1 + (NSTEP - 1) >= n0 - 1;  // wave-uniform
+  // Which grid edges the unit reaches (wave-uniform), as in fdm_stream_rk:
+  // 0 interior; 1 / 2 one axis-1 edge only; 4 / 5 the same with a Dirichlet
+  // value on all the unit's rows of it (interior stencils, the edge lane
+  // replaced at every step); 6 row edges only, Dirichlet on all the unit's
+  // columns there; 3 anything else (1-D grids included).
+  const bool rows_edge = !ax0 || r_first <= 0 || i_end - 1 + (NSTEP - 1) >= n0 - 1;
+  const bool lo_edge = c0 <= 0, hi_edge = c0 + 63 >= n1 - 1;
+  int kind = rows_edge || (lo_edge && hi_edge) ? 3 : (lo_edge ? 1 : (hi_edge ? 2 : 0));
+  if (kind == 1 || kind == 2) {
+    const int rk = min(max(r_first + lane, 0), n0 - 1) + (kind == 1 ? 0 : n0);
+    const unsigned long long want = NT >= 64 ? ~0ull : (1ull << NT) - 1ull;
+    bool all = true;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const double v = bcg[f * fstride + 2 * n1 + rk];
+      all = all && (__ballot(v == v) & want) == want;
+    }
+    if (all) kind += 3;
+  } else if (kind == 3 && ax0 && !lo_edge && !hi_edge) {
+    const bool top = r_first <= 0, bot = i_end - 1 + (NSTEP - 1) >= n0 - 1;
+    bool all = true;
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      const double *fb = bcg + f * fstride;
+      const double vt = fb[cc], vb = fb[n1 + cc];
+      if (top) all = all && __ballot(vt == vt) == ~0ull;
+      if (bot) all = all && __ballot(vb == vb) == ~0ull;
+    }
+    if (all) kind = 6;
+  }
 
   const C *in_hi = static_cast<const C *>(a.in_hi) + G.off;
   const C *in_lo = P::kComp ? static_cast<const C *>(a.in_lo) + G.off : nullptr;
@@ -1414,8 +1442,15 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
     }
   };
 
-  auto run = [&](auto edge_tag) {
-    constexpr bool kE = decltype(edge_tag)::value;
+  auto run = [&](auto kind_tag) {
+    constexpr int kKind = decltype(kind_tag)::value;
+    constexpr bool kE = kKind != 0;
+    constexpr bool kRows = kKind == 3;                 // axis-0 edge rules
+    constexpr bool kRowDir = kKind == 6;               // axis-0 Dirichlet rows
+    constexpr bool kLo1 = kKind == 1 || kKind == 3;    // axis-1 edge rules
+    constexpr bool kHi1 = kKind == 2 || kKind == 3;
+    constexpr bool kNeedLo = kLo1 || kKind == 4;       // axis-1 edge values
+    constexpr bool kNeedHi = kHi1 || kKind == 5;
     C rn_lo[F], rn_hi[F], pe_lo[F], pe_hi[F];
     D rd_lo[F], rd_hi[F], pd_lo[F], pd_hi[F];
 #pragma unroll
@@ -1425,17 +1460,19 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
       if (kE) {
         const double *fb = bcg + f * fstride;
         const double *nb = fb + 2 * (n0 + n1);
-        if (ax0) {
+        if ((kRows || kRowDir) && ax0) {
           rd_lo[f] = static_cast<D>(fb[cc]);
           rd_hi[f] = static_cast<D>(fb[n1 + cc]);
+        }
+        if (kRows && ax0) {
           rn_lo[f] = static_cast<C>(nb[cc]);
           rn_hi[f] = static_cast<C>(nb[n1 + cc]);
         }
         const int rk = min(max(r_first + lane, 0), n0 - 1);
-        pd_lo[f] = static_cast<D>(fb[2 * n1 + rk]);
-        pd_hi[f] = static_cast<D>(fb[2 * n1 + n0 + rk]);
-        pe_lo[f] = static_cast<C>(nb[2 * n1 + rk]);
-        pe_hi[f] = static_cast<C>(nb[2 * n1 + n0 + rk]);
+        if (kNeedLo) pd_lo[f] = static_cast<D>(fb[2 * n1 + rk]);
+        if (kNeedHi) pd_hi[f] = static_cast<D>(fb[2 * n1 + n0 + rk]);
+        if (kLo1) pe_lo[f] = static_cast<C>(nb[2 * n1 + rk]);
+        if (kHi1) pe_hi[f] = static_cast<C>(nb[2 * n1 + n0 + rk]);
       }
     }
     // One step of row r = r_first + k from rows (yu, yc, yd) of the
@@ -1443,20 +1480,19 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
     auto step_row = [&](int k, const C *yu, const C *yc, const C *yc_lo, const C *yd,
                         const C *fill, C *nh, C *nl) {
       const int r = r_first + k;
-      const bool lo0 = kE && ax0 && r == 0, hi0 = kE && ax0 && r == n0 - 1;
-      const bool l1 = kE && lo1, h1 = kE && hi1;
+      const bool lo0 = kRows && ax0 && r == 0, hi0 = kRows && ax0 && r == n0 - 1;
+      const bool dlo0 = kRowDir && r == 0, dhi0 = kRowDir && r == n0 - 1;
+      const bool l1 = kLo1 && lo1, h1 = kHi1 && hi1;
       C e_lo[F], e_hi[F];
       D d_lo[F], d_hi[F];
 #pragma unroll
       for (int f = 0; f < F; ++f) {
         e_lo[f] = e_hi[f] = static_cast<C>(NAN);
         d_lo[f] = d_hi[f] = static_cast<D>(NAN);
-        if (kE) {
-          d_lo[f] = lane_value(pd_lo[f], k);
-          d_hi[f] = lane_value(pd_hi[f], k);
-          e_lo[f] = lane_value(pe_lo[f], k);
-          e_hi[f] = lane_value(pe_hi[f], k);
-        }
+        if (kNeedLo) d_lo[f] = lane_value(pd_lo[f], k);
+        if (kNeedHi) d_hi[f] = lane_value(pd_hi[f], k);
+        if (kLo1) e_lo[f] = lane_value(pe_lo[f], k);
+        if (kHi1) e_hi[f] = lane_value(pe_hi[f], k);
       }
       Quant<C> q[F];
 #define PML_FQN(F_)                                                                          \
@@ -1491,11 +1527,18 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
           state_add<P>(hv, lv, d);
         }
         if (kE) {
-          D dv = lo0 ? rd_lo[f] : (hi0 ? rd_hi[f] : static_cast<D>(NAN));
-          dv = (dv == dv) ? dv : (l1 ? d_lo[f] : (h1 ? d_hi[f] : static_cast<D>(NAN)));
+          D dv = (lo0 || dlo0) ? rd_lo[f] : ((hi0 || dhi0) ? rd_hi[f] : static_cast<D>(NAN));
+          dv = (dv == dv) ? dv
+                          : ((kNeedLo && lo1) ? d_lo[f]
+                                              : ((kNeedHi && hi1) ? d_hi[f]
+                                                                  : static_cast<D>(NAN)));
           C sh, sl;
           state_set<P>(sh, sl, static_cast<double>(dv));
-          const bool fix = dv == dv;
+          // the Dirichlet paths know their fixed lane / rows: no FP64 compare
+          const bool fix = kKind == 4   ? lo1
+                           : kKind == 5 ? hi1
+                           : kKind == 6 ? (r == 0 || r == n0 - 1)
+                                        : dv == dv;
           hv = fix ? sh : hv;
           lv = fix ? sl : lv;
         }
@@ -1545,8 +1588,13 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
       }
     }
   };
-  if (edge) run(std::true_type{});
-  else run(std::false_type{});
+  if (kind == 0) run(std::integral_constant<int, 0>{});
+  else if (kind == 6) run(std::integral_constant<int, 6>{});
+  else if (kind == 4) run(std::integral_constant<int, 4>{});
+  else if (kind == 5) run(std::integral_constant<int, 5>{});
+  else if (kind == 1) run(std::integral_constant<int, 1>{});
+  else if (kind == 2) run(std::integral_constant<int, 2>{});
+  else run(std::integral_constant<int, 3>{});
 }
 
 // Imposes the Dirichlet values on a state in place: one workgroup per
