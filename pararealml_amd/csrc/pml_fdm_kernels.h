@@ -377,6 +377,38 @@ __device__ __forceinline__ double wave_shl1<double>(double x, double fill) {
   return __builtin_bit_cast(double, r);
 }
 
+// The same shifts with bound_ctrl on and a zero `old`: the lane whose
+// source is outside the wavefront reads 0, and the shift can fold into the
+// instruction that uses it (no copy of a fill value first).
+template <typename C>
+__device__ __forceinline__ C wave_shr1_z(C x);
+template <typename C>
+__device__ __forceinline__ C wave_shl1_z(C x);
+template <>
+__device__ __forceinline__ float wave_shr1_z<float>(float x) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x138, 0xf, 0xf, true));
+}
+template <>
+__device__ __forceinline__ float wave_shl1_z<float>(float x) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), 0x130, 0xf, 0xf, true));
+}
+template <>
+__device__ __forceinline__ double wave_shr1_z<double>(double x) {
+  const long long zero = 0;
+  const long long r = __builtin_amdgcn_update_dpp(zero, __builtin_bit_cast(long long, x), 0x138,
+                                                  0xf, 0xf, true);
+  return __builtin_bit_cast(double, r);
+}
+template <>
+__device__ __forceinline__ double wave_shl1_z<double>(double x) {
+  const long long zero = 0;
+  const long long r = __builtin_amdgcn_update_dpp(zero, __builtin_bit_cast(long long, x), 0x130,
+                                                  0xf, 0xf, true);
+  return __builtin_bit_cast(double, r);
+}
+
 // Value of lane `l` (wave-uniform index) of x, as a scalar.
 template <typename T>
 __device__ __forceinline__ T lane_value(T x, int l) {
@@ -1187,7 +1219,10 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
       }
     }
     // dt-weighted right-hand side of row r = r_first + k from stage-input
-    // rows (yu, yc, yd); also returns the row's Dirichlet values.
+    // rows (yu, yc, yd); also returns the row's Dirichlet values.  `fill`
+    // feeds lanes 0 / 63 their outer neighbours (the halo columns); where
+    // those lanes' results are not valid anyway it is null and the shifts
+    // read 0 there.
     auto stage_k = [&](int k, const C *yu, const C *yc, const C *yd, const C *fill, C w, C *kk,
                        D *dv) {
       const int r = r_first + k;
@@ -1222,7 +1257,8 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
       Quant<C> q[F];
 #define PML_FQK(F_)                                                                          \
   if constexpr (F_ < F) {                                                                    \
-    const C yw = wave_shr1<C>(yc[F_], fill[F_]), ye = wave_shl1<C>(yc[F_], fill[F_]);        \
+    const C yw = fill ? wave_shr1<C>(yc[F_], fill[F_]) : wave_shr1_z<C>(yc[F_]);            \
+    const C ye = fill ? wave_shl1<C>(yc[F_], fill[F_]) : wave_shl1_z<C>(yc[F_]);            \
     field_quant_sel<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD>(q[F_], yc[F_], yw, ye, yu[F_],    \
                                                            yd[F_], l1, h1, lo0, hi0, ax0,    \
                                                            e_lo[F_], e_hi[F_], rn_lo[F_],    \
@@ -1268,12 +1304,12 @@ __global__ __launch_bounds__(256) void fdm_stream_rk(FdmArgs a) {
         if (s == 1) {
           // Stage 1 is valid on every lane when the halo columns are loaded.
           stage_k(k, w[t % NW], w[(t + 1) % NW], w[(t + 2) % NW],
-                  kHalo ? wh[(t + 1) % NW] : w[(t + 1) % NW], S == 2 ? hdt : dt, kk, dv);
+                  kHalo ? wh[(t + 1) % NW] : nullptr, S == 2 ? hdt : dt, kk, dv);
         } else {
           // Y_s rows r-1, r, r+1 were produced at iterations t-2, t-1, t.
           const C *yu = y[s - 2][(t + 1) % 3], *yc = y[s - 2][(t + 2) % 3],
                   *yd = y[s - 2][t % 3];
-          stage_k(k, yu, yc, yd, yc, dt, kk, dv);
+          stage_k(k, yu, yc, yd, nullptr, dt, kk, dv);
         }
         C *acc = kacc[k % NK];
 #pragma unroll
