@@ -940,7 +940,8 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
       Quant<C> q[F];
 #define PML_FQS(F_)                                                                          \
   if constexpr (F_ < F) {                                                                    \
-    const C yw = wave_shr1<C>(yc[F_], fill[F_]), ye = wave_shl1<C>(yc[F_], fill[F_]);        \
+    const C yw = fill ? wave_shr1<C>(yc[F_], fill[F_]) : wave_shr1_z<C>(yc[F_]);            \
+    const C ye = fill ? wave_shl1<C>(yc[F_], fill[F_]) : wave_shl1_z<C>(yc[F_]);            \
     field_quant_sel<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD>(q[F_], yc[F_], yw, ye, yu[F_],    \
                                                            yd[F_], l1, h1, lo0, hi0, ax0,    \
                                                            e_lo[F_], e_hi[F_], rn_lo[F_],    \
@@ -1013,7 +1014,8 @@ __global__ __launch_bounds__(256) void fdm_stream2(FdmArgs a) {
           C nh[F], nl[F];
           const int au = (t - 2) % NW, ac = (t - 1) % NW, ad = t % NW;
           // The fill (lanes 0 / 63) is never stored.
-          step_row(t - 1, ah[au], ah[ac], al[ac], ah[ad], ah[ac], nh, nl);
+          // lanes 0 / 63 of the second step are not stored: no fill
+          step_row(t - 1, ah[au], ah[ac], al[ac], ah[ad], nullptr, nh, nl);
           store_row(r - 1, nh, nl);
         }
       }
@@ -1533,7 +1535,8 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
       Quant<C> q[F];
 #define PML_FQN(F_)                                                                          \
   if constexpr (F_ < F) {                                                                    \
-    const C yw = wave_shr1<C>(yc[F_], fill[F_]), ye = wave_shl1<C>(yc[F_], fill[F_]);        \
+    const C yw = fill ? wave_shr1<C>(yc[F_], fill[F_]) : wave_shr1_z<C>(yc[F_]);            \
+    const C ye = fill ? wave_shl1<C>(yc[F_], fill[F_]) : wave_shl1_z<C>(yc[F_]);            \
     field_quant_sel<C, Need<Eq, F_>::kL, Need<Eq, F_>::kD>(q[F_], yc[F_], yw, ye, yu[F_],    \
                                                            yd[F_], l1, h1, lo0, hi0, ax0,    \
                                                            e_lo[F_], e_hi[F_], rn_lo[F_],    \
@@ -1609,8 +1612,9 @@ __global__ __launch_bounds__(256) void fdm_stream_n(FdmArgs a) {
         } else {
           // Step s-1 rows r-1, r, r+1 were produced at iterations t-2, t-1, t.
           const int su = (t + 1) % 3, sc = (t + 2) % 3, sd = t % 3;
-          step_row(k, yh[s - 2][su], yh[s - 2][sc], yl[s - 2][sc], yh[s - 2][sd],
-                   yh[s - 2][sc], nh, nl);
+          // lanes 0 / 63 are not valid after the first step: no fill
+          step_row(k, yh[s - 2][su], yh[s - 2][sc], yl[s - 2][sc], yh[s - 2][sd], nullptr,
+                   nh, nl);
         }
         if (s < NSTEP) {
 #pragma unroll
