@@ -58,6 +58,16 @@
  *   pml_graph_cache_clear  <- no counterpart (the reference has no device
  *                             caches): releases the step-loop graphs and
  *                             work-unit tables pml_fdm_solve keeps
+ *   pml_interp_regular / pml_snapshots_time_inner
+ *                          <- the scipy.interpolate.interpn call of
+ *                             Solution.y and of discrete_y's orientation
+ *                             change (solution.py:114-180)
+ *   pml_snapshot_diff      <- the matched-time subtraction of Solution.diff
+ *                             (solution.py:182-258)
+ *   pml_supervised_rows    <- the row layout of
+ *                             SupervisedMLOperator._generate_data
+ *                             (operators/ml/supervised/
+ *                             supervised_ml_operator.py:362-379, 440-499)
  */
 #ifndef PML_H
 #define PML_H
