@@ -16,17 +16,24 @@ namespace pml {
 static const std::vector<Variant> *find_variants(int eq, int integ, int prec, int ndim,
                                                  int *n_fields) {
   using Table = const std::vector<Variant> *(*)(int, int, int, int *);
-  Table a = nullptr, b = nullptr, c = nullptr;
+  static const Table tables[3][5] = {
+      {fdm_table_f32_a, fdm_table_f32_b, fdm_table_f32_c, fdm_table_f32_d, fdm_table_f32_e},
+      {fdm_table_f64_a, fdm_table_f64_b, fdm_table_f64_c, fdm_table_f64_d, fdm_table_f64_e},
+      {fdm_table_f32c_a, fdm_table_f32c_b, fdm_table_f32c_c, fdm_table_f32c_d,
+       fdm_table_f32c_e},
+  };
+  int row;
   switch (prec) {
-    case PML_F32: a = fdm_table_f32_a; b = fdm_table_f32_b; c = fdm_table_f32_c; break;
-    case PML_F64: a = fdm_table_f64_a; b = fdm_table_f64_b; c = fdm_table_f64_c; break;
-    case PML_F32C: a = fdm_table_f32c_a; b = fdm_table_f32c_b; c = fdm_table_f32c_c; break;
+    case PML_F32: row = 0; break;
+    case PML_F64: row = 1; break;
+    case PML_F32C: row = 2; break;
     default: return nullptr;
   }
-  const std::vector<Variant> *v = a(eq, integ, ndim, n_fields);
-  if (!v) v = b(eq, integ, ndim, n_fields);
-  if (!v) v = c(eq, integ, ndim, n_fields);
-  return v;
+  for (Table t : tables[row]) {
+    const std::vector<Variant> *v = t(eq, integ, ndim, n_fields);
+    if (v) return v;
+  }
+  return nullptr;
 }
 
 // Device copies of work-unit and grid tables.  They are derived from the

@@ -1,5 +1,4 @@
-// Fused stepping kernel instantiations: PrecF32C, diffusion / Cahn-Hilliard /
-// convection-diffusion.
+// Fused stepping kernel instantiations: PrecF32C, diffusion.
 #include "pml_fdm_kernels.h"
 
 namespace pml {

@@ -1,5 +1,4 @@
-// Fused stepping kernel instantiations: PrecF64, diffusion / Cahn-Hilliard /
-// convection-diffusion.
+// Fused stepping kernel instantiations: PrecF64, diffusion.
 #include "pml_fdm_kernels.h"
 
 namespace pml {

@@ -1,5 +1,4 @@
-// Fused stepping kernel instantiations: PrecF64, shallow water / wave /
-// Burgers.
+// Fused stepping kernel instantiations: PrecF64, shallow water / wave.
 #include "pml_fdm_kernels.h"
 
 namespace pml {

@@ -2004,16 +2004,29 @@ PML_DECLARE_TABLE(fdm_table_f32c_b)
 PML_DECLARE_TABLE(fdm_table_f32_c)
 PML_DECLARE_TABLE(fdm_table_f64_c)
 PML_DECLARE_TABLE(fdm_table_f32c_c)
+PML_DECLARE_TABLE(fdm_table_f32_d)
+PML_DECLARE_TABLE(fdm_table_f64_d)
+PML_DECLARE_TABLE(fdm_table_f32c_d)
+PML_DECLARE_TABLE(fdm_table_f32_e)
+PML_DECLARE_TABLE(fdm_table_f64_e)
+PML_DECLARE_TABLE(fdm_table_f32c_e)
 #undef PML_DECLARE_TABLE
 
-// Group a: diffusion, Cahn-Hilliard, convection-diffusion.
-// Group b: shallow water, wave, Burgers.
-// Group c: Navier-Stokes (2-D).
+// Group a: diffusion.  Group d: Cahn-Hilliard, convection-diffusion.
+// Group b: shallow water, wave.  Group e: Burgers.  Group c: Navier-Stokes
+// (2-D).  Five groups per precision keep the translation units small enough
+// to compile in parallel.
 #define PML_DEFINE_TABLE_A(NAME, PR)                                                   \
   const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields) {       \
     (void)ndim;                                                                        \
+    if (eq != PML_EQ_DIFFUSION) return nullptr;                                        \
+    *n_fields = 1;                                                                     \
+    return variants_by_int<EqDiffusion, PR>(integ);                                    \
+  }
+#define PML_DEFINE_TABLE_D(NAME, PR)                                                   \
+  const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields) {       \
+    (void)ndim;                                                                        \
     switch (eq) {                                                                      \
-      case PML_EQ_DIFFUSION: *n_fields = 1; return variants_by_int<EqDiffusion, PR>(integ); \
       case PML_EQ_CAHN_HILLIARD:                                                       \
         *n_fields = 2;                                                                 \
         return variants_by_int<EqCahnHilliard, PR>(integ);                             \
@@ -2030,15 +2043,18 @@ PML_DECLARE_TABLE(fdm_table_f32c_c)
         *n_fields = 3;                                                                 \
         return ndim == 2 ? variants_by_int<EqShallowWater, PR>(integ) : nullptr;       \
       case PML_EQ_WAVE: *n_fields = 2; return variants_by_int<EqWave, PR>(integ);      \
-      case PML_EQ_BURGERS:                                                             \
-        if (ndim == 1) {                                                               \
-          *n_fields = 1;                                                               \
-          return variants_by_int<EqBurgers1, PR>(integ);                               \
-        }                                                                              \
-        *n_fields = 2;                                                                 \
-        return variants_by_int<EqBurgers2, PR>(integ);                                 \
       default: return nullptr;                                                         \
     }                                                                                  \
+  }
+#define PML_DEFINE_TABLE_E(NAME, PR)                                                   \
+  const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields) {       \
+    if (eq != PML_EQ_BURGERS) return nullptr;                                          \
+    if (ndim == 1) {                                                                   \
+      *n_fields = 1;                                                                   \
+      return variants_by_int<EqBurgers1, PR>(integ);                                   \
+    }                                                                                  \
+    *n_fields = 2;                                                                     \
+    return variants_by_int<EqBurgers2, PR>(integ);                                     \
   }
 #define PML_DEFINE_TABLE_C(NAME, PR)                                                   \
   const std::vector<Variant> *NAME(int eq, int integ, int ndim, int *n_fields) {       \
